@@ -1,0 +1,242 @@
+/* TAL-SH device-unified C API, MI355X implementation (libtalsh_amd.so).
+ *
+ * This header is the drop-in boundary: the entry points below carry the
+ * reference's names, argument meanings, return codes and struct layouts
+ * (talsh.h:25-597 of TAL-SH), so that code written against TAL-SH's C API
+ * (test.cpp, talshxx.hpp, talshf.F90's bind(c) interfaces) links against this
+ * library unchanged.  Each declaration cites the reference declaration it
+ * replaces.  Execution goes to HIP kernels on gfx950; there is no CPU
+ * execution path: an operation scheduled on DEV_HOST returns
+ * TALSH_NOT_AVAILABLE.  Host images are still supported as data sources and
+ * destinations (host<->device copies follow the coherence letters).
+ *
+ * Semantic choices where the reference is silent or inconsistent:
+ *  - talshTensorNorm2 (documented in TALSH_manual.txt:893-913, never
+ *    declared in the reference talsh.h) returns the SQUARED Frobenius norm,
+ *    as the reference's CPU kernels and tests do (tensor_algebra_cpu.F90:
+ *    2032-2129, test.cpp:1073), accumulated in FP64 for every data kind.
+ *  - GPUs may be addressed with either the kind-1 numbering (flat ids 1..8,
+ *    the numbering the reference's own HIP build uses) or DEV_AMD_GPU
+ *    (flat ids 17..24); both name the same devices.
+ */
+#ifndef TALSH_H_
+#define TALSH_H_
+
+#include "tensor_algebra.h"
+
+#ifdef __cplusplus
+#define TALSH_DEFAULT(x) = x
+#else
+#define TALSH_DEFAULT(x)
+#endif
+
+#define TALSH_MAX_DEV_PRESENT 16
+#define TALSH_NO_HOST_BUFFER 16777216
+#define TALSH_GFLOP_THRESH_GPU 4
+
+/* error codes (talsh.h:25-41) */
+#define TALSH_SUCCESS 0
+#define TALSH_FAILURE -666
+#define TALSH_NOT_AVAILABLE -888
+#define TALSH_NOT_IMPLEMENTED -999
+#define TALSH_NOT_INITIALIZED 1000000
+#define TALSH_ALREADY_INITIALIZED 1000001
+#define TALSH_INVALID_ARGS 1000002
+#define TALSH_INTEGER_OVERFLOW 1000003
+#define TALSH_OBJECT_NOT_EMPTY 1000004
+#define TALSH_OBJECT_IS_EMPTY 1000005
+#define TALSH_IN_PROGRESS 1000006
+#define TALSH_NOT_ALLOWED 1000007
+#define TALSH_LIMIT_EXCEEDED 1000008
+#define TALSH_NOT_FOUND 1000009
+#define TALSH_OBJECT_BROKEN 1000010
+#define TALSH_INVALID_REQUEST 1000011
+
+/* task status (talsh.h:43-50) */
+#define TALSH_TASK_ERROR 1999999
+#define TALSH_TASK_EMPTY 2000000
+#define TALSH_TASK_SCHEDULED 2000001
+#define TALSH_TASK_STARTED 2000002
+#define TALSH_TASK_INPUT_READY 2000003
+#define TALSH_TASK_OUTPUT_READY 2000004
+#define TALSH_TASK_COMPLETED 2000005
+
+/* operation kinds used by this library (talsh.h:53-70) */
+#define TALSH_TENSOR_NOOP -1
+#define TALSH_TENSOR_INIT 68
+#define TALSH_TENSOR_NORM1 69
+#define TALSH_TENSOR_NORM2 70
+#define TALSH_TENSOR_COPY 77
+#define TALSH_TENSOR_PERMUTE 78
+#define TALSH_TENSOR_SCALE 79
+#define TALSH_TENSOR_ADD 80
+#define TALSH_TENSOR_CONTRACT 82
+
+/* Dense tensor with up to TALSH_MAX_DEV_PRESENT body images (talsh.h:87-94, 40 B). */
+typedef struct {
+  talsh_tens_shape_t *shape_p;
+  talsh_dev_rsc_t *dev_rsc;
+  int *data_kind;
+  int *avail;
+  int dev_rsc_len;
+  int ndev;
+} talsh_tens_t;
+
+/* Tensor operation argument (talsh.h:104-107, 16 B). */
+typedef struct {
+  talsh_tens_t *tens_p;
+  int source_image;
+} talshTensArg_t;
+
+/* Task handle (talsh.h:110-121, 120 B); task_p points at a HIP task object. */
+typedef struct {
+  void *task_p;
+  int task_error;
+  int dev_kind;
+  int data_kind;
+  int coherence;
+  int num_args;
+  talshTensArg_t tens_args[MAX_TENSOR_OPERANDS];
+  double data_vol;
+  double flops;
+  double exec_time;
+} talsh_task_t;
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- helpers and control (talsh.h:149-229) ---- */
+int talshValidDataKind(int datk, int *datk_size);
+int talshInit(size_t *host_buf_size, int *host_arg_max, int ngpus, int gpu_list[], int nmics, int mic_list[],
+              int namds, int amd_list[]);
+int talshShutdown(void);
+int talshDeviceCount(int dev_kind, int *dev_count);
+int talshFlatDevId(int dev_kind, int dev_num);
+int talshKindDevId(int dev_id, int *dev_kind);
+int talshDeviceState(int dev_num, int dev_kind TALSH_DEFAULT(DEV_NULL));
+int talshDeviceState_(int dev_num, int dev_kind);
+int talshDeviceBusyLeast(int dev_kind TALSH_DEFAULT(DEV_NULL));
+int talshDeviceBusyLeast_(int dev_kind);
+size_t talshDeviceMemorySize(int dev_num, int dev_kind TALSH_DEFAULT(DEV_NULL));
+size_t talshDeviceMemorySize_(int dev_num, int dev_kind);
+size_t talshDeviceBufferSize(int dev_num, int dev_kind TALSH_DEFAULT(DEV_NULL));
+size_t talshDeviceBufferSize_(int dev_num, int dev_kind);
+size_t talshDeviceTensorSize(int dev_num, int dev_kind TALSH_DEFAULT(DEV_NULL));
+size_t talshDeviceTensorSize_(int dev_num, int dev_kind);
+size_t talshDeviceBufferFreeSize(int dev_num, int dev_kind TALSH_DEFAULT(DEV_NULL));
+size_t talshDeviceBufferFreeSize_(int dev_num, int dev_kind);
+double talshDeviceGetFlops(int dev_kind TALSH_DEFAULT(DEV_DEFAULT), int dev_id TALSH_DEFAULT(DEV_DEFAULT));
+int talshStats(int dev_id TALSH_DEFAULT(-1), int dev_kind TALSH_DEFAULT(DEV_NULL));
+int talshStats_(int dev_id, int dev_kind);
+
+/* ---- tensor blocks (talsh.h:232-311) ---- */
+int talshTensorCreate(talsh_tens_t **tens_block);
+int talshTensorClean(talsh_tens_t *tens_block);
+int talshTensorIsEmpty(const talsh_tens_t *tens_block);
+int talshTensorConstruct(talsh_tens_t *tens_block, int data_kind, int tens_rank, const int tens_dims[],
+                         int dev_id TALSH_DEFAULT(0), void *ext_mem TALSH_DEFAULT(NULL),
+                         int in_hab TALSH_DEFAULT(-1), talsh_tens_init_i init_method TALSH_DEFAULT(NULL),
+                         double init_val_real TALSH_DEFAULT(0.0), double init_val_imag TALSH_DEFAULT(0.0));
+int talshTensorConstruct_(talsh_tens_t *tens_block, int data_kind, int tens_rank, const int tens_dims[], int dev_id,
+                          void *ext_mem, int in_hab, talsh_tens_init_i init_method, double init_val_real,
+                          double init_val_imag);
+int talshTensorImportData(talsh_tens_t *tens_block, int data_kind, const void *ext_data);
+int talshTensorDestruct(talsh_tens_t *tens_block);
+int talshTensorDestroy(talsh_tens_t *tens_block);
+int talshTensorRank(const talsh_tens_t *tens_block);
+size_t talshTensorVolume(const talsh_tens_t *tens_block);
+size_t talshTensorSizeAllImages(const talsh_tens_t *tens_block, int *num_images);
+const int *talshTensorDimExtents(const talsh_tens_t *tens_block, int *rank);
+int talshTensorShape(const talsh_tens_t *tens_block, talsh_tens_shape_t *tens_shape);
+int talshTensorDataKind(const talsh_tens_t *tens_block, int *num_images, int *data_kinds);
+int talshTensorReshape(talsh_tens_t *tens_block, int tens_rank, const int tens_dims[]);
+int talshTensorInUse(const talsh_tens_t *tens_block);
+int talshTensorPresence(const talsh_tens_t *tens_block, int *ncopies, int copies[], int data_kinds[],
+                        int dev_kind TALSH_DEFAULT(DEV_NULL), int dev_id TALSH_DEFAULT(-1));
+int talshTensorPresence_(const talsh_tens_t *tens_block, int *ncopies, int copies[], int data_kinds[],
+                         int dev_kind, int dev_id);
+int talshTensorGetBodyAccess(talsh_tens_t *tens_block, void **body_p, int data_kind, int dev_id,
+                             int dev_kind TALSH_DEFAULT(DEV_NULL));
+int talshTensorGetBodyAccess_(talsh_tens_t *tens_block, void **body_p, int data_kind, int dev_id, int dev_kind);
+int talshTensorGetBodyAccessConst(const talsh_tens_t *tens_block, const void **body_p, int data_kind, int dev_id,
+                                  int dev_kind TALSH_DEFAULT(DEV_NULL));
+int talshTensorGetScalar(talsh_tens_t *tens_block, double *scalar_real, double *scalar_imag);
+void talshTensorPrint(const talsh_tens_t *tens_block);
+void talshTensorPrintInfo(const talsh_tens_t *tens_block);
+
+/* ---- tasks (talsh.h:332-372) ---- */
+int talshTaskCreate(talsh_task_t **talsh_task);
+int talshTaskClean(talsh_task_t *talsh_task);
+int talshTaskIsEmpty(const talsh_task_t *talsh_task);
+int talshTaskDestruct(talsh_task_t *talsh_task);
+int talshTaskDestroy(talsh_task_t *talsh_task);
+int talshTaskDevId(talsh_task_t *talsh_task, int *dev_kind TALSH_DEFAULT(NULL));
+int talshTaskDevId_(talsh_task_t *talsh_task, int *dev_kind);
+int talshTaskArgCoherence(const talsh_task_t *talsh_task);
+const talshTensArg_t *talshTaskTensArgs(const talsh_task_t *talsh_task, int *num_args);
+int talshTaskStatus(talsh_task_t *talsh_task);
+int talshTaskComplete(talsh_task_t *talsh_task, int *stats, int *ierr);
+int talshTaskWait(talsh_task_t *talsh_task, int *stats);
+int talshTasksWait(int ntasks, talsh_task_t talsh_tasks[], int stats[]);
+int talshTaskTime(talsh_task_t *talsh_task, double *total, double *comput TALSH_DEFAULT(NULL),
+                  double *input TALSH_DEFAULT(NULL), double *output TALSH_DEFAULT(NULL),
+                  double *mmul TALSH_DEFAULT(NULL));
+int talshTaskTime_(talsh_task_t *talsh_task, double *total, double *comput, double *input, double *output,
+                   double *mmul);
+void talshTaskPrint(const talsh_task_t *talsh_task);
+
+/* ---- placement (talsh.h:437-453) ---- */
+int talshTensorPlace(talsh_tens_t *tens, int dev_id, int dev_kind TALSH_DEFAULT(DEV_NULL),
+                     void *dev_mem TALSH_DEFAULT(NULL), int copy_ctrl TALSH_DEFAULT(COPY_M),
+                     talsh_task_t *talsh_task TALSH_DEFAULT(NULL));
+int talshTensorPlace_(talsh_tens_t *tens, int dev_id, int dev_kind, void *dev_mem, int copy_ctrl,
+                      talsh_task_t *talsh_task);
+int talshTensorDiscard(talsh_tens_t *tens, int dev_id, int dev_kind TALSH_DEFAULT(DEV_NULL));
+int talshTensorDiscard_(talsh_tens_t *tens, int dev_id, int dev_kind);
+int talshTensorDiscardOther(talsh_tens_t *tens, int dev_id, int dev_kind TALSH_DEFAULT(DEV_NULL));
+int talshTensorDiscardOther_(talsh_tens_t *tens, int dev_id, int dev_kind);
+
+/* ---- tensor operations (talsh.h:455-530) ---- */
+int talshTensorInit(talsh_tens_t *dtens, double val_real, double val_imag, int dev_id TALSH_DEFAULT(DEV_DEFAULT),
+                    int dev_kind TALSH_DEFAULT(DEV_DEFAULT), int copy_ctrl TALSH_DEFAULT(COPY_M),
+                    talsh_task_t *talsh_task TALSH_DEFAULT(NULL));
+int talshTensorInit_(talsh_tens_t *dtens, double val_real, double val_imag, int dev_id, int dev_kind,
+                     int copy_ctrl, talsh_task_t *talsh_task);
+int talshTensorScale(talsh_tens_t *dtens, double val_real, double val_imag, int dev_id TALSH_DEFAULT(DEV_DEFAULT),
+                     int dev_kind TALSH_DEFAULT(DEV_DEFAULT), int copy_ctrl TALSH_DEFAULT(COPY_M),
+                     talsh_task_t *talsh_task TALSH_DEFAULT(NULL));
+int talshTensorScale_(talsh_tens_t *dtens, double val_real, double val_imag, int dev_id, int dev_kind,
+                      int copy_ctrl, talsh_task_t *talsh_task);
+int talshTensorCopy(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens,
+                    int dev_id TALSH_DEFAULT(DEV_DEFAULT), int dev_kind TALSH_DEFAULT(DEV_DEFAULT),
+                    int copy_ctrl TALSH_DEFAULT(COPY_MT), talsh_task_t *talsh_task TALSH_DEFAULT(NULL));
+int talshTensorCopy_(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, int dev_id, int dev_kind,
+                     int copy_ctrl, talsh_task_t *talsh_task);
+int talshTensorAdd(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, double scale_real TALSH_DEFAULT(1.0),
+                   double scale_imag TALSH_DEFAULT(0.0), int dev_id TALSH_DEFAULT(DEV_DEFAULT),
+                   int dev_kind TALSH_DEFAULT(DEV_DEFAULT), int copy_ctrl TALSH_DEFAULT(COPY_MT),
+                   talsh_task_t *talsh_task TALSH_DEFAULT(NULL));
+int talshTensorAdd_(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, double scale_real,
+                    double scale_imag, int dev_id, int dev_kind, int copy_ctrl, talsh_task_t *talsh_task);
+int talshTensorContract(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, talsh_tens_t *rtens,
+                        double scale_real TALSH_DEFAULT(1.0), double scale_imag TALSH_DEFAULT(0.0),
+                        int dev_id TALSH_DEFAULT(DEV_DEFAULT), int dev_kind TALSH_DEFAULT(DEV_DEFAULT),
+                        int copy_ctrl TALSH_DEFAULT(COPY_MTT), int accumulative TALSH_DEFAULT(YEP),
+                        talsh_task_t *talsh_task TALSH_DEFAULT(NULL));
+int talshTensorContract_(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, talsh_tens_t *rtens,
+                         double scale_real, double scale_imag, int dev_id, int dev_kind, int copy_ctrl,
+                         int accumulative, talsh_task_t *talsh_task);
+/* Squared 2-norm into a rank-0 tensor dtens (TALSH_manual.txt:893-913; see the header comment). */
+int talshTensorNorm2(talsh_tens_t *dtens, talsh_tens_t *ltens, int dev_id TALSH_DEFAULT(DEV_DEFAULT),
+                     int dev_kind TALSH_DEFAULT(DEV_DEFAULT), int copy_ctrl TALSH_DEFAULT(COPY_TT),
+                     talsh_task_t *talsh_task TALSH_DEFAULT(NULL));
+
+/* ---- debugging (talsh.h:593) ---- */
+double talshTensorImageNorm1_cpu(const talsh_tens_t *talsh_tens);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TALSH_H_ */

@@ -1,0 +1,58 @@
+/* MI355X extensions of libtalsh_amd.
+ *
+ * These entry points have no counterpart in the reference C API; they expose
+ * configuration and the batched/serving forms of the contraction path that
+ * only this build provides.  Everything here is optional: a caller that only
+ * uses talsh.h gets the reference behaviour with the defaults below.
+ */
+#ifndef TALSH_AMD_H_
+#define TALSH_AMD_H_
+
+#include <stddef.h>
+
+#include "talsh.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Fast math (talsh.h:167-174 of the reference): on this build "fast math"
+ * selects the 3-term split-BF16 MFMA GEMM for R4/C4 contractions (FP32
+ * accumulation); disabled, R4/C4 use the FP32 MFMA GEMM.  R8/C8 always use
+ * the FP64 MFMA GEMM. */
+int talshEnableFastMath(int dev_kind, int dev_id);
+int talshDisableFastMath(int dev_kind, int dev_id);
+int talshQueryFastMath(int dev_kind, int dev_id);
+
+/* Size in bytes of the per-GPU buffer pool taken at talshInit (default: 80 %
+ * of the free device memory).  Must be called before talshInit. */
+int talshAmdSetDeviceBufferSize(size_t bytes);
+
+/* Number of HIP streams per GPU used for tasks (default 4, at most 32).
+ * Must be called before talshInit. */
+int talshAmdSetNumStreams(int n);
+
+/* Blocks until all work queued on GPU `gpu` (kind-specific number) is done. */
+int talshAmdDeviceSynchronize(int gpu);
+
+/* Symbolic -> digital contraction pattern (the reference's C-callable
+ * converter, talshf.F90:619-667, declared in talshc.cpp:109): 0 on success. */
+int talsh_get_contr_ptrn_str2dig(const char *c_str, int *dig_ptrn, int *drank, int *lrank, int *rrank,
+                                 int *conj_bits);
+
+/* The TTGT plan talshTensorContract would use for a pattern and shapes:
+ * mnk = {M, N, K}; flags = {A is R, A transposed, B transposed, D transposed};
+ * ops = GEMM forms of A and B ('N'/'T'); perms = O2N maps of the transposed
+ * operands (A, then B, then D->D'; -1 where not transposed); moved = elements
+ * copied by the transposes.  Host-only query, no device work. */
+int talshAmdPlanContraction(const char *pattern, const int *ddims, const int *ldims, const int *rdims,
+                            int accumulative, long long *mnk, int *flags, char *ops, int *perms, double *moved);
+
+/* Copies a short build description into buf. */
+int talshAmdBuildInfo(char *buf, int len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TALSH_AMD_H_ */

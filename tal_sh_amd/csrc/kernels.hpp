@@ -1,0 +1,41 @@
+// Device-kernel launchers of libtalsh_amd (gfx950).  All launchers are
+// asynchronous on the given stream and validate their shapes on the host
+// before launching; a nonzero return means nothing was launched.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace tal {
+
+enum PermOp { PERM_COPY = 0, PERM_ADD = 1 };
+
+// dst = perm(src) (PERM_COPY) or dst += alpha * perm(src) (PERM_ADD) for a
+// dlf tensor of `rank` dims with extents `dims` (input order) and a 0-based
+// O2N permutation `o2n`.  Complex kinds may conjugate src.  Bit-exact.
+int launch_permute(int dk, const void *src, void *dst, int rank, const int *dims, const int *o2n, int op,
+                   double ar, double ai, int conj, hipStream_t s);
+
+// d = val (all elements)
+int launch_init(int dk, void *d, size_t vol, double vr, double vi, hipStream_t s);
+// d = d * alpha
+int launch_scale(int dk, void *d, size_t vol, double ar, double ai, hipStream_t s);
+// *out (data kind dk, rank-0 body) = sum |x|^2 over src, FP64 two-stage
+// deterministic reduction; `partials` is device scratch of norm2_scratch_elems() doubles.
+size_t norm2_scratch_elems();
+int launch_norm2(int dk, const void *src, size_t vol, double *partials, void *out, hipStream_t s);
+
+// GEMM precision modes for real single precision.
+enum R4Mode { R4_FP32 = 0, R4_SPLIT_BF16 = 1 };
+
+// C(MxN, col-major, ldc) = alpha * op(A) * op(B) + beta * C, beta in {0,1}.
+// ta: 'N' A is MxK (lda>=M); 'T' A is KxM (lda>=K).
+// tb: 'N' B is KxN (ldb>=K); 'T' B is NxK (ldb>=N).
+// conja/conjb conjugate complex operands.  Complex kinds use interleaved (re,im) storage.
+int launch_gemm(int dk, int r4mode, char ta, char tb, int conja, int conjb, int64_t M, int64_t N, int64_t K,
+                double ar, double ai, const void *A, int64_t lda, const void *B, int64_t ldb, int beta_one, void *C,
+                int64_t ldc, hipStream_t s);
+
+}  // namespace tal

@@ -1,0 +1,789 @@
+// Tensor operations of the C API: Init, Scale, Copy, Add, Contract, Norm2,
+// Place, Discard.  Each call validates and plans on the host, acquires all
+// device buffers up front (rolling back on TRY_LATER / DEVICE_UNABLE, which
+// leave the task empty, as the reference does), then enqueues transfers and
+// kernels on one HIP stream of the execution GPU and returns; completion and
+// coherence bookkeeping happen when the task is observed complete
+// (talshTaskComplete / talshTaskWait), or before returning for a NULL task.
+//
+// Reference counterparts: talshTensorInit/Scale (talshc.cpp:3562-4529),
+// talshTensorCopy (:4530-4787), talshTensorAdd (:4795-5066),
+// talshTensorContract (:5073-5385) and their GPU back ends
+// gpu_tensor_block_{init,scale,copy,add,contract_dlf,place}
+// (tensor_algebra_gpu_nvidia.cu:4089-7188); coherence letters as in
+// DOC/TALSH_manual.txt:719-750 (D discard, M move, T temporary, K keep).
+#include <cstring>
+
+#include "plan.hpp"
+#include "runtime.hpp"
+#include "talsh_amd.h"
+
+namespace tal {
+
+namespace {
+
+int coh_letter(int coh_ctrl, int nargs, int arg) { return (coh_ctrl >> (2 * (nargs - 1 - arg))) & 3; }
+
+struct Operand {
+  talsh_tens_t *t = nullptr;
+  int coh = COPY_T;
+  bool output = false;
+  bool read = true;       // the operation reads the operand's value
+  int img = -1;           // source image index at scheduling time
+  void *src_p = nullptr;  // source image body
+  int src_dev = -1;       // canonical flat id of the source image
+  bool resident = false;  // source image is on the execution GPU
+  void *dptr = nullptr;   // body used by the kernels
+  int dhandle = -1;       // pool handle when a buffer was allocated on the GPU
+  int dk = NO_TYPE;
+  size_t bytes = 0;
+};
+
+class Op {
+ public:
+  Op(int g, talsh_task_t *user_task) : g_(g), user_(user_task) {}
+  ~Op() {
+    if (!committed_) rollback();
+  }
+
+  int add(talsh_tens_t *t, int coh, bool output, bool read) {
+    if (!t || talshTensorIsEmpty(t) == YEP || tensor_healthy(t) != YEP) return TALSH_INVALID_ARGS;
+    Operand o;
+    o.t = t;
+    o.coh = coh;
+    o.output = output;
+    o.read = read;
+    const int exec = flat_gpu(g_);
+    int pick = -1;
+    for (int i = 0; i < t->ndev; ++i)
+      if (t->avail[i] == YEP && t->dev_rsc[i].dev_id == exec) { pick = i; break; }
+    if (pick < 0)
+      for (int i = 0; i < t->ndev; ++i)
+        if (t->avail[i] == YEP && t->dev_rsc[i].dev_id == DEV_HOST) { pick = i; break; }
+    if (pick < 0)
+      for (int i = 0; i < t->ndev; ++i)
+        if (t->avail[i] == YEP && gpu_index(t->dev_rsc[i].dev_id) >= 0) { pick = i; break; }
+    if (pick < 0) return TALSH_NOT_AVAILABLE;
+    o.img = pick;
+    o.src_p = t->dev_rsc[pick].gmem_p;
+    o.src_dev = t->dev_rsc[pick].dev_id;
+    o.resident = (o.src_dev == exec);
+    o.dk = t->data_kind[pick];
+    o.bytes = shape_volume(t->shape_p) * (size_t)dk_size(o.dk);
+    ops_.push_back(o);
+    return TALSH_SUCCESS;
+  }
+
+  Operand &operand(int i) { return ops_[i]; }
+
+  // Device buffers: one per non-resident operand plus the requested temporaries.
+  int acquire(const std::vector<size_t> &temp_bytes) {
+    for (auto &o : ops_) {
+      if (o.resident) {
+        o.dptr = o.src_p;
+        continue;
+      }
+      const int e = dev_alloc(g_, o.bytes, &o.dptr, &o.dhandle);
+      if (e) return e;
+    }
+    for (size_t b : temp_bytes) {
+      void *p = nullptr;
+      int h = -1;
+      const int e = dev_alloc(g_, b, &p, &h);
+      if (e) return e;
+      temps_.push_back(p);
+      temp_handles_.push_back(h);
+    }
+    return TALSH_SUCCESS;
+  }
+  void *temp(int i) const { return temps_[i]; }
+
+  // Starts the task: stream, ordering against earlier work, input transfers.
+  int begin(int data_kind, int coh_ctrl) {
+    Device &d = rt().gpus[g_];
+    (void)hipSetDevice(d.hip_id);
+    task_ = task_new(g_);
+    s_ = task_->stream;
+    (void)hipEventRecord(task_->ev_start->ev, s_);
+    for (auto &o : ops_) {
+      if (o.resident) {
+        if (o.output) hazard_wait_write(o.dptr, s_);
+        else hazard_wait_read(o.dptr, s_);
+      } else {
+        hazard_wait_read(o.src_p, s_);
+        if (o.read) {
+          if (hipMemcpyAsync(o.dptr, o.src_p, o.bytes, hipMemcpyDefault, s_) != hipSuccess) return TALSH_FAILURE;
+          task_->bytes += (double)o.bytes;
+          d.traffic_in += (double)o.bytes;
+        }
+      }
+    }
+    for (auto p : temps_) hazard_wait_write(p, s_);
+    (void)hipEventRecord(task_->ev_in->ev, s_);
+    data_kind_ = data_kind;
+    coh_ctrl_ = coh_ctrl;
+    return TALSH_SUCCESS;
+  }
+
+  hipStream_t stream() const { return s_; }
+  Task *task() { return task_; }
+
+  void mark_gemm(bool start) {
+    EvRef &e = start ? task_->ev_mm0 : task_->ev_mm1;
+    e = new_event(g_);
+    (void)hipEventRecord(e->ev, s_);
+  }
+
+  // Ends the task: output transfers, completion event, coherence bookkeeping.
+  int end(double flops) {
+    Device &d = rt().gpus[g_];
+    (void)hipEventRecord(task_->ev_comp->ev, s_);
+    for (auto &o : ops_) {
+      if (o.output && !o.resident && (o.coh == COPY_T || o.coh == COPY_K)) {
+        hazard_wait_write(o.src_p, s_);
+        if (hipMemcpyAsync(o.src_p, o.dptr, o.bytes, hipMemcpyDefault, s_) != hipSuccess) return TALSH_FAILURE;
+        task_->bytes += (double)o.bytes;
+        d.traffic_out += (double)o.bytes;
+      }
+    }
+    (void)hipEventRecord(task_->ev_done->ev, s_);
+    if (hipGetLastError() != hipSuccess) return TALSH_FAILURE;
+    // ordering state of every buffer touched
+    for (auto &o : ops_) {
+      if (o.output) {
+        hazard_set_writer(o.dptr, task_->ev_done);
+        if (!o.resident) hazard_set_writer(o.src_p, task_->ev_done);
+      } else {
+        hazard_add_reader(o.dptr, task_->ev_done);
+        if (!o.resident) hazard_add_reader(o.src_p, task_->ev_done);
+      }
+    }
+    for (auto p : temps_) hazard_set_writer(p, task_->ev_done);
+    // output images other than the source become stale
+    for (auto &o : ops_)
+      if (o.output) {
+        image_discard_other(o.t, o.img);
+        o.img = 0;
+      }
+    // images in use
+    for (auto &o : ops_) {
+      const int i = find_image(o.t, o.src_p);
+      if (i < 0) continue;
+      if (o.output || o.coh == COPY_D || (o.coh == COPY_M && !o.resident)) o.t->avail[i] = NOPE;
+    }
+    // deferred bookkeeping
+    const int g = g_;
+    std::vector<Operand> ops = ops_;
+    std::vector<void *> temps = temps_;
+    std::vector<int> th = temp_handles_;
+    task_->on_finalize.push_back([g, ops, temps, th]() {
+      for (size_t i = 0; i < temps.size(); ++i) dev_free(g, temps[i], th[i]);
+      for (const auto &o : ops) finalize_operand(g, o);
+    });
+    task_->flops = flops;
+    d.tasks_submitted++;
+    d.active++;
+    committed_ = true;
+    return TALSH_SUCCESS;
+  }
+
+  // Publishes the task through the user's handle, or waits for it when the
+  // call is blocking.
+  int publish(int nargs) {
+    talsh_task_t local;
+    talsh_task_t *tt = user_ ? user_ : &local;
+    if (!user_) talshTaskClean(&local);
+    tt->task_p = task_;
+    tt->task_error = -1;
+    tt->dev_kind = DEV_NVIDIA_GPU;
+    tt->data_kind = data_kind_;
+    tt->coherence = coh_ctrl_;
+    tt->num_args = nargs;
+    for (int i = 0; i < nargs && i < MAX_TENSOR_OPERANDS; ++i) {
+      tt->tens_args[i].tens_p = ops_[i].t;
+      tt->tens_args[i].source_image = ops_[i].img;
+    }
+    tt->flops = task_->flops;
+    tt->data_vol = task_->bytes;
+    task_ = nullptr;
+    if (user_) return TALSH_SUCCESS;
+    const int st = task_finalize_if_done(tt, true);
+    talshTaskDestruct(tt);
+    return st == TALSH_TASK_COMPLETED ? TALSH_SUCCESS : TALSH_TASK_ERROR;
+  }
+
+ private:
+  static int find_image(const talsh_tens_t *t, const void *p) {
+    for (int i = 0; i < t->ndev; ++i)
+      if (t->dev_rsc[i].gmem_p == p) return i;
+    return -1;
+  }
+
+  static void finalize_operand(int g, const Operand &o) {
+    talsh_tens_t *t = o.t;
+    if (talshTensorIsEmpty(t) == YEP) return;
+    const int si = find_image(t, o.src_p);
+    if (o.resident) {
+      if (si >= 0) {
+        if (o.coh == COPY_D) image_discard(t, si);
+        else t->avail[si] = YEP;
+      }
+      return;
+    }
+    switch (o.coh) {
+      case COPY_D:
+        dev_free(g, o.dptr, o.dhandle);
+        if (si >= 0) image_discard(t, si);
+        break;
+      case COPY_M:
+        if (si >= 0) image_discard(t, si);
+        image_add(t, flat_gpu(g), o.dptr, o.dhandle, 0, o.dk);
+        break;
+      case COPY_T:
+        dev_free(g, o.dptr, o.dhandle);
+        if (si >= 0) t->avail[si] = YEP;
+        break;
+      default:  // COPY_K
+        if (si >= 0) t->avail[si] = YEP;
+        image_add(t, flat_gpu(g), o.dptr, o.dhandle, 0, o.dk);
+        break;
+    }
+  }
+
+  void rollback() {
+    if (s_) (void)hipStreamSynchronize(s_);  // transfers already queued must land before buffers are reused
+    for (auto &o : ops_)
+      if (!o.resident && o.dptr && o.dhandle >= 0) rt().gpus[g_].arena.release(o.dhandle);
+    for (size_t i = 0; i < temps_.size(); ++i) rt().gpus[g_].arena.release(temp_handles_[i]);
+    if (task_) task_free(task_);
+    task_ = nullptr;
+  }
+
+  int g_;
+  talsh_task_t *user_;
+  std::vector<Operand> ops_;
+  std::vector<void *> temps_;
+  std::vector<int> temp_handles_;
+  Task *task_ = nullptr;
+  hipStream_t s_ = nullptr;
+  bool committed_ = false;
+  int data_kind_ = NO_TYPE, coh_ctrl_ = 0;
+};
+
+// Execution GPU for (dev_id, dev_kind); operands are consulted when the
+// caller leaves the choice to the runtime.  Host execution is not provided.
+int exec_gpu(int dev_id, int dev_kind, std::initializer_list<const talsh_tens_t *> tens, int *g) {
+  Runtime &r = rt();
+  int flat;
+  if (dev_kind == DEV_DEFAULT) {
+    if (dev_id == DEV_DEFAULT) {
+      for (const talsh_tens_t *t : tens) {
+        if (!t || !t->dev_rsc) continue;
+        for (int i = 0; i < t->ndev; ++i) {
+          const int gi = gpu_index(t->dev_rsc[i].dev_id);
+          if (gi >= 0) { *g = gi; return TALSH_SUCCESS; }
+        }
+      }
+      const int b = talshDeviceBusyLeast(DEV_NVIDIA_GPU);
+      if (b < 0) return TALSH_NOT_AVAILABLE;
+      *g = b;
+      return TALSH_SUCCESS;
+    }
+    flat = dev_id;
+  } else if (dev_kind == DEV_HOST) {
+    return TALSH_NOT_AVAILABLE;
+  } else if (dev_kind == DEV_NVIDIA_GPU || dev_kind == DEV_AMD_GPU) {
+    if (dev_id == DEV_DEFAULT) {
+      const int b = talshDeviceBusyLeast(DEV_NVIDIA_GPU);
+      if (b < 0) return TALSH_NOT_AVAILABLE;
+      *g = b;
+      return TALSH_SUCCESS;
+    }
+    flat = talshFlatDevId(dev_kind, dev_id);
+  } else {
+    return TALSH_NOT_AVAILABLE;
+  }
+  if (canonical_dev(flat) == DEV_HOST) return TALSH_NOT_AVAILABLE;
+  const int gi = gpu_index(flat);
+  if (gi < 0) return TALSH_INVALID_ARGS;
+  (void)r;
+  *g = gi;
+  return TALSH_SUCCESS;
+}
+
+int check_task(talsh_task_t *tt) {
+  if (tt && talshTaskIsEmpty(tt) != YEP) return TALSH_OBJECT_NOT_EMPTY;
+  return TALSH_SUCCESS;
+}
+
+int deferred(int g, int e) {
+  if (e == TRY_LATER || e == DEVICE_UNABLE) rt().gpus[g].tasks_deferred++;
+  return e;
+}
+
+std::vector<int> dims_of(const talsh_tens_t *t) {
+  return std::vector<int>(t->shape_p->dims, t->shape_p->dims + t->shape_p->num_dim);
+}
+
+}  // namespace
+}  // namespace tal
+
+using namespace tal;
+
+extern "C" {
+
+int talshTensorInit(talsh_tens_t *dtens, double val_real, double val_imag, int dev_id, int dev_kind, int copy_ctrl,
+                    talsh_task_t *talsh_task) {
+  std::lock_guard<std::recursive_mutex> lk(rt().mu);
+  if (!rt().on) return TALSH_NOT_INITIALIZED;
+  if (int e = check_task(talsh_task)) return e;
+  int g;
+  if (int e = exec_gpu(dev_id, dev_kind, {dtens}, &g)) return e;
+  Op op(g, talsh_task);
+  if (int e = op.add(dtens, coh_letter(copy_ctrl, 1, 0), true, false)) return e;
+  if (int e = op.acquire({})) return deferred(g, e);
+  Operand &d = op.operand(0);
+  if (int e = op.begin(d.dk, copy_ctrl)) return e;
+  if (launch_init(d.dk, d.dptr, shape_volume(dtens->shape_p), val_real, val_imag, op.stream())) return TALSH_FAILURE;
+  if (int e = op.end(0.0)) return e;
+  return op.publish(1);
+}
+int talshTensorInit_(talsh_tens_t *dtens, double val_real, double val_imag, int dev_id, int dev_kind,
+                     int copy_ctrl, talsh_task_t *talsh_task) {
+  return talshTensorInit(dtens, val_real, val_imag, dev_id, dev_kind, copy_ctrl, talsh_task);
+}
+
+int talshTensorScale(talsh_tens_t *dtens, double val_real, double val_imag, int dev_id, int dev_kind,
+                     int copy_ctrl, talsh_task_t *talsh_task) {
+  std::lock_guard<std::recursive_mutex> lk(rt().mu);
+  if (!rt().on) return TALSH_NOT_INITIALIZED;
+  if (int e = check_task(talsh_task)) return e;
+  int g;
+  if (int e = exec_gpu(dev_id, dev_kind, {dtens}, &g)) return e;
+  Op op(g, talsh_task);
+  if (int e = op.add(dtens, coh_letter(copy_ctrl, 1, 0), true, true)) return e;
+  if (int e = op.acquire({})) return deferred(g, e);
+  Operand &d = op.operand(0);
+  if (int e = op.begin(d.dk, copy_ctrl)) return e;
+  const size_t vol = shape_volume(dtens->shape_p);
+  if (launch_scale(d.dk, d.dptr, vol, val_real, val_imag, op.stream())) return TALSH_FAILURE;
+  if (int e = op.end((d.dk == C4 || d.dk == C8) ? 6.0 * vol : 1.0 * vol)) return e;
+  return op.publish(1);
+}
+int talshTensorScale_(talsh_tens_t *dtens, double val_real, double val_imag, int dev_id, int dev_kind,
+                      int copy_ctrl, talsh_task_t *talsh_task) {
+  return talshTensorScale(dtens, val_real, val_imag, dev_id, dev_kind, copy_ctrl, talsh_task);
+}
+
+// Unary pattern -> 0-based O2N permutation of L into D, with shape checks.
+static int unary_perm(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, std::vector<int> *o2n, int *conj) {
+  DigitalPattern p;
+  if (parse_pattern(cptrn, &p) != 0) return TALSH_INVALID_ARGS;
+  if (p.rrank != 0 || p.drank != p.lrank) return TALSH_INVALID_ARGS;
+  if (talshTensorIsEmpty(dtens) == YEP || talshTensorIsEmpty(ltens) == YEP) return TALSH_OBJECT_IS_EMPTY;
+  if (dtens->shape_p->num_dim != p.drank || ltens->shape_p->num_dim != p.lrank) return TALSH_INVALID_ARGS;
+  o2n->resize(p.lrank);
+  for (int i = 0; i < p.lrank; ++i) {
+    const int t = p.dig[i];
+    if (t < 1 || t > p.drank || ltens->shape_p->dims[i] != dtens->shape_p->dims[t - 1]) return TALSH_INVALID_ARGS;
+    (*o2n)[i] = t - 1;
+  }
+  *conj = ((p.conj >> 1) & 1) ^ (p.conj & 1);
+  return TALSH_SUCCESS;
+}
+
+int talshTensorCopy(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, int dev_id, int dev_kind,
+                    int copy_ctrl, talsh_task_t *talsh_task) {
+  std::lock_guard<std::recursive_mutex> lk(rt().mu);
+  if (!rt().on) return TALSH_NOT_INITIALIZED;
+  if (!cptrn || !dtens || !ltens) return TALSH_INVALID_ARGS;
+  if (int e = check_task(talsh_task)) return e;
+  std::vector<int> o2n;
+  int conj = 0;
+  if (int e = unary_perm(cptrn, dtens, ltens, &o2n, &conj)) return e;
+  int g;
+  if (int e = exec_gpu(dev_id, dev_kind, {dtens, ltens}, &g)) return e;
+  Op op(g, talsh_task);
+  if (int e = op.add(dtens, coh_letter(copy_ctrl, 2, 0), true, false)) return e;
+  if (int e = op.add(ltens, coh_letter(copy_ctrl, 2, 1), false, true)) return e;
+  if (op.operand(0).dk != op.operand(1).dk) return TALSH_INVALID_ARGS;
+  if (int e = op.acquire({})) return deferred(g, e);
+  const int dk = op.operand(0).dk;
+  if (int e = op.begin(dk, copy_ctrl)) return e;
+  std::vector<int> ld = dims_of(ltens);
+  if (launch_permute(dk, op.operand(1).dptr, op.operand(0).dptr, (int)ld.size(), ld.data(), o2n.data(), PERM_COPY,
+                     1.0, 0.0, conj, op.stream()))
+    return TALSH_FAILURE;
+  if (int e = op.end(0.0)) return e;
+  return op.publish(2);
+}
+int talshTensorCopy_(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, int dev_id, int dev_kind,
+                     int copy_ctrl, talsh_task_t *talsh_task) {
+  return talshTensorCopy(cptrn, dtens, ltens, dev_id, dev_kind, copy_ctrl, talsh_task);
+}
+
+int talshTensorAdd(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, double scale_real,
+                   double scale_imag, int dev_id, int dev_kind, int copy_ctrl, talsh_task_t *talsh_task) {
+  std::lock_guard<std::recursive_mutex> lk(rt().mu);
+  if (!rt().on) return TALSH_NOT_INITIALIZED;
+  if (!cptrn || !dtens || !ltens) return TALSH_INVALID_ARGS;
+  if (int e = check_task(talsh_task)) return e;
+  std::vector<int> o2n;
+  int conj = 0;
+  if (int e = unary_perm(cptrn, dtens, ltens, &o2n, &conj)) return e;
+  int g;
+  if (int e = exec_gpu(dev_id, dev_kind, {dtens, ltens}, &g)) return e;
+  Op op(g, talsh_task);
+  if (int e = op.add(dtens, coh_letter(copy_ctrl, 2, 0), true, true)) return e;
+  if (int e = op.add(ltens, coh_letter(copy_ctrl, 2, 1), false, true)) return e;
+  if (op.operand(0).dk != op.operand(1).dk) return TALSH_INVALID_ARGS;
+  if (int e = op.acquire({})) return deferred(g, e);
+  const int dk = op.operand(0).dk;
+  if (int e = op.begin(dk, copy_ctrl)) return e;
+  std::vector<int> ld = dims_of(ltens);
+  if (launch_permute(dk, op.operand(1).dptr, op.operand(0).dptr, (int)ld.size(), ld.data(), o2n.data(), PERM_ADD,
+                     scale_real, scale_imag, conj, op.stream()))
+    return TALSH_FAILURE;
+  const double vol = (double)shape_volume(dtens->shape_p);
+  if (int e = op.end((dk == C4 || dk == C8) ? 8.0 * vol : 2.0 * vol)) return e;
+  return op.publish(2);
+}
+int talshTensorAdd_(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, double scale_real,
+                    double scale_imag, int dev_id, int dev_kind, int copy_ctrl, talsh_task_t *talsh_task) {
+  return talshTensorAdd(cptrn, dtens, ltens, scale_real, scale_imag, dev_id, dev_kind, copy_ctrl, talsh_task);
+}
+
+int talshTensorContract(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, talsh_tens_t *rtens,
+                        double scale_real, double scale_imag, int dev_id, int dev_kind, int copy_ctrl,
+                        int accumulative, talsh_task_t *talsh_task) {
+  std::lock_guard<std::recursive_mutex> lk(rt().mu);
+  if (!rt().on) return TALSH_NOT_INITIALIZED;
+  if (!cptrn || !dtens || !ltens || !rtens) return TALSH_INVALID_ARGS;
+  if (int e = check_task(talsh_task)) return e;
+  if (talshTensorIsEmpty(dtens) == YEP || talshTensorIsEmpty(ltens) == YEP || talshTensorIsEmpty(rtens) == YEP)
+    return TALSH_OBJECT_IS_EMPTY;
+  DigitalPattern p;
+  if (parse_pattern(cptrn, &p) != 0) return TALSH_INVALID_ARGS;
+  if (p.drank != dtens->shape_p->num_dim || p.lrank != ltens->shape_p->num_dim || p.rrank != rtens->shape_p->num_dim)
+    return TALSH_INVALID_ARGS;
+  ContractPlan plan;
+  const int pe = plan_contraction(p, dtens->shape_p->dims, ltens->shape_p->dims, rtens->shape_p->dims,
+                                  accumulative != NOPE, &plan);
+  if (pe == 2) return DEVICE_UNABLE;
+  if (pe != 0) return TALSH_INVALID_ARGS;
+  int g;
+  if (int e = exec_gpu(dev_id, dev_kind, {dtens, ltens, rtens}, &g)) return e;
+  Op op(g, talsh_task);
+  const bool acc = accumulative != NOPE;
+  if (int e = op.add(dtens, coh_letter(copy_ctrl, 3, 0), true, acc)) return e;
+  if (int e = op.add(ltens, coh_letter(copy_ctrl, 3, 1), false, true)) return e;
+  if (int e = op.add(rtens, coh_letter(copy_ctrl, 3, 2), false, true)) return e;
+  const int dk = op.operand(0).dk;
+  if (op.operand(1).dk != dk || op.operand(2).dk != dk) return TALSH_INVALID_ARGS;
+  const bool cplx = (dk == C4 || dk == C8);
+  // temporaries: [A'], [B'], [D']
+  std::vector<size_t> temps;
+  const int xi = plan.swap ? 2 : 1, yi = plan.swap ? 1 : 2;
+  int ta = -1, tb = -1, td = -1;
+  if (plan.a.transpose) { ta = (int)temps.size(); temps.push_back(op.operand(xi).bytes); }
+  if (plan.b.transpose) { tb = (int)temps.size(); temps.push_back(op.operand(yi).bytes); }
+  if (plan.d_transpose) { td = (int)temps.size(); temps.push_back(op.operand(0).bytes); }
+  if (int e = op.acquire(temps)) return deferred(g, e);
+  if (int e = op.begin(dk, copy_ctrl)) return e;
+  hipStream_t s = op.stream();
+  Operand &D = op.operand(0), &X = op.operand(xi), &Y = op.operand(yi);
+  const void *A = X.dptr, *B = Y.dptr;
+  void *C = D.dptr;
+  if (ta >= 0) {
+    std::vector<int> xd = dims_of(X.t);
+    if (launch_permute(dk, X.dptr, op.temp(ta), (int)xd.size(), xd.data(), plan.a.o2n.data(), PERM_COPY, 1, 0, 0, s))
+      return TALSH_FAILURE;
+    A = op.temp(ta);
+  }
+  if (tb >= 0) {
+    std::vector<int> yd = dims_of(Y.t);
+    if (launch_permute(dk, Y.dptr, op.temp(tb), (int)yd.size(), yd.data(), plan.b.o2n.data(), PERM_COPY, 1, 0, 0, s))
+      return TALSH_FAILURE;
+    B = op.temp(tb);
+  }
+  if (td >= 0) {
+    C = op.temp(td);
+    if (acc) {
+      std::vector<int> dd = dims_of(dtens);
+      if (launch_permute(dk, D.dptr, C, (int)dd.size(), dd.data(), plan.d_o2n.data(), PERM_COPY, 1, 0, 0, s))
+        return TALSH_FAILURE;
+    }
+  }
+  int lconj = (p.conj >> 1) & 1, rconj = (p.conj >> 2) & 1;
+  if (p.conj & 1) { lconj ^= 1; rconj ^= 1; }
+  if (!cplx) lconj = rconj = 0;
+  op.mark_gemm(true);
+  if (launch_gemm(dk, rt().r4mode, plan.a.op, plan.b.op, plan.swap ? rconj : lconj, plan.swap ? lconj : rconj,
+                  plan.M, plan.N, plan.K, scale_real, cplx ? scale_imag : 0.0, A, plan.a.ld, B, plan.b.ld,
+                  acc ? 1 : 0, C, std::max<int64_t>(1, plan.M), s))
+    return TALSH_FAILURE;
+  op.mark_gemm(false);
+  if (td >= 0) {
+    if (launch_permute(dk, C, D.dptr, (int)plan.dprime_dims.size(), plan.dprime_dims.data(),
+                       plan.d_back_o2n.data(), PERM_COPY, 1, 0, 0, s))
+      return TALSH_FAILURE;
+  }
+  const double flops = (cplx ? 8.0 : 2.0) * (double)plan.M * (double)plan.N * (double)plan.K;
+  if (int e = op.end(flops)) return e;
+  return op.publish(3);
+}
+int talshTensorContract_(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, talsh_tens_t *rtens,
+                         double scale_real, double scale_imag, int dev_id, int dev_kind, int copy_ctrl,
+                         int accumulative, talsh_task_t *talsh_task) {
+  return talshTensorContract(cptrn, dtens, ltens, rtens, scale_real, scale_imag, dev_id, dev_kind, copy_ctrl,
+                             accumulative, talsh_task);
+}
+
+int talshTensorNorm2(talsh_tens_t *dtens, talsh_tens_t *ltens, int dev_id, int dev_kind, int copy_ctrl,
+                     talsh_task_t *talsh_task) {
+  std::lock_guard<std::recursive_mutex> lk(rt().mu);
+  if (!rt().on) return TALSH_NOT_INITIALIZED;
+  if (!dtens || !ltens) return TALSH_INVALID_ARGS;
+  if (int e = check_task(talsh_task)) return e;
+  if (talshTensorIsEmpty(dtens) == YEP || talshTensorIsEmpty(ltens) == YEP) return TALSH_OBJECT_IS_EMPTY;
+  if (shape_volume(dtens->shape_p) != 1) return TALSH_INVALID_ARGS;
+  int g;
+  if (int e = exec_gpu(dev_id, dev_kind, {ltens, dtens}, &g)) return e;
+  Op op(g, talsh_task);
+  if (int e = op.add(dtens, coh_letter(copy_ctrl, 2, 0), true, false)) return e;
+  if (int e = op.add(ltens, coh_letter(copy_ctrl, 2, 1), false, true)) return e;
+  const int dk = op.operand(1).dk;
+  if (op.operand(0).dk != dk) return TALSH_INVALID_ARGS;
+  if (int e = op.acquire({norm2_scratch_elems() * sizeof(double)})) return deferred(g, e);
+  if (int e = op.begin(dk, copy_ctrl)) return e;
+  const size_t vol = shape_volume(ltens->shape_p);
+  if (launch_norm2(dk, op.operand(1).dptr, vol, (double *)op.temp(0), op.operand(0).dptr, op.stream()))
+    return TALSH_FAILURE;
+  if (int e = op.end((dk == C4 || dk == C8) ? 4.0 * vol : 2.0 * vol)) return e;
+  return op.publish(2);
+}
+
+// Places a body image on a device (talshc.cpp:3318-3480): COPY_M moves it
+// (source image discarded at completion), COPY_K keeps both.
+int talshTensorPlace(talsh_tens_t *tens, int dev_id, int dev_kind, void *dev_mem, int copy_ctrl,
+                     talsh_task_t *talsh_task) {
+  std::lock_guard<std::recursive_mutex> lk(rt().mu);
+  if (!rt().on) return TALSH_NOT_INITIALIZED;
+  if (!tens) return TALSH_INVALID_ARGS;
+  if (talshTensorIsEmpty(tens) == YEP) return TALSH_OBJECT_IS_EMPTY;
+  if (tensor_healthy(tens) != YEP) return TALSH_FAILURE;
+  if (copy_ctrl != COPY_M && copy_ctrl != COPY_K) return TALSH_INVALID_ARGS;
+  if (int e = check_task(talsh_task)) return e;
+  const int target = canonical_dev(dev_kind == DEV_NULL ? dev_id : talshFlatDevId(dev_kind, dev_id));
+  if (target < 0) return TALSH_INVALID_ARGS;
+  const int tg = gpu_index(target);
+  if (target != DEV_HOST && tg < 0) return TALSH_INVALID_ARGS;
+  // already there?
+  int src = -1;
+  for (int i = 0; i < tens->ndev; ++i)
+    if (tens->avail[i] == YEP && tens->dev_rsc[i].dev_id == target) return TALSH_SUCCESS;
+  for (int i = 0; i < tens->ndev; ++i)
+    if (tens->avail[i] == YEP && (src < 0 || tens->dev_rsc[i].dev_id == DEV_HOST)) src = i;
+  if (src < 0) return TALSH_NOT_FOUND;
+  const int sg = gpu_index(tens->dev_rsc[src].dev_id);
+  const int g = tg >= 0 ? tg : sg;  // GPU that drives the transfer
+  if (g < 0) return TALSH_FAILURE;
+  const int dk = tens->data_kind[src];
+  const size_t bytes = shape_volume(tens->shape_p) * (size_t)dk_size(dk);
+  void *dst = dev_mem;
+  int h = -1, attached = dev_mem ? 1 : 0;
+  if (!dst) {
+    const int e = (target == DEV_HOST) ? host_alloc(bytes, &dst, &h) : dev_alloc(tg, bytes, &dst, &h);
+    if (e) return deferred(g, e);
+  }
+  Device &d = rt().gpus[g];
+  (void)hipSetDevice(d.hip_id);
+  Task *task = task_new(g);
+  hipStream_t s = task->stream;
+  void *sp = tens->dev_rsc[src].gmem_p;
+  (void)hipEventRecord(task->ev_start->ev, s);
+  hazard_wait_read(sp, s);
+  hazard_wait_write(dst, s);
+  (void)hipEventRecord(task->ev_in->ev, s);
+  if (hipMemcpyAsync(dst, sp, bytes, hipMemcpyDefault, s) != hipSuccess) {
+    task_free(task);
+    if (!attached) {
+      if (target == DEV_HOST) host_free(dst, h);
+      else dev_free(tg, dst, h);
+    }
+    return TALSH_FAILURE;
+  }
+  (void)hipEventRecord(task->ev_comp->ev, s);
+  (void)hipEventRecord(task->ev_done->ev, s);
+  hazard_add_reader(sp, task->ev_done);
+  hazard_set_writer(dst, task->ev_done);
+  if (target == DEV_HOST) d.traffic_out += (double)bytes;
+  else d.traffic_in += (double)bytes;
+  task->bytes = (double)bytes;
+  if (copy_ctrl == COPY_M) tens->avail[src] = NOPE;
+  task->on_finalize.push_back([tens, sp, dst, h, attached, target, dk, copy_ctrl]() {
+    if (talshTensorIsEmpty(tens) == YEP) return;
+    int si = -1;
+    for (int i = 0; i < tens->ndev; ++i)
+      if (tens->dev_rsc[i].gmem_p == sp) si = i;
+    if (si >= 0) {
+      if (copy_ctrl == COPY_M) image_discard(tens, si);
+      else tens->avail[si] = YEP;
+    }
+    image_add(tens, target, dst, h, attached, dk);
+  });
+  d.tasks_submitted++;
+  d.active++;
+  talsh_task_t local;
+  talsh_task_t *tt = talsh_task ? talsh_task : &local;
+  if (!talsh_task) talshTaskClean(&local);
+  tt->task_p = task;
+  tt->task_error = -1;
+  tt->dev_kind = DEV_NVIDIA_GPU;
+  tt->data_kind = dk;
+  tt->coherence = copy_ctrl;
+  tt->num_args = 1;
+  tt->tens_args[0].tens_p = tens;
+  tt->tens_args[0].source_image = src;
+  tt->data_vol = (double)bytes;
+  if (talsh_task) return TALSH_SUCCESS;
+  const int st = task_finalize_if_done(tt, true);
+  talshTaskDestruct(tt);
+  return st == TALSH_TASK_COMPLETED ? TALSH_SUCCESS : TALSH_TASK_ERROR;
+}
+int talshTensorPlace_(talsh_tens_t *tens, int dev_id, int dev_kind, void *dev_mem, int copy_ctrl,
+                      talsh_task_t *talsh_task) {
+  return talshTensorPlace(tens, dev_id, dev_kind, dev_mem, copy_ctrl, talsh_task);
+}
+
+int talshTensorDiscard(talsh_tens_t *tens, int dev_id, int dev_kind) {
+  std::lock_guard<std::recursive_mutex> lk(rt().mu);
+  if (!rt().on) return TALSH_NOT_INITIALIZED;
+  if (!tens) return TALSH_INVALID_ARGS;
+  if (talshTensorIsEmpty(tens) == YEP) return TALSH_OBJECT_IS_EMPTY;
+  const int target = canonical_dev(dev_kind == DEV_NULL ? dev_id : talshFlatDevId(dev_kind, dev_id));
+  for (int i = tens->ndev - 1; i >= 0; --i)
+    if (tens->dev_rsc[i].dev_id == target) {
+      if (tens->avail[i] != YEP) return TALSH_NOT_AVAILABLE;
+      image_discard(tens, i);
+    }
+  return TALSH_SUCCESS;
+}
+int talshTensorDiscard_(talsh_tens_t *tens, int dev_id, int dev_kind) {
+  return talshTensorDiscard(tens, dev_id, dev_kind);
+}
+
+int talshTensorDiscardOther(talsh_tens_t *tens, int dev_id, int dev_kind) {
+  std::lock_guard<std::recursive_mutex> lk(rt().mu);
+  if (!rt().on) return TALSH_NOT_INITIALIZED;
+  if (!tens) return TALSH_INVALID_ARGS;
+  if (talshTensorIsEmpty(tens) == YEP) return TALSH_OBJECT_IS_EMPTY;
+  const int target = canonical_dev(dev_kind == DEV_NULL ? dev_id : talshFlatDevId(dev_kind, dev_id));
+  int keep = -1;
+  for (int i = 0; i < tens->ndev; ++i)
+    if (tens->dev_rsc[i].dev_id == target) keep = i;
+  if (keep < 0) return TALSH_NOT_FOUND;
+  for (int i = 0; i < tens->ndev; ++i)
+    if (i != keep && tens->avail[i] != YEP) return TALSH_NOT_AVAILABLE;
+  image_discard_other(tens, keep);
+  return TALSH_SUCCESS;
+}
+int talshTensorDiscardOther_(talsh_tens_t *tens, int dev_id, int dev_kind) {
+  return talshTensorDiscardOther(tens, dev_id, dev_kind);
+}
+
+// Fast math (talsh.h:167-174): on this build it selects the split-BF16 MFMA
+// path for R4/C4 GEMMs; R8/C8 are unaffected.
+int talshEnableFastMath(int dev_kind, int dev_id) {
+  (void)dev_kind;
+  (void)dev_id;
+  rt().r4mode = R4_SPLIT_BF16;
+  return TALSH_SUCCESS;
+}
+int talshDisableFastMath(int dev_kind, int dev_id) {
+  (void)dev_kind;
+  (void)dev_id;
+  rt().r4mode = R4_FP32;
+  return TALSH_SUCCESS;
+}
+int talshQueryFastMath(int dev_kind, int dev_id) {
+  (void)dev_kind;
+  (void)dev_id;
+  return rt().r4mode == R4_SPLIT_BF16 ? YEP : NOPE;
+}
+
+// ---- extensions (talsh_amd.h) ----
+int talshAmdSetDeviceBufferSize(size_t bytes) {
+  if (rt().on) return TALSH_ALREADY_INITIALIZED;
+  rt().dev_buffer_request = bytes;
+  return TALSH_SUCCESS;
+}
+int talshAmdSetNumStreams(int n) {
+  if (rt().on) return TALSH_ALREADY_INITIALIZED;
+  if (n < 1 || n > 32) return TALSH_INVALID_ARGS;
+  rt().nstreams = n;
+  return TALSH_SUCCESS;
+}
+int talshAmdDeviceSynchronize(int gpu) {
+  std::lock_guard<std::recursive_mutex> lk(rt().mu);
+  const int g = gpu_index(flat_gpu(gpu));
+  if (g < 0) return TALSH_INVALID_ARGS;
+  (void)hipSetDevice(rt().gpus[g].hip_id);
+  return hipDeviceSynchronize() == hipSuccess ? TALSH_SUCCESS : TALSH_FAILURE;
+}
+int talshAmdBuildInfo(char *buf, int len) {
+  const char *s = "libtalsh_amd gfx950 MFMA f64/f32 TTGT";
+  if (!buf || len <= 0) return TALSH_INVALID_ARGS;
+  std::strncpy(buf, s, (size_t)len - 1);
+  buf[len - 1] = 0;
+  return TALSH_SUCCESS;
+}
+
+}  // extern "C"
+
+// ---- pattern / plan queries ----
+extern "C" {
+
+// Symbolic -> digital pattern, same signature and meaning as the reference's
+// C-callable converter (talshf.F90:619-667, declared in talshc.cpp:109).
+int talsh_get_contr_ptrn_str2dig(const char *c_str, int *dig_ptrn, int *drank, int *lrank, int *rrank,
+                                 int *conj_bits) {
+  if (!c_str || !dig_ptrn || !drank || !lrank || !rrank || !conj_bits) return -1;
+  DigitalPattern p;
+  *drank = *lrank = *rrank = -1;
+  *conj_bits = 0;
+  const int e = parse_pattern(c_str, &p);
+  if (e) return e;
+  *drank = p.drank;
+  *lrank = p.lrank;
+  *rrank = p.rrank;
+  *conj_bits = p.conj;
+  for (int i = 0; i < p.lrank + p.rrank; ++i) dig_ptrn[i] = p.dig[i];
+  return 0;
+}
+
+// Contraction plan chosen for a pattern and shapes (see talsh_amd.h).
+int talshAmdPlanContraction(const char *pattern, const int *ddims, const int *ldims, const int *rdims,
+                            int accumulative, long long *mnk, int *flags, char *ops, int *perms, double *moved) {
+  DigitalPattern p;
+  if (!pattern || parse_pattern(pattern, &p) != 0) return TALSH_INVALID_ARGS;
+  ContractPlan c;
+  const int e = plan_contraction(p, ddims, ldims, rdims, accumulative, &c);
+  if (e == 2) return DEVICE_UNABLE;
+  if (e) return TALSH_INVALID_ARGS;
+  if (mnk) { mnk[0] = c.M; mnk[1] = c.N; mnk[2] = c.K; }
+  if (flags) { flags[0] = c.swap; flags[1] = c.a.transpose; flags[2] = c.b.transpose; flags[3] = c.d_transpose; }
+  if (ops) { ops[0] = c.a.op; ops[1] = c.b.op; }
+  if (perms) {
+    // [A o2n (rank of A)] [B o2n] [D->D' o2n]; -1 fills unused entries
+    int *q = perms;
+    const int ra = c.swap ? p.rrank : p.lrank, rb = c.swap ? p.lrank : p.rrank;
+    for (int i = 0; i < ra; ++i) *q++ = c.a.transpose ? c.a.o2n[i] : -1;
+    for (int i = 0; i < rb; ++i) *q++ = c.b.transpose ? c.b.o2n[i] : -1;
+    for (int i = 0; i < p.drank; ++i) *q++ = c.d_transpose ? c.d_o2n[i] : -1;
+  }
+  if (moved) *moved = c.moved_elems;
+  return TALSH_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,281 @@
+// Pattern parsing and contraction planning.
+//
+// parse_pattern follows the semantics of talsh_get_contr_ptrn_str2dig
+// (talshf.F90:619-667) and get_contr_pattern_dig (tensor_algebra_cpu.F90:
+// 5307-5531): arguments NAME[+](idx,...), index separators ',' or '|', a
+// trailing '+' on an index marks contravariance (ignored for the digits), a
+// '+' after an argument name conjugates it, a trailing scalar factor "*2" is
+// dropped, and a unary pattern gets a rank-0 right operand.
+//
+// plan_contraction replaces the reference GPU matricization
+// get_contr_permutations (tensor_algebra_cpu.F90:5613-5755, called from
+// tensor_algebra_gpu_nvidia.cu:6083-6095), which always produces the TN form
+// and transposes an operand unless the fixed TN order happens to match.  Here
+// both operand roles (L as A, or R as A), both storage forms of each operand
+// ('N' or 'T') and three orderings for the M, N and K index groups are
+// enumerated, and the plan that copies the fewest elements is kept; operands
+// already in a usable layout go straight into the GEMM.
+#include "plan.hpp"
+
+#include <algorithm>
+#include <cctype>
+#include <cstring>
+#include <string>
+
+namespace tal {
+
+namespace {
+
+bool label_char(char c) { return std::isalnum(static_cast<unsigned char>(c)) != 0; }
+
+struct Arg {
+  std::vector<std::string> idx;
+  bool conj = false;
+};
+
+int split_args(const std::string &s, std::vector<Arg> *args) {
+  size_t i = 0;
+  while (i < s.size()) {
+    size_t open = s.find('(', i);
+    if (open == std::string::npos) break;
+    if (args->size() >= 3) return 1;
+    Arg a;
+    if (open >= 2 && s[open - 1] == '+' && (label_char(s[open - 2]) || s[open - 2] == '_')) a.conj = true;
+    size_t close = s.find(')', open + 1);
+    if (close == std::string::npos) return 5;
+    std::string body = s.substr(open + 1, close - open - 1);
+    if (!body.empty()) {
+      size_t start = 0;
+      while (true) {
+        size_t sep = body.find_first_of(",|", start);
+        std::string lab = body.substr(start, sep == std::string::npos ? std::string::npos : sep - start);
+        if (lab.empty()) return 2;
+        // alphanumeric label with an optional trailing '+' (contravariant)
+        if (lab.size() > 1 && lab.back() == '+') lab.pop_back();
+        for (char c : lab)
+          if (!label_char(c)) return 4;
+        a.idx.push_back(lab);
+        if (sep == std::string::npos) break;
+        start = sep + 1;
+      }
+    }
+    args->push_back(a);
+    i = close + 1;
+  }
+  return 0;
+}
+
+}  // namespace
+
+int parse_pattern(const char *pattern, DigitalPattern *out) {
+  if (!pattern || !out) return 16;
+  std::string s(pattern);
+  if (s.empty()) return 16;
+  if (s.size() > 1024) return 17;
+  // drop a trailing scalar factor "...)*2.0"; add a rank-0 right operand to unary patterns
+  size_t star = s.find('*');
+  if (star != std::string::npos && s.find('(', star) == std::string::npos) {
+    s.resize(star);
+    if (s.empty() || s.back() != ')') return 3;
+    star = std::string::npos;
+  }
+  if (star == std::string::npos) s += "*R()";
+  std::vector<Arg> args;
+  int err = split_args(s, &args);
+  if (err) return err;
+  if (args.size() != 3) return 15;
+  const int nd = (int)args[0].idx.size(), nl = (int)args[1].idx.size(), nr = (int)args[2].idx.size();
+  std::vector<int> dig(nl + nr, 0);
+  // every label must occur exactly twice (D-L, D-R or L-R), or D-L-R for a hyper index
+  std::vector<std::pair<int, int>> occ;  // (arg, pos)
+  std::vector<std::vector<bool>> done(3);
+  for (int a = 0; a < 3; ++a) done[a].assign(args[a].idx.size(), false);
+  for (int a = 0; a < 3; ++a)
+    for (int x = 0; x < (int)args[a].idx.size(); ++x) {
+      if (done[a][x]) continue;
+      occ.clear();
+      occ.emplace_back(a, x);
+      for (int b = a; b < 3; ++b)
+        for (int y = (b == a ? x + 1 : 0); y < (int)args[b].idx.size(); ++y)
+          if (args[b].idx[y] == args[a].idx[x]) occ.emplace_back(b, y);
+      if (occ.size() < 2) return 11;
+      if (occ.size() > 3) return 10;
+      for (auto &o : occ) done[o.first][o.second] = true;
+      const int a0 = occ[0].first, p0 = occ[0].second, a1 = occ[1].first, p1 = occ[1].second;
+      if (occ.size() == 3) {
+        if (!(a0 == 0 && a1 == 1 && occ[2].first == 2)) return 9;
+        dig[p1] = p0 + 1;
+        dig[nl + occ[2].second] = p0 + 1;
+      } else if (a0 == 0 && a1 == 1) {
+        dig[p1] = p0 + 1;
+      } else if (a0 == 0 && a1 == 2) {
+        dig[nl + p1] = p0 + 1;
+      } else if (a0 == 1 && a1 == 2) {
+        dig[p0] = -(p1 + 1);
+        dig[nl + p1] = -(p0 + 1);
+      } else {
+        return 10;
+      }
+    }
+  out->drank = nd;
+  out->lrank = nl;
+  out->rrank = nr;
+  out->conj = (args[0].conj ? 1 : 0) | (args[1].conj ? 2 : 0) | (args[2].conj ? 4 : 0);
+  out->dig = dig;
+  return 0;
+}
+
+namespace {
+
+constexpr int kContr = 1000;  // symbol of a contracted pair = kContr + its L position
+
+std::vector<int> filter(const std::vector<int> &seq, bool contracted) {
+  std::vector<int> r;
+  for (int s : seq)
+    if ((s >= kContr) == contracted) r.push_back(s);
+  return r;
+}
+
+std::vector<int> cat(const std::vector<int> &a, const std::vector<int> &b) {
+  std::vector<int> r(a);
+  r.insert(r.end(), b.begin(), b.end());
+  return r;
+}
+
+// O2N map from `seq` (current order) to `target` (0-based positions).
+std::vector<int> o2n_of(const std::vector<int> &seq, const std::vector<int> &target) {
+  std::vector<int> o(seq.size());
+  for (size_t i = 0; i < seq.size(); ++i)
+    o[i] = (int)(std::find(target.begin(), target.end(), seq[i]) - target.begin());
+  return o;
+}
+
+double vol_of(const int *dims, int n) {
+  double v = 1;
+  for (int i = 0; i < n; ++i) v *= dims[i];
+  return v;
+}
+
+}  // namespace
+
+int plan_contraction(const DigitalPattern &p, const int *ddims, const int *ldims, const int *rdims,
+                     int accumulative, ContractPlan *plan) {
+  const int nd = p.drank, nl = p.lrank, nr = p.rrank;
+  if (nd < 0 || nl < 0 || nr < 0 || (int)p.dig.size() != nl + nr) return 1;
+  std::vector<int> sl(nl), sr(nr), sd(nd);
+  std::vector<int> dcount(nd, 0);
+  std::vector<int64_t> ext_of(kContr + nl + 1, 0);  // extent per symbol
+  for (int i = 0; i < nd; ++i) {
+    sd[i] = i;
+    ext_of[i] = ddims[i];
+  }
+  for (int i = 0; i < nl; ++i) {
+    const int c = p.dig[i];
+    if (c > 0) {
+      if (c > nd || ldims[i] != ddims[c - 1]) return 1;
+      sl[i] = c - 1;
+      dcount[c - 1]++;
+    } else if (c < 0) {
+      const int j = -c - 1;
+      if (j >= nr || p.dig[nl + j] != -(i + 1) || ldims[i] != rdims[j]) return 1;
+      sl[i] = kContr + i;
+      ext_of[kContr + i] = ldims[i];
+    } else {
+      return 1;
+    }
+  }
+  for (int j = 0; j < nr; ++j) {
+    const int c = p.dig[nl + j];
+    if (c > 0) {
+      if (c > nd || rdims[j] != ddims[c - 1]) return 1;
+      sr[j] = c - 1;
+      dcount[c - 1]++;
+    } else if (c < 0) {
+      const int i = -c - 1;
+      if (i >= nl || p.dig[i] != -(j + 1)) return 1;
+      sr[j] = kContr + i;
+    } else {
+      return 1;
+    }
+  }
+  for (int i = 0; i < nd; ++i) {
+    if (dcount[i] == 0) return 1;
+    if (dcount[i] > 1) return 2;  // hyper-contraction index
+  }
+  const double vd = vol_of(ddims, nd), vl = vol_of(ldims, nl), vr = vol_of(rdims, nr);
+
+  bool have = false;
+  ContractPlan best;
+  int best_ntrans = 99;
+  for (int swap = 0; swap < 2; ++swap) {
+    const std::vector<int> &sx = swap ? sr : sl, &sy = swap ? sl : sr;
+    const int *xdims = swap ? rdims : ldims, *ydims = swap ? ldims : rdims;
+    const int nx = swap ? nr : nl, ny = swap ? nl : nr;
+    const double vx = swap ? vr : vl, vy = swap ? vl : vr;
+    const std::vector<int> mx = filter(sx, false), ny_ = filter(sy, false);
+    std::vector<int> m_sorted(mx), n_sorted(ny_);
+    std::sort(m_sorted.begin(), m_sorted.end());
+    std::sort(n_sorted.begin(), n_sorted.end());
+    const std::vector<int> kx = filter(sx, true), ky = filter(sy, true);
+    for (int mo = 0; mo < 2; ++mo)
+      for (int no = 0; no < 2; ++no)
+        for (int ko = 0; ko < 2; ++ko) {
+          const std::vector<int> &M = mo ? mx : m_sorted, &N = no ? ny_ : n_sorted, &K = ko ? ky : kx;
+          ContractPlan c;
+          c.swap = swap != 0;
+          const std::vector<int> dseq = cat(M, N);
+          c.d_transpose = (dseq != sd);
+          // operand X (A): 'T' form = K++M, 'N' form = M++K
+          const std::vector<int> xt = cat(K, M), xn = cat(M, K);
+          if (sx == xt) { c.a.op = 'T'; }
+          else if (sx == xn) { c.a.op = 'N'; }
+          else {
+            c.a.transpose = true;
+            const bool m_first = !sx.empty() && sx[0] < kContr;
+            c.a.op = m_first ? 'N' : 'T';
+            c.a.o2n = o2n_of(sx, m_first ? xn : xt);
+          }
+          // operand Y (B): 'N' form = K++N, 'T' form = N++K
+          const std::vector<int> yn = cat(K, N), yt = cat(N, K);
+          if (sy == yn) { c.b.op = 'N'; }
+          else if (sy == yt) { c.b.op = 'T'; }
+          else {
+            c.b.transpose = true;
+            const bool n_first = !sy.empty() && sy[0] < kContr;
+            c.b.op = n_first ? 'T' : 'N';
+            c.b.o2n = o2n_of(sy, n_first ? yt : yn);
+          }
+          c.M = 1;
+          for (int s : M) c.M *= ext_of[s];
+          c.N = 1;
+          for (int s : N) c.N *= ext_of[s];
+          c.K = 1;
+          for (int s : K) c.K *= ext_of[s];
+          c.a.ld = std::max<int64_t>(1, c.a.op == 'T' ? c.K : c.M);
+          c.b.ld = std::max<int64_t>(1, c.b.op == 'N' ? c.K : c.N);
+          c.moved_elems = (c.a.transpose ? vx : 0) + (c.b.transpose ? vy : 0) +
+                          (c.d_transpose ? vd * (accumulative ? 2 : 1) : 0);
+          if (c.d_transpose) {
+            c.d_o2n = o2n_of(sd, dseq);
+            c.d_back_o2n = o2n_of(dseq, sd);
+            c.dprime_dims.resize(nd);
+            for (int i = 0; i < nd; ++i) c.dprime_dims[i] = (int)ext_of[dseq[i]];
+          }
+          const int ntrans = (int)c.a.transpose + (int)c.b.transpose + (int)c.d_transpose;
+          if (!have || c.moved_elems < best.moved_elems ||
+              (c.moved_elems == best.moved_elems && ntrans < best_ntrans)) {
+            best = c;
+            best_ntrans = ntrans;
+            have = true;
+          }
+          (void)nx;
+          (void)ny;
+          (void)xdims;
+          (void)ydims;
+        }
+  }
+  *plan = best;
+  return 0;
+}
+
+}  // namespace tal

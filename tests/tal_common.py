@@ -89,8 +89,6 @@ class Oracle:
         self.lib = ctypes.CDLL(path)
         self.lib.orc_norm2.restype = ctypes.c_double
         self.lib.orc_norm2.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_void_p]
-        for name in ("orc_add", "orc_contract"):
-            pass
 
     def parse(self, pattern):
         dig = (ctypes.c_int * 128)()
@@ -148,6 +146,12 @@ class Oracle:
 
     def norm2(self, dk, a):
         return self.lib.orc_norm2(dk, a.size, a.ctypes.data_as(ctypes.c_void_p))
+
+    def scale(self, dk, a, alpha):
+        a = a.copy()
+        self.lib.orc_scale(dk, ctypes.c_size_t(a.size), a.ctypes.data_as(ctypes.c_void_p),
+                           ctypes.c_double(alpha[0]), ctypes.c_double(alpha[1]))
+        return a
 
 
 class Reference:

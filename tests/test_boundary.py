@@ -1,0 +1,185 @@
+"""CPU tests of the C-ABI boundary and host logic of libtalsh_amd.so.
+
+No kernel runs here: the library is loaded, its exported symbols are checked
+against include/*.h, struct layouts against the reference ABI sizes, the
+product's own pattern parser against the reference's golden vectors, and the
+contraction planner by executing its plans with numpy and comparing with the
+oracle.  Host-side error paths run with TAL-SH initialised on zero GPUs.
+"""
+import ctypes
+import json
+import os
+import random
+import re
+
+import numpy as np
+import pytest
+
+import tal_sh_amd as ts
+from tal_common import C8, R8, random_contraction, tensor
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(REPO, "tests", "golden")
+
+
+def header_functions():
+    names = set()
+    for h in ("talsh.h", "talsh_amd.h"):
+        text = open(os.path.join(REPO, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\**\s*\**\s*([A-Za-z_][A-Za-z0-9_]*)\s*\(",
+                             text, flags=re.M):
+            if m.group(1) not in ("if", "while", "for", "return", "sizeof", "typedef"):
+                names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ts.load()
+    names = header_functions()
+    assert "talshTensorContract" in names and "talshTensorNorm2" in names and len(names) > 70
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_struct_layouts_match_reference_abi():
+    # sizes/offsets of the reference structs on x86-64 (talsh.h:87-121, tensor_algebra.h:221-252)
+    assert ctypes.sizeof(ts.TalshTens) == 40
+    assert ctypes.sizeof(ts.TalshTask) == 120
+    assert ctypes.sizeof(ts.TalshTensArg) == 16
+    assert ctypes.sizeof(ts.DevRsc) == 24
+    assert ts.TalshTask.data_vol.offset == 96 and ts.TalshTask.exec_time.offset == 112
+    assert ts.TalshTens.ndev.offset == 36 and ts.DevRsc.mem_attached.offset == 20
+
+
+def test_header_constants():
+    text = open(os.path.join(REPO, "include", "tensor_algebra.h")).read()
+    assert "#define TRY_LATER -918273645" in text and "#define DEVICE_UNABLE -546372819" in text
+    assert ts.coh("M", "T", "T") == 26 and ts.coh("M", "T") == 6 and ts.coh("T", "T") == 10
+
+
+def test_product_parser_matches_reference_golden():
+    lib = ts.load()
+    rows = json.load(open(os.path.join(GOLDEN, "patterns.json")))["patterns"]
+    dig = (ctypes.c_int * 128)()
+    dr, lr, rr, cj = (ctypes.c_int() for _ in range(4))
+    for row in rows:
+        err = lib.talsh_get_contr_ptrn_str2dig(row["pattern"].encode(), dig, ctypes.byref(dr), ctypes.byref(lr),
+                                               ctypes.byref(rr), ctypes.byref(cj))
+        if row["err"] != 0:
+            assert err != 0, row["pattern"]
+            continue
+        assert err == 0, row["pattern"]
+        n = lr.value + rr.value
+        assert (dr.value, lr.value, rr.value, cj.value, list(dig[:n])) == (
+            row["drank"], row["lrank"], row["rrank"], row["conj"], row["dig"]), row["pattern"]
+
+
+def plan(pattern, dd, ld, rd, acc=1):
+    lib = ts.load()
+    mnk = (ctypes.c_longlong * 3)()
+    flags = (ctypes.c_int * 4)()
+    ops = (ctypes.c_char * 2)()
+    perms = (ctypes.c_int * 200)()
+    moved = ctypes.c_double()
+    ia = lambda a: (ctypes.c_int * max(1, len(a)))(*a)
+    err = lib.talshAmdPlanContraction(pattern.encode(), ia(dd), ia(ld), ia(rd), acc, mnk, flags, ops, perms,
+                                      ctypes.byref(moved))
+    return err, list(mnk), list(flags), ops.raw.decode(), list(perms), moved.value
+
+
+def execute_plan_numpy(oracle, dk, pattern, dd, D, ld, L, rd, R, alpha):
+    """Runs the planner's TTGT recipe with numpy matmul (as the GPU path does)."""
+    err, (M, N, K), (swap, at, bt, dt), ops, perms, moved = plan(pattern, dd, ld, rd)
+    assert err == 0
+    X, xd, Y, yd = (R, rd, L, ld) if swap else (L, ld, R, rd)
+    q = 0
+    pa = perms[q:q + len(xd)]; q += len(xd)
+    pb = perms[q:q + len(yd)]; q += len(yd)
+    pd = perms[q:q + len(dd)]
+    if at:
+        X = oracle.copy(dk, xd, [p + 1 for p in pa], X)
+    if bt:
+        Y = oracle.copy(dk, yd, [p + 1 for p in pb], Y)
+    A = X.reshape((K, M) if ops[0] == "T" else (M, K), order="F")
+    B = Y.reshape((K, N) if ops[1] == "N" else (N, K), order="F")
+    opA = A.T if ops[0] == "T" else A
+    opB = B if ops[1] == "N" else B.T
+    if dt:
+        Dp = oracle.copy(dk, dd, [p + 1 for p in pd], D)
+        dpd = [0] * len(dd)
+        for i, p in enumerate(pd):
+            dpd[p] = dd[i]
+    else:
+        Dp, dpd = D.copy(), dd
+    C = Dp.reshape((M, N), order="F") + alpha * (opA @ opB)
+    C = C.reshape(-1, order="F")
+    if dt:
+        inv = [0] * len(dd)
+        for i, p in enumerate(pd):
+            inv[p] = i
+        return oracle.copy(dk, dpd, [i + 1 for i in inv], C), moved
+    return C, moved
+
+
+def test_planner_recipes_reproduce_oracle(oracle):
+    rng = random.Random(17)
+    for t in range(150):
+        dk = R8
+        pat, dd, ld, rd = random_contraction(rng, max_rank=5, ext_lo=1, ext_hi=4, allow_scalar=True)
+        L, R, D = tensor(dk, ld, t), tensor(dk, rd, t + 1), tensor(dk, dd, t + 2)
+        got, moved = execute_plan_numpy(oracle, dk, pat, dd, D, ld, L, rd, R, 0.5)
+        want = oracle.contract(dk, pat, dd, D, ld, L, rd, R, (0.5, 0.0), 1)
+        assert np.max(np.abs(got - want)) <= 1e-13 * max(1.0, np.max(np.abs(want))), pat
+
+
+def test_planner_avoids_transposes_when_layout_allows():
+    # plain matrix products in every storage form need no copies
+    for pat in ("D(a,b)+=L(a,i)*R(i,b)", "D(a,b)+=L(i,a)*R(i,b)", "D(a,b)+=L(a,i)*R(b,i)",
+                "D(a,b)+=L(i,a)*R(b,i)", "D(b,a)+=L(a,i)*R(i,b)", "D(a,b,c)+=L(a,b,i)*R(i,c)"):
+        ld = {"D(a,b,c)+=L(a,b,i)*R(i,c)": [3, 4, 5]}.get(pat, [3, 5])
+        err, mnk, flags, ops, perms, moved = plan(pat, [3, 4, 6] if "c" in pat else [3, 4] if pat[2] == "a" else
+                                                  [4, 3], ld if "c" in pat else
+                                                  ([3, 5] if "L(a" in pat else [5, 3]),
+                                                  [5, 6] if "c" in pat else ([5, 4] if "R(i" in pat else [4, 5]))
+        assert err == 0, pat
+        assert moved == 0 and flags[1:] == [0, 0, 0], (pat, flags)
+    # the configs' patterns interleave indices: all three operands are transposed
+    err, mnk, flags, ops, perms, moved = plan("D(a,b,c,d)+=L(c,i,b,j)*R(d,j,a,i)", [32] * 4, [32] * 4, [32] * 4)
+    assert err == 0 and mnk == [1024, 1024, 1024] and flags[1:] == [1, 1, 1]
+
+
+def test_host_side_errors_without_gpu():
+    lib = ts.load()
+    hb, ham = ctypes.c_size_t(0), ctypes.c_int(0)
+    assert lib.talshInit(ctypes.byref(hb), ctypes.byref(ham), 0, None, 0, None, 0, None) == 0
+    try:
+        assert lib.talshInit(ctypes.byref(hb), ctypes.byref(ham), 0, None, 0, None, 0, None) == \
+            ts.TALSH_ALREADY_INITIALIZED
+        d, l, r = (ts.Tensor(R8, dims, 0) for dims in ([2, 3], [2, 4], [4, 3]))
+        # invalid pattern / shape mismatch are rejected before any device work
+        assert ts.contract("D(a,b)+=L(a,i)*R(i", d, l, r) == ts.TALSH_INVALID_ARGS
+        assert ts.contract("D(a,b)+=L(a,i)*R(b,i)", d, l, r) == ts.TALSH_INVALID_ARGS
+        # host execution is not provided by this build
+        assert ts.contract("D(a,b)+=L(a,i)*R(i,b)", d, l, r, dev_kind=ts.DEV_HOST, dev_id=0) == \
+            ts.TALSH_NOT_AVAILABLE
+        assert ts.contract("D(a,b)+=L(a,i)*R(i,b)", d, l, r) == ts.TALSH_NOT_AVAILABLE
+        # hyper-contraction indices are not on this path
+        dh, lh, rh = ts.Tensor(R8, [2, 3], 0), ts.Tensor(R8, [2, 4, 3], 0), ts.Tensor(R8, [4, 3], 0)
+        assert ts.contract("D(a,c)+=L(a,i,c)*R(i,c)", dh, lh, rh) == ts.DEVICE_UNABLE
+        # host images: import/export and scalar access
+        a = tensor(R8, [2, 3], 5)
+        d.upload(a)
+        assert np.array_equal(d.host_array(), a)
+        s = ts.Tensor(R8, [], 0, init_val=2.5)
+        assert s.scalar() == 2.5
+        assert lib.talshTensorRank(d.ptr) == 2 and lib.talshTensorVolume(d.ptr) == 6
+        # task handles
+        task = ts.Task()
+        assert lib.talshTaskIsEmpty(task.ptr) == ts.YEP
+        st = ctypes.c_int()
+        assert lib.talshTaskWait(task.ptr, ctypes.byref(st)) == ts.TALSH_OBJECT_IS_EMPTY
+        for t in (d, l, r, dh, lh, rh, s):
+            t.destruct()
+    finally:
+        assert lib.talshShutdown() == 0

@@ -1,0 +1,308 @@
+"""GPU parity of libtalsh_amd against the CPU oracle, through the C ABI.
+
+Copies, transposes, permuted adds, scale and init must be bit-exact; norms and
+contractions are compared with the oracle's long-double result:
+  R8/C8 contractions: normwise relative error <= 1e-12
+  R4/C4 contractions (FP32 MFMA): normwise relative error <= 1e-5
+  norm2: relative error <= 1e-12 (R8/C8), <= 1e-6 (R4/C4)
+"""
+import random
+
+import numpy as np
+import pytest
+
+from tal_common import C4, C8, R4, R8, NP_DTYPE, random_contraction, random_permutation_pattern, tensor
+
+pytestmark = pytest.mark.gpu
+
+KINDS = (R4, R8, C4, C8)
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def relerr(got, want):
+    den = np.linalg.norm(want.astype(np.complex128))
+    return np.linalg.norm((got - want).astype(np.complex128)) / (den if den > 0 else 1.0)
+
+
+def host(ts, dk, dims, data):
+    return ts.Tensor(dk, dims, 0, data=data)
+
+
+def on_gpu(ts, dk, dims, data):
+    """Tensor resident on GPU 0 holding `data` (uploaded through a Host tensor)."""
+    t = ts.Tensor(dk, dims, ts.gpu_flat(0))
+    h = host(ts, dk, dims, data)
+    lab = ",".join("i%d" % k for k in range(len(dims)))
+    ts.check(ts.copy("D(%s)=L(%s)" % (lab, lab), t, h), "upload")
+    h.destruct()
+    return t
+
+
+def perm_case(rng, rank, lo, hi):
+    pat, perm, labels = random_permutation_pattern(rng, rank)
+    ldims = [rng.randint(lo, hi) for _ in range(rank)]
+    o2n = [labels.index(x) + 1 for x in perm]  # L dim k (label perm[k]) goes to D position
+    ddims = [0] * rank
+    for k, t in enumerate(o2n):
+        ddims[t - 1] = ldims[k]
+    return pat, ldims, ddims, o2n
+
+
+def test_copy_small_all_kinds(talsh_gpu, oracle):
+    ts = talsh_gpu
+    rng = random.Random(1)
+    for dk in KINDS:
+        for t in range(25):
+            rank = rng.randint(1, 8)
+            pat, ld, dd, o2n = perm_case(rng, rank, 1, 7)
+            say("copy", dk, pat, ld)
+            L = tensor(dk, ld, 10 + t)
+            lt = on_gpu(ts, dk, ld, L)
+            dt = ts.Tensor(dk, dd, ts.gpu_flat(0))
+            ts.check(ts.copy(pat, dt, lt), "copy")
+            assert dt.host_array().tobytes() == oracle.copy(dk, ld, o2n, L).tobytes(), (dk, pat, ld)
+
+
+def test_copy_conjugate_and_host_operands(talsh_gpu, oracle):
+    ts = talsh_gpu
+    for dk in (C4, C8):
+        say("copy conj", dk)
+        L = tensor(dk, [5, 6, 7], 3)
+        lh = host(ts, dk, [5, 6, 7], L)
+        dh = ts.Tensor(dk, [7, 5, 6], 0)
+        # both operands on Host: executed on the GPU, result copied back (COPY_TT)
+        ts.check(ts.copy("D(c,a,b)=L+(a,b,c)", dh, lh, copy_ctrl=ts.coh("T", "T")), "copy")
+        assert dh.images() == [0] and lh.images() == [0]
+        assert dh.host_array().tobytes() == oracle.copy(dk, [5, 6, 7], [2, 3, 1], L, conj=1).tobytes()
+
+
+def test_transpose_config2_bit_exact(talsh_gpu, oracle):
+    """rank-6 R8 16^6: identity, full reversal and 32 seeded random permutations."""
+    ts = talsh_gpu
+    dims = [16] * 6
+    L = tensor(R8, dims, 2026)
+    lt = on_gpu(ts, R8, dims, L)
+    rng = random.Random(1234)
+    labels = list("abcdef")
+    perms = [labels[:], labels[::-1]]
+    while len(perms) < 34:
+        p = labels[:]
+        rng.shuffle(p)
+        if p not in perms:
+            perms.append(p)
+    dt = ts.Tensor(R8, dims, ts.gpu_flat(0))
+    for p in perms:
+        pat = "D(a,b,c,d,e,f)=L(%s)" % ",".join(p)
+        say("config2 copy", pat)
+        ts.check(ts.copy(pat, dt, lt), "copy")
+        o2n = [labels.index(x) + 1 for x in p]
+        got = dt.host_array()
+        assert got.tobytes() == oracle.copy(R8, dims, o2n, L).tobytes(), pat
+        dt.place(ts.gpu_flat(0))
+
+
+def test_add_bit_exact(talsh_gpu, oracle):
+    ts = talsh_gpu
+    rng = random.Random(2)
+    for dk in KINDS:
+        for t in range(12):
+            pat, ld, dd, o2n = perm_case(rng, rng.randint(1, 6), 1, 7)
+            pat = pat.replace(")=L", ")+=L")
+            alpha = (0.37, -0.21) if dk in (C4, C8) else (0.37, 0.0)
+            say("add", dk, pat, ld)
+            L, D = tensor(dk, ld, 40 + t), tensor(dk, dd, 80 + t)
+            lt, dt = on_gpu(ts, dk, ld, L), on_gpu(ts, dk, dd, D)
+            ts.check(ts.add(pat, dt, lt, alpha=complex(*alpha)), "add")
+            want = oracle.add(dk, ld, o2n, L, D, alpha)
+            assert dt.host_array().tobytes() == want.tobytes(), (dk, pat)
+
+
+def test_add_config2_half(talsh_gpu, oracle):
+    ts = talsh_gpu
+    dims = [16] * 6
+    L, D = tensor(R8, dims, 7), tensor(R8, dims, 8)
+    lt, dt = on_gpu(ts, R8, dims, L), on_gpu(ts, R8, dims, D)
+    say("config2 add D+=0.5*perm(L)")
+    ts.check(ts.add("D(a,b,c,d,e,f)+=L(c,a,f,b,e,d)", dt, lt, alpha=0.5), "add")
+    o2n = [list("abcdef").index(x) + 1 for x in "cafbed"]
+    assert dt.host_array().tobytes() == oracle.add(R8, dims, o2n, L, D, (0.5, 0.0)).tobytes()
+
+
+def test_scale_init_bit_exact(talsh_gpu, oracle):
+    ts = talsh_gpu
+    for dk in KINDS:
+        say("scale/init", dk)
+        D = tensor(dk, [9, 10, 11], 5)
+        dt = on_gpu(ts, dk, [9, 10, 11], D)
+        alpha = (-1.25, 0.5) if dk in (C4, C8) else (-1.25, 0.0)
+        ts.check(ts.scale(dt, complex(*alpha)), "scale")
+        assert dt.host_array().tobytes() == oracle.scale(dk, D, alpha).tobytes()
+        dt.place(ts.gpu_flat(0))
+        ts.check(ts.init_value(dt, complex(0.75, -0.5) if dk in (C4, C8) else 0.75), "init")
+        want = np.full(D.size, complex(0.75, -0.5) if dk in (C4, C8) else 0.75).astype(NP_DTYPE[dk])
+        assert dt.host_array().tobytes() == want.tobytes()
+
+
+@pytest.mark.parametrize("dk", KINDS)
+def test_norm2(talsh_gpu, oracle, dk):
+    ts = talsh_gpu
+    for dims in ([1], [7], [1000, 3], [16] * 6 if dk == R8 else [64, 64, 64]):
+        say("norm2", dk, dims)
+        L = tensor(dk, dims, 11)
+        lt = on_gpu(ts, dk, dims, L)
+        d = ts.Tensor(dk, [], ts.gpu_flat(0))
+        ts.check(ts.norm2(d, lt), "norm2")
+        got = d.scalar().real
+        want = oracle.norm2(dk, L)
+        tol = 1e-12 if dk in (R8, C8) else 1e-6
+        assert abs(got - want) <= tol * want, (dk, dims, got, want)
+
+
+def test_contract_config1_r8(talsh_gpu, oracle):
+    """D(a,b,c,d)+=L(c,i,b,j)*R(d,j,a,i), extents 32, R8, alpha 1, accumulative."""
+    ts = talsh_gpu
+    pat, n = "D(a,b,c,d)+=L(c,i,b,j)*R(d,j,a,i)", 32
+    dims = [n] * 4
+    L, R, D = tensor(R8, dims, 1), tensor(R8, dims, 2), tensor(R8, dims, 3)
+    lt, rt, dt = on_gpu(ts, R8, dims, L), on_gpu(ts, R8, dims, R), on_gpu(ts, R8, dims, D)
+    say("config1 contraction")
+    ts.check(ts.contract(pat, dt, lt, rt, alpha=1.0), "contract")
+    want = oracle.contract(R8, pat, dims, D, dims, L, dims, R, (1.0, 0.0), 1)
+    e = relerr(dt.host_array(), want)
+    print("config1 normwise relative error %.3e" % e)
+    assert e <= 1e-12
+
+
+def test_contract_random_all_kinds(talsh_gpu, oracle):
+    ts = talsh_gpu
+    rng = random.Random(99)
+    for dk in KINDS:
+        for t in range(30):
+            pat, dd, ld, rd = random_contraction(rng, max_rank=4, ext_lo=1, ext_hi=9, allow_scalar=(t % 5 == 0))
+            if dk in (C4, C8) and t % 3 == 1:  # conjugated operands
+                pat = pat.replace("L(", "L+(") if t % 2 else pat.replace("R(", "R+(")
+            acc = t % 2
+            alpha = complex(0.5, 0.25) if dk in (C4, C8) else 0.75
+            say("contract", dk, pat, dd, ld, rd, "acc", acc)
+            L, R, D = tensor(dk, ld, t), tensor(dk, rd, t + 1), tensor(dk, dd, t + 2)
+            lt, rt, dt = on_gpu(ts, dk, ld, L), on_gpu(ts, dk, rd, R), on_gpu(ts, dk, dd, D)
+            ts.check(ts.contract(pat, dt, lt, rt, alpha=alpha, accumulative=acc), "contract")
+            want = oracle.contract(dk, pat, dd, D, ld, L, rd, R, (alpha.real, alpha.imag) if dk in (C4, C8)
+                                   else (alpha, 0.0), acc)
+            tol = 1e-12 if dk in (R8, C8) else 1e-5
+            assert relerr(dt.host_array(), want) <= tol, (dk, pat)
+
+
+def test_contract_reference_shapes(talsh_gpu, oracle):
+    """A sample of the reference benchmark shapes (tensor_contractions.txt), reduced in size."""
+    import json
+    import os
+
+    ts = talsh_gpu
+    rows = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "patterns.json")))
+    for s in rows["tensor_contractions_txt"][::4]:
+        scale = lambda v: [max(2, x // 12) for x in v]
+        dd, ld, rd = scale(s["ddims"]), scale(s["ldims"]), scale(s["rdims"])
+        # keep the extents consistent across operands
+        err, drk, lrk, rrk, cj, dig = oracle.parse(s["pattern"])
+        ext = {}
+        for i, x in enumerate(dig[:lrk]):
+            if x > 0:
+                ext[("d", x - 1)] = ld[i]
+        for j, x in enumerate(dig[lrk:]):
+            if x > 0:
+                ext[("d", x - 1)] = rd[j]
+            else:
+                rd[j] = ld[-x - 1]
+        dd = [ext[("d", k)] for k in range(drk)]
+        say("reference shape", s["pattern"], dd, ld, rd)
+        L, R, D = tensor(R8, ld, 1), tensor(R8, rd, 2), tensor(R8, dd, 3)
+        lt, rt, dt = on_gpu(ts, R8, ld, L), on_gpu(ts, R8, rd, R), on_gpu(ts, R8, dd, D)
+        ts.check(ts.contract(s["pattern"], dt, lt, rt), "contract")
+        want = oracle.contract(R8, s["pattern"], dd, D, ld, L, rd, R)
+        assert relerr(dt.host_array(), want) <= 1e-12, s["pattern"]
+
+
+def test_contract_host_operands_coherence(talsh_gpu, oracle):
+    """Operands on Host; COPY_MTT moves D to the GPU, COPY_TTT returns it to Host."""
+    ts = talsh_gpu
+    pat = "D(a,b,i,j)+=L(c,b,d,a)*R(j,d,i,c)"
+    dd, ld, rd = [8, 8, 6, 6], [8, 8, 8, 8], [6, 8, 6, 8]
+    L, R, D = tensor(R8, ld, 4), tensor(R8, rd, 5), tensor(R8, dd, 6)
+    want = oracle.contract(R8, pat, dd, D, ld, L, rd, R, (2.0, 0.0), 1)
+    for c, d_on_gpu in ((ts.coh("M", "T", "T"), True), (ts.coh("T", "T", "T"), False),
+                        (ts.coh("K", "K", "T"), None)):
+        say("coherence", c)
+        lh, rh, dh = host(ts, R8, ld, L), host(ts, R8, rd, R), host(ts, R8, dd, D)
+        ts.check(ts.contract(pat, dh, lh, rh, alpha=2.0, copy_ctrl=c), "contract")
+        imgs = dh.images()
+        if d_on_gpu is True:
+            assert imgs == [ts.gpu_flat(0)]
+        elif d_on_gpu is False:
+            assert imgs == [0]
+        else:
+            assert sorted(imgs) == [0, ts.gpu_flat(0)] and sorted(lh.images()) == [0, ts.gpu_flat(0)]
+        assert rh.images() == [0]
+        assert relerr(dh.host_array(), want) <= 1e-12
+
+
+def test_async_tasks_and_timing(talsh_gpu, oracle):
+    ts = talsh_gpu
+    pat = "D(a,b,c,d)+=L(c,i,b,j)*R(d,j,a,i)"
+    dims = [16] * 4
+    L, R = tensor(R8, dims, 1), tensor(R8, dims, 2)
+    lt, rt = on_gpu(ts, R8, dims, L), on_gpu(ts, R8, dims, R)
+    outs, tasks = [], []
+    for k in range(6):
+        dt = on_gpu(ts, R8, dims, np.zeros(16 ** 4))
+        tk = ts.Task()
+        say("async task", k)
+        ts.check(ts.contract(pat, dt, lt, rt, alpha=float(k + 1), task=tk), "contract async")
+        assert ts.load().talshTaskIsEmpty(tk.ptr) == ts.NOPE
+        outs.append(dt)
+        tasks.append(tk)
+    # a scheduled task's output is in use; a non-empty task handle is refused
+    assert ts.contract(pat, outs[0], lt, rt, task=tasks[0]) == ts.TALSH_OBJECT_NOT_EMPTY
+    want = oracle.contract(R8, pat, dims, np.zeros(16 ** 4), dims, L, dims, R, (1.0, 0.0), 1)
+    for k, (dt, tk) in enumerate(zip(outs, tasks)):
+        done = 0
+        while not done:
+            done, st, ierr = tk.complete()
+        assert st == ts.TALSH_TASK_COMPLETED and ierr == 0
+        tm = tk.times()
+        assert tm["total"] > 0 and tm["mmul"] > 0 and tm["total"] >= tm["mmul"]
+        assert tk.t.flops == 2.0 * 256 ** 3
+        assert tk.destruct() == 0
+        assert relerr(dt.host_array(), (k + 1) * want) <= 1e-12
+
+
+def test_pool_limits(talsh_gpu):
+    import ctypes
+    import gc
+
+    ts = talsh_gpu
+    lib = ts.load()
+    gc.collect()
+    size = lib.talshDeviceBufferSize(0, ts.DEV_NVIDIA_GPU)
+    free0 = lib.talshDeviceBufferFreeSize(0, ts.DEV_NVIDIA_GPU)
+    assert size > 0 and free0 <= size
+    say("pool limits", size, free0)
+
+    def construct(t, cols):
+        lib.talshTensorClean(ctypes.byref(t))
+        dims = (ctypes.c_int * 2)(1024, cols)
+        return lib.talshTensorConstruct(ctypes.byref(t), R8, 2, dims, ts.gpu_flat(0), None, -1, None, 0.0, 0.0)
+
+    # larger than the whole pool: DEVICE_UNABLE (not an error, nothing allocated)
+    t = ts.TalshTens()
+    assert construct(t, int(size // 8 // 1024 + 1024)) == ts.DEVICE_UNABLE
+    # two tensors of 60 % of the free pool: the second one has to wait (TRY_LATER)
+    a, b = ts.TalshTens(), ts.TalshTens()
+    m = int(0.6 * free0 / 8 / 1024)
+    assert construct(a, m) == 0
+    assert construct(b, m) == ts.TRY_LATER
+    assert lib.talshTensorDestruct(ctypes.byref(a)) == 0
+    assert lib.talshDeviceBufferFreeSize(0, ts.DEV_NVIDIA_GPU) == free0
