@@ -1,0 +1,277 @@
+"""Benchmark of the TAL-SH contraction hot path on one MI355X.
+
+Headline (`value`): R8 contraction TFLOP/s of the large dense contraction
+    D(a,b,c,d) += L(a,i,c,j) * R(j,b,i,d), all extents 96
+(84.9 M elements per tensor, TTGT GEMM M=N=K=9216, 1.5655e12 FLOP per step),
+through talshTensorContract on GPU-resident tensors; one step = one
+contraction (input transposes + FP64 MFMA GEMM + output transpose).
+Secondary figures on the same line: the same contraction in R4 and C8, the
+32^4 contraction, and the rank-6 16^6 transpose / permuted-add / norm2 sweep
+(34 permutations) as GB/s.  `roofline` is computed for the FP64 GEMM kernel
+from its HIP-event duration on the task stream (talshTaskTime "mmul").
+`cpu_baseline` times TAL-SH's own CPU path (oracle/_ref, built from the
+reference sources) on a bounded sample of the same contraction.
+"""
+import argparse
+import ctypes
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+sys.path.insert(0, os.path.join(REPO, "tests"))
+
+import tal_sh_amd as ts  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
+BF16_PEAK_TFLOPS = 2500.0
+
+R4, R8, C4, C8 = ts.R4, ts.R8, ts.C4, ts.C8
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def uniform(seed, n):
+    from tal_common import uniform as u
+    return u(seed, n)
+
+
+def gpu_tensor(dk, dims, seed):
+    """GPU-resident tensor with seeded uniform[-1,1] data (uploaded once)."""
+    n = int(np.prod(dims))
+    if dk in (R4, R8):
+        data = uniform(seed, n).astype(ts.NP_DTYPE[dk])
+    else:
+        data = (uniform(seed, n) + 1j * uniform(seed + 1, n)).astype(ts.NP_DTYPE[dk])
+    t = ts.Tensor(dk, dims, ts.gpu_flat(0))
+    h = ts.Tensor(dk, dims, 0, data=data)
+    lab = ",".join("i%d" % k for k in range(len(dims)))
+    ts.check(ts.copy("D(%s)=L(%s)" % (lab, lab), t, h), "upload")
+    h.destruct()
+    return t, data
+
+
+def timed_contractions(pat, d, l, r, steps, warmup, alpha=1.0):
+    lib = ts.load()
+    for _ in range(warmup):
+        ts.check(ts.contract(pat, d, l, r, alpha=alpha), "contract")
+    lib.talshAmdDeviceSynchronize(0)
+    mm, tot = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        tk = ts.Task()
+        ts.check(ts.contract(pat, d, l, r, alpha=alpha, task=tk), "contract")
+        tk.wait()
+        tm = tk.times()
+        mm.append(tm["mmul"])
+        tot.append(tm["total"])
+        tk.destruct()
+    lib.talshAmdDeviceSynchronize(0)
+    wall = time.perf_counter() - t0
+    return wall, mm, tot
+
+
+def sampled_error(oracle, dk, pat, dims, D0, L, R, Dout, alpha, nsamples=64, seed=5):
+    """Normwise relative error on sampled elements (each a K-term dot product in long double)."""
+    rng = np.random.default_rng(seed)
+    idx = rng.choice(D0.size, size=nsamples, replace=False)
+    num = den = 0.0
+    for off in idx:
+        el = oracle.contract_element(dk, pat, dims, int(off), dims, L, dims, R)
+        want = complex(D0[off]) + alpha * el
+        num += abs(complex(Dout[off]) - want) ** 2
+        den += abs(want) ** 2
+    return float(np.sqrt(num / den))
+
+
+def bench_contraction(oracle, dk, pat, n, steps, warmup, check=True, seed=100):
+    dims = [n] * 4
+    log("contraction", dk, pat, dims)
+    l, L = gpu_tensor(dk, dims, seed)
+    r, R = gpu_tensor(dk, dims, seed + 2)
+    d, D0 = gpu_tensor(dk, dims, seed + 4)
+    wall, mm, tot = timed_contractions(pat, d, l, r, steps, warmup)
+    flop = (8.0 if dk in (C4, C8) else 2.0) * float(n) ** 6
+    res = {"tflops": flop * steps / wall / 1e12, "ms_per_step": 1e3 * wall / steps,
+           "gemm_ms": 1e3 * float(np.mean(mm)), "gemm_tflops": flop / float(np.mean(mm)) / 1e12,
+           "task_ms": 1e3 * float(np.mean(tot)), "flop_per_step": flop}
+    if check:
+        # D now holds D0 + (warmup+steps) * L*R: check against the oracle on samples
+        out = d.host_array()
+        reps = warmup + steps
+        res["rel_err_vs_fp64"] = sampled_error(oracle, dk, pat, dims, D0, L, R, out, float(reps))
+    for t in (l, r, d):
+        t.destruct()
+    return res
+
+
+def bench_transposes(oracle, steps):
+    """Config 2: rank-6 R8 16^6, identity + reversal + 32 seeded permutations; add; norm2."""
+    import random
+
+    dims = [16] * 6
+    nbytes = 16 ** 6 * 8
+    l, L = gpu_tensor(R8, dims, 2026)
+    d = ts.Tensor(R8, dims, ts.gpu_flat(0))
+    labels = list("abcdef")
+    rng = random.Random(1234)
+    perms = [labels[:], labels[::-1]]
+    while len(perms) < 34:
+        p = labels[:]
+        rng.shuffle(p)
+        if p not in perms:
+            perms.append(p)
+    lib = ts.load()
+    per = []
+    exact = True
+    for p in perms:
+        pat = "D(a,b,c,d,e,f)=L(%s)" % ",".join(p)
+        ts.check(ts.copy(pat, d, l), "copy")  # warm-up
+        tms = []
+        for _ in range(steps):
+            tk = ts.Task()
+            ts.check(ts.copy(pat, d, l, task=tk), "copy")
+            tk.wait()
+            tms.append(tk.times()["comput"])
+            tk.destruct()
+        per.append(float(np.median(tms)))
+    # bit-exact check of the last permutation
+    o2n = [labels.index(x) + 1 for x in perms[-1]]
+    exact = d.host_array().tobytes() == oracle.copy(R8, dims, o2n, L).tobytes()
+    d.place(ts.gpu_flat(0))
+    copy_gbs = [2 * nbytes / t / 1e9 for t in per]
+    # permuted add D += 0.5 * perm(L)
+    tms = []
+    for _ in range(steps + 1):
+        tk = ts.Task()
+        ts.check(ts.add("D(a,b,c,d,e,f)+=L(c,a,f,b,e,d)", d, l, alpha=0.5, task=tk), "add")
+        tk.wait()
+        tms.append(tk.times()["comput"])
+        tk.destruct()
+    add_gbs = 3 * nbytes / float(np.median(tms[1:])) / 1e9
+    # norm2
+    s = ts.Tensor(R8, [], ts.gpu_flat(0))
+    tms = []
+    for _ in range(steps + 1):
+        tk = ts.Task()
+        ts.check(ts.norm2(s, l, copy_ctrl=ts.coh("M", "T"), task=tk), "norm2")
+        tk.wait()
+        tms.append(tk.times()["comput"])
+        tk.destruct()
+    norm_gbs = nbytes / float(np.median(tms[1:])) / 1e9
+    for t in (l, d, s):
+        t.destruct()
+    lib.talshAmdDeviceSynchronize(0)
+    return {"copy_gbs_mean": float(np.mean(copy_gbs)), "copy_gbs_min": float(np.min(copy_gbs)),
+            "copy_gbs_identity": copy_gbs[0], "copy_gbs_reversal": copy_gbs[1],
+            "copy_frac_hbm_mean": float(np.mean(copy_gbs)) / HBM_PEAK_GBS, "copy_bit_exact": bool(exact),
+            "add_gbs": add_gbs, "norm2_gbs": norm_gbs, "permutations": len(perms)}
+
+
+def cpu_baseline(budget_s=30.0):
+    """TAL-SH's own CPU path on a bounded sample of the headline contraction.
+
+    Sample: the same pattern with extents a=b=i=j=96 and c=d=32 (M=N=3072,
+    K=9216 kept whole): 1.74e11 FLOP, 1/9 of one full step."""
+    from tal_common import REF_SO, Oracle, Reference
+
+    pat = "D(a,b,c,d)+=L(a,i,c,j)*R(j,b,i,d)"
+    dd, ld, rd = [96, 96, 32, 32], [96, 96, 32, 96], [96, 96, 96, 32]
+    L = uniform(11, int(np.prod(ld)))
+    R = uniform(12, int(np.prod(rd)))
+    D = uniform(13, int(np.prod(dd)))
+    flop = 2.0 * 3072 * 3072 * 9216
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    if os.path.exists(REF_SO):
+        ref = Reference()
+        kind = "reference"
+        run = lambda: ref.contract(R8, pat, dd, D, ld, L, rd, R)
+    else:
+        orc = Oracle()
+        kind = "port"
+        run = lambda: orc.contract(R8, pat, dd, D, ld, L, rd, R)
+    times = []
+    t_start = time.perf_counter()
+    while len(times) < 3 and time.perf_counter() - t_start < budget_s:
+        t0 = time.perf_counter()
+        run()
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    return {"value": flop / t / 1e12, "unit": "TFLOP/s", "cores": cores, "kind": kind,
+            "sample": "D(a,b,c,d)+=L(a,i,c,j)*R(j,b,i,d) R8 with c=d=32, others 96 (M=N=3072, K=9216; "
+                      "1.74e11 FLOP), median of %d runs, %.2f s each, OMP threads %d" % (len(times), t, cores)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="headline only")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1).", file=sys.stderr)
+        sys.exit(2)
+    from tal_common import Oracle
+
+    oracle = Oracle()
+    err = ts.init(host_buf_size=4 << 30, dev_buffer_size=48 << 30)
+    if err:
+        raise RuntimeError("talshInit failed: %d" % err)
+    pat3 = "D(a,b,c,d)+=L(a,i,c,j)*R(j,b,i,d)"
+    head = bench_contraction(oracle, R8, pat3, 96, args.steps, args.warmup)
+    log("headline", head)
+    secondary = {}
+    if not args.quick:
+        secondary["r4_fp32_config3"] = bench_contraction(oracle, R4, pat3, 96, max(2, args.steps // 2), 1)
+        ts.load().talshEnableFastMath(ts.DEV_NVIDIA_GPU, 0)
+        if ts.load().talshQueryFastMath(ts.DEV_NVIDIA_GPU, 0):
+            secondary["r4_split_bf16_config3"] = bench_contraction(oracle, R4, pat3, 96, max(2, args.steps // 2), 1)
+        ts.load().talshDisableFastMath(ts.DEV_NVIDIA_GPU, 0)
+        secondary["c8_config3"] = bench_contraction(oracle, C8, pat3, 96, max(2, args.steps // 4), 1)
+        secondary["r8_config1"] = bench_contraction(oracle, R8, "D(a,b,c,d)+=L(c,i,b,j)*R(d,j,a,i)", 32,
+                                                    50, 5)
+        secondary["transpose_config2"] = bench_transposes(oracle, 5)
+        log("secondary", secondary)
+    ts.shutdown()
+    cpu = None if args.no_cpu else cpu_baseline()
+    gemm_s = head["gemm_ms"] * 1e-3
+    achieved = head["flop_per_step"] / gemm_s / 1e12
+    line = {
+        "metric": "R8/R4 contraction TFLOP/s (%MFMA peak); transpose GB/s (%HBM); p99 task latency",
+        "value": round(head["tflops"], 3),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded uniform[-1,1])",
+        "config": {"workload": "large dense contraction D(a,b,c,d)+=L(a,i,c,j)*R(j,b,i,d), R8, extents 96 "
+                               "(GEMM M=N=K=9216), GPU-resident operands, accumulative, alpha 1",
+                   "parallelism": "single GPU"},
+        "rel_err_vs_fp64": head.get("rel_err_vs_fp64"),
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                     "kernel": "FP64 MFMA GEMM (2*M*N*K FLOP per launch / HIP-event duration)"},
+        "cpu_baseline": cpu,
+        "secondary": secondary,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

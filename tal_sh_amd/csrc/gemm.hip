@@ -224,6 +224,8 @@ int op_code(char t) {
 
 }  // namespace
 
+bool gemm_split_bf16_available() { return false; }
+
 int launch_gemm(int dk, int r4mode, char ta, char tb, int conja, int conjb, int64_t M, int64_t N, int64_t K,
                 double ar, double ai, const void *A, int64_t lda, const void *B, int64_t ldb, int beta_one, void *C,
                 int64_t ldc, hipStream_t s) {

@@ -30,6 +30,9 @@ int launch_norm2(int dk, const void *src, size_t vol, double *partials, void *ou
 // GEMM precision modes for real single precision.
 enum R4Mode { R4_FP32 = 0, R4_SPLIT_BF16 = 1 };
 
+// Whether the split-BF16 R4 GEMM is built into this library.
+bool gemm_split_bf16_available();
+
 // C(MxN, col-major, ldc) = alpha * op(A) * op(B) + beta * C, beta in {0,1}.
 // ta: 'N' A is MxK (lda>=M); 'T' A is KxM (lda>=K).
 // tb: 'N' B is KxN (ldb>=K); 'T' B is NxK (ldb>=N).

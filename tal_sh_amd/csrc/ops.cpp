@@ -697,6 +697,7 @@ int talshTensorDiscardOther_(talsh_tens_t *tens, int dev_id, int dev_kind) {
 int talshEnableFastMath(int dev_kind, int dev_id) {
   (void)dev_kind;
   (void)dev_id;
+  if (!gemm_split_bf16_available()) return TALSH_NOT_AVAILABLE;
   rt().r4mode = R4_SPLIT_BF16;
   return TALSH_SUCCESS;
 }
