@@ -218,6 +218,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only")
+    ap.add_argument("--only", default="", help="comma list of secondary parts to run (r4,bf16,c8,config1,transpose)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1).", file=sys.stderr)
@@ -229,20 +230,29 @@ def main():
     if err:
         raise RuntimeError("talshInit failed: %d" % err)
     pat3 = "D(a,b,c,d)+=L(a,i,c,j)*R(j,b,i,d)"
-    head = bench_contraction(oracle, R8, pat3, 96, args.steps, args.warmup)
+    only = set(x for x in args.only.split(",") if x)
+    want = lambda part: (not args.quick and not only) or part in only
+    head = bench_contraction(oracle, R8, pat3, 96, args.steps, args.warmup) if (not only or "head" in only) else {}
     log("headline", head)
     secondary = {}
-    if not args.quick:
+    if want("r4"):
         secondary["r4_fp32_config3"] = bench_contraction(oracle, R4, pat3, 96, max(2, args.steps // 2), 1)
+    if want("bf16"):
         ts.load().talshEnableFastMath(ts.DEV_NVIDIA_GPU, 0)
         if ts.load().talshQueryFastMath(ts.DEV_NVIDIA_GPU, 0):
             secondary["r4_split_bf16_config3"] = bench_contraction(oracle, R4, pat3, 96, max(2, args.steps // 2), 1)
         ts.load().talshDisableFastMath(ts.DEV_NVIDIA_GPU, 0)
+    if want("c8"):
         secondary["c8_config3"] = bench_contraction(oracle, C8, pat3, 96, max(2, args.steps // 4), 1)
-        secondary["r8_config1"] = bench_contraction(oracle, R8, "D(a,b,c,d)+=L(c,i,b,j)*R(d,j,a,i)", 32,
-                                                    50, 5)
+    if want("config1"):
+        secondary["r8_config1"] = bench_contraction(oracle, R8, "D(a,b,c,d)+=L(c,i,b,j)*R(d,j,a,i)", 32, 50, 5)
+    if want("transpose"):
         secondary["transpose_config2"] = bench_transposes(oracle, 5)
-        log("secondary", secondary)
+    log("secondary", secondary)
+    if not head:
+        ts.shutdown()
+        print(json.dumps({"secondary": secondary}), flush=True)
+        return
     ts.shutdown()
     cpu = None if args.no_cpu else cpu_baseline()
     gemm_s = head["gemm_ms"] * 1e-3

@@ -4,20 +4,24 @@
 // (tensor_algebra_gpu_nvidia.cu:6851-7060, cuBLAS/hipBLAS) and its in-house
 // fallback gpu_matrix_multiply_tn__ (:2263-2312).  Contract:
 //   C(MxN, column-major, ldc) = alpha * op(A) * op(B) + beta * C,  beta in {0,1}
-// with op(A) in {A, A^T} and op(B) in {B, B^T}, each optionally conjugated, so that the
-// contraction planner can feed tensor operands in whichever matrix form their
-// storage already has.
+// with op(A) in {A, A^T} and op(B) in {B, B^T}, each optionally conjugated,
+// so that the contraction planner can feed tensor operands in whichever
+// matrix form their storage already has.
 //
 // Kernel shape: 256 threads = 4 wave64s in a 2x2 grid; each wave owns a
 // (BM/2)x(BN/2) slab of C built from 16x16 MFMA tiles:
 //   R8/C8: v_mfma_f64_16x16x4_f64   R4/C4: v_mfma_f32_16x16x4_f32
-// Operand tiles of BK=16 are staged through LDS in K-contiguous rows padded by
-// one element; the next K tile is prefetched into registers while the current
-// one is multiplied.  Complex products use four real MFMAs per step
-// (re += ar*br - ai*bi, im += ar*bi + ai*br), with conjugation folded into the
-// sign of the imaginary operand.  MFMA fragment layout (16x16x4): A lane l
-// holds A[l%16][l/16], B lane l holds B[l/16][l%16]; accumulator register r of
-// lane l holds C[4*(l/16)+r][l%16] for f32 and C[4*r+l/16][l%16] for f64.
+// Operand tiles (BK=16) are double-buffered in LDS as K-contiguous rows padded
+// by one 16-byte vector (conflict-free fragment reads, aligned rows); the next
+// tile is loaded from HBM into registers with 16-byte accesses while the
+// current one is multiplied, so each K step needs a single barrier.  Blocks
+// are renumbered so that each XCD (dispatch round-robins workgroups over the
+// 8 XCDs) walks a compact band of C tiles and the operand panels it streams
+// stay in its own L2.  Complex products use four real MFMAs per step
+// (re += ar*br - ai*bi, im += ar*bi + ai*br), conjugation folded into the sign
+// of the imaginary operand.  MFMA fragment layout (16x16x4): A lane l holds
+// A[l%16][l/16], B lane l holds B[l/16][l%16]; accumulator register r of lane
+// l holds C[4*(l/16)+r][l%16] for f32 and C[4*r+l/16][l%16] for f64.
 #include "kernels.hpp"
 
 #include <hip/hip_runtime.h>
@@ -48,65 +52,134 @@ struct Mfma<float> {
 
 constexpr int kThreads = 256;
 constexpr int BK = 16;
-constexpr int PAD = 1;
+constexpr int kXcds = 8;
 
-// layout codes: 0 = N, 1 = T; CJA/CJB conjugate the complex operand
-template <typename S, int NC, int OPA, int OPB, int CJA, int CJB, int BM, int BN>
-__global__ __launch_bounds__(kThreads) void gemm_kernel(int64_t M, int64_t N, int64_t K, const S *__restrict__ A,
+// W scalars moved as one access (W*sizeof(S) is 16 bytes when vectorised).
+template <typename S, int W>
+struct Vec {
+  S v[W];
+};
+template <typename S, int W>
+__device__ inline Vec<S, W> ld(const S *p) {
+  Vec<S, W> r;
+  if constexpr (sizeof(S) * W == 16) {
+    const uint4 u = *reinterpret_cast<const uint4 *>(p);
+    __builtin_memcpy(&r, &u, 16);
+  } else {
+#pragma unroll
+    for (int i = 0; i < W; ++i) r.v[i] = p[i];
+  }
+  return r;
+}
+
+// Maps the launch index to a C tile so that consecutive tiles of one XCD are
+// neighbours: XCD x (= linear id % 8) gets the x-th contiguous slice of the
+// tile sequence, walked in groups of 8 tile-rows.
+__device__ inline void tile_of(int64_t gx, int64_t gy, int64_t &bx, int64_t &by) {
+  const int64_t total = gx * gy;
+  const int64_t id = (int64_t)blockIdx.x;
+  int64_t t = id;
+  if (total >= kXcds * 4 && total % kXcds == 0) t = (id % kXcds) * (total / kXcds) + id / kXcds;
+  constexpr int64_t G = 8;  // tile-rows per group
+  const int64_t group = t / (G * gy);
+  const int64_t first = group * G;
+  const int64_t rows = min(G, gx - first);
+  const int64_t in = t - group * G * gy;
+  bx = first + in % rows;
+  by = in / rows;
+}
+
+// layout codes: 0 = N, 1 = T; CJA/CJB conjugate the complex operand; W is the
+// vector width in scalars along each operand's contiguous axis.
+template <typename S, int NC, int OPA, int OPB, int CJA, int CJB, int BM, int BN, int W>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void gemm_kernel(int64_t M, int64_t N, int64_t K, const S *__restrict__ A,
                                                         int64_t lda, const S *__restrict__ B, int64_t ldb,
                                                         S *__restrict__ C, int64_t ldc, S alpha_r, S alpha_i,
-                                                        int beta_one) {
+                                                        int beta_one, int64_t gx, int64_t gy) {
   using MF = Mfma<S>;
   using acc_t = typename MF::acc_t;
   constexpr int SM = BM / 32, SN = BN / 32;  // 16x16 tiles per wave along M, N
-  constexpr int EA = BM * BK / kThreads, EB = BN * BK / kThreads;
-  static_assert(EA >= 1 && EB >= 1, "tile too small");
-  __shared__ S As[NC][BM][BK + PAD];
-  __shared__ S Bs[NC][BN][BK + PAD];
+  constexpr int PAD = W;                     // keeps rows 16-byte aligned
+  constexpr int RS = BK + PAD;               // LDS row stride (scalars)
+  // A tile: BM x BK elements (BM*BK*NC scalars) moved in vectors of W scalars
+  // along the contiguous axis (m for 'N', k for 'T'; re/im interleaved).
+  constexpr int ACONT = (OPA == 0 ? BM : BK) * NC, BCONT = (OPB == 0 ? BK : BN) * NC;
+  constexpr int AVEC = BM * BK * NC / W / kThreads, BVEC = BN * BK * NC / W / kThreads;
+  static_assert(AVEC >= 1 && BVEC >= 1, "tile too small");
+  constexpr int ATPL = ACONT / W, BTPL = BCONT / W;  // threads per contiguous line
+  static_assert(kThreads % ATPL == 0 && kThreads % BTPL == 0, "line split");
+  __shared__ S As[2][NC][BM][RS];
+  __shared__ S Bs[2][NC][BN][RS];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
-  const int64_t m0 = (int64_t)blockIdx.x * BM, n0 = (int64_t)blockIdx.y * BN;
+  int64_t bx, by;
+  tile_of(gx, gy, bx, by);
+  const int64_t m0 = bx * BM, n0 = by * BN;
 
-  // load mapping
-  // A: OPA==0 (N): A(m,k) = A[m + k*lda], m contiguous; else A(m,k) = A[k + m*lda], k contiguous.
-  int am, ak, am_step, ak_step;
-  if (OPA == 0) { am = tid % BM; ak = tid / BM; am_step = 0; ak_step = kThreads / BM; }
-  else { ak = tid % BK; am = tid / BK; ak_step = 0; am_step = kThreads / BK; }
-  // B: OPB==0 (N): B(k,n) = B[k + n*ldb], k contiguous; else B(k,n) = B[n + k*ldb], n contiguous.
-  int bn, bk, bn_step, bk_step;
-  if (OPB == 0) { bk = tid % BK; bn = tid / BK; bk_step = 0; bn_step = kThreads / BK; }
-  else { bn = tid % BN; bk = tid / BN; bn_step = 0; bk_step = kThreads / BN; }
+  // per-thread load coordinates: contiguous-axis scalar offset and line index
+  const int a_c = (tid % ATPL) * W, a_l = tid / ATPL;
+  const int b_c = (tid % BTPL) * W, b_l = tid / BTPL;
+  constexpr int ALS = kThreads / ATPL, BLS = kThreads / BTPL;  // line step per vector
 
-  S ra[NC][EA], rb[NC][EB];
+  Vec<S, W> ra[AVEC], rb[BVEC];
 
   auto load_tiles = [&](int64_t k0) {
 #pragma unroll
-    for (int e = 0; e < EA; ++e) {
-      const int64_t m = m0 + am + e * am_step, k = k0 + ak + e * ak_step;
+    for (int e = 0; e < AVEC; ++e) {
+      const int line = a_l + e * ALS;
+      int64_t m, k;
+      if (OPA == 0) { m = m0 + a_c / NC; k = k0 + line; }
+      else { m = m0 + line; k = k0 + a_c / NC; }
       const bool ok = (m < M) && (k < K);
       const int64_t off = (OPA == 0) ? (m + k * lda) : (k + m * lda);
+      if (ok) {
+        ra[e] = ld<S, W>(A + off * NC + (a_c % NC));
+      } else {
 #pragma unroll
-      for (int c = 0; c < NC; ++c) ra[c][e] = ok ? A[off * NC + c] : (S)0;
+        for (int i = 0; i < W; ++i) ra[e].v[i] = (S)0;
+      }
     }
 #pragma unroll
-    for (int e = 0; e < EB; ++e) {
-      const int64_t n = n0 + bn + e * bn_step, k = k0 + bk + e * bk_step;
+    for (int e = 0; e < BVEC; ++e) {
+      const int line = b_l + e * BLS;
+      int64_t n, k;
+      if (OPB == 0) { k = k0 + b_c / NC; n = n0 + line; }
+      else { k = k0 + line; n = n0 + b_c / NC; }
       const bool ok = (n < N) && (k < K);
       const int64_t off = (OPB == 0) ? (k + n * ldb) : (n + k * ldb);
+      if (ok) {
+        rb[e] = ld<S, W>(B + off * NC + (b_c % NC));
+      } else {
 #pragma unroll
-      for (int c = 0; c < NC; ++c) rb[c][e] = ok ? B[off * NC + c] : (S)0;
+        for (int i = 0; i < W; ++i) rb[e].v[i] = (S)0;
+      }
     }
   };
-  auto store_tiles = [&]() {
+  // scatter register vectors into the K-contiguous LDS layout [c][row][k]
+  auto store_tiles = [&](int buf) {
 #pragma unroll
-    for (int e = 0; e < EA; ++e)
+    for (int e = 0; e < AVEC; ++e) {
+      const int line = a_l + e * ALS;
 #pragma unroll
-      for (int c = 0; c < NC; ++c) As[c][am + e * am_step][ak + e * ak_step] = ra[c][e];
+      for (int i = 0; i < W; ++i) {
+        const int sc = a_c + i;  // scalar index along the contiguous axis
+        const int c = sc % NC, x = sc / NC;
+        if (OPA == 0) As[buf][c][x][line] = ra[e].v[i];
+        else As[buf][c][line][x] = ra[e].v[i];
+      }
+    }
 #pragma unroll
-    for (int e = 0; e < EB; ++e)
+    for (int e = 0; e < BVEC; ++e) {
+      const int line = b_l + e * BLS;
 #pragma unroll
-      for (int c = 0; c < NC; ++c) Bs[c][bn + e * bn_step][bk + e * bk_step] = rb[c][e];
+      for (int i = 0; i < W; ++i) {
+        const int sc = b_c + i;
+        const int c = sc % NC, x = sc / NC;
+        if (OPB == 0) Bs[buf][c][line][x] = rb[e].v[i];
+        else Bs[buf][c][x][line] = rb[e].v[i];
+      }
+    }
   };
 
   acc_t acc[NC][SM][SN];
@@ -119,13 +192,14 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(int64_t M, int64_t N, in
 
   const S sa = CJA ? (S)-1 : (S)1;  // sign of A's imaginary part
   const S sb = CJB ? (S)-1 : (S)1;
+  const int64_t nk = (K + BK - 1) / BK;
 
   load_tiles(0);
-  for (int64_t k0 = 0; k0 < K; k0 += BK) {
-    __syncthreads();
-    store_tiles();
-    __syncthreads();
-    if (k0 + BK < K) load_tiles(k0 + BK);
+  store_tiles(0);
+  __syncthreads();
+  for (int64_t kt = 0; kt < nk; ++kt) {
+    const int buf = (int)(kt & 1);
+    if (kt + 1 < nk) load_tiles((kt + 1) * BK);
 #pragma unroll
     for (int ks = 0; ks < BK / 4; ++ks) {
       const int kk = ks * 4 + (lane >> 4);
@@ -133,11 +207,11 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(int64_t M, int64_t N, in
 #pragma unroll
       for (int i = 0; i < SM; ++i)
 #pragma unroll
-        for (int c = 0; c < NC; ++c) fa[c][i] = As[c][wm * (BM / 2) + i * 16 + (lane & 15)][kk];
+        for (int c = 0; c < NC; ++c) fa[c][i] = As[buf][c][wm * (BM / 2) + i * 16 + (lane & 15)][kk];
 #pragma unroll
       for (int j = 0; j < SN; ++j)
 #pragma unroll
-        for (int c = 0; c < NC; ++c) fb[c][j] = Bs[c][wn * (BN / 2) + j * 16 + (lane & 15)][kk];
+        for (int c = 0; c < NC; ++c) fb[c][j] = Bs[buf][c][wn * (BN / 2) + j * 16 + (lane & 15)][kk];
 #pragma unroll
       for (int i = 0; i < SM; ++i)
 #pragma unroll
@@ -153,6 +227,8 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(int64_t M, int64_t N, in
           }
         }
     }
+    if (kt + 1 < nk) store_tiles(buf ^ 1);
+    __syncthreads();
   }
 
   // epilogue
@@ -183,9 +259,19 @@ template <typename S, int NC, int OPA, int OPB, int CJA, int CJB, int BM, int BN
 int launch_typed(int64_t M, int64_t N, int64_t K, const void *A, int64_t lda, const void *B, int64_t ldb,
                  void *C, int64_t ldc, double ar, double ai, int beta_one, hipStream_t s) {
   const int64_t gx = (M + BM - 1) / BM, gy = (N + BN - 1) / BN;
-  if (gx > 0x7fffffffLL || gy > 65535) return -4;
-  gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN><<<dim3((unsigned)gx, (unsigned)gy), dim3(kThreads), 0, s>>>(
-      M, N, K, (const S *)A, lda, (const S *)B, ldb, (S *)C, ldc, (S)ar, (S)ai, beta_one);
+  const int64_t nblk = gx * gy;
+  if (nblk > 0x7fffffffLL) return -4;
+  // 16-byte accesses need every contiguous run (and every base) to split into whole vectors
+  constexpr int WV = (int)(16 / sizeof(S));
+  const int64_t acont = (OPA == 0 ? M : K) * NC, bcont = (OPB == 0 ? K : N) * NC;
+  const bool vec = ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0) && acont % WV == 0 &&
+                   bcont % WV == 0 && (lda * NC) % WV == 0 && (ldb * NC) % WV == 0;
+  if (vec)
+    gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, WV><<<dim3((unsigned)nblk), dim3(kThreads), 0, s>>>(
+        M, N, K, (const S *)A, lda, (const S *)B, ldb, (S *)C, ldc, (S)ar, (S)ai, beta_one, gx, gy);
+  else
+    gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, 1><<<dim3((unsigned)nblk), dim3(kThreads), 0, s>>>(
+        M, N, K, (const S *)A, lda, (const S *)B, ldb, (S *)C, ldc, (S)ar, (S)ai, beta_one, gx, gy);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

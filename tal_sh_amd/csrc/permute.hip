@@ -98,105 +98,192 @@ struct Real<double2> {
 };
 
 // ------------------------------------------------------------ permute args
+// After fusion a permutation is a 3-D tile problem over dims a (input
+// fastest, stride 1), b and c, with every other dim enumerated by the block
+// index.  Tile mode: b is the output's fastest dim (output stride 1); row
+// mode: a is fastest on both sides and b is the dim with the next smallest
+// output stride.  c is an outer dim carried through the tile so that one
+// block moves kElems elements with all of its loads issued before any store.
 struct PermArgs {
-  int nouter;                 // number of outer (block-enumerated) dims
-  int64_t ea, eb;             // extents of tile dims a and b
-  int64_t isb, osa;           // input stride of b, output stride of a
-  int64_t ntile_a, ntile_b;   // tile counts along a and b
-  int64_t oe[kMaxRank];       // outer extents
-  int64_t ois[kMaxRank];      // outer input strides
-  int64_t oos[kMaxRank];      // outer output strides
+  int nouter;
+  int64_t ea, eb, ec;          // tile-dim extents
+  int64_t isb, isc;            // input strides of b and c (a has stride 1)
+  int64_t osa, osb, osc;       // output strides of a, b, c
+  int64_t ntile_a, ntile_b, ntile_c;
+  int tc;                      // tile size along c (<= TCMAX of the kernel)
+  int64_t oe[kMaxRank];        // outer extents
+  int64_t ois[kMaxRank];       // outer input strides
+  int64_t oos[kMaxRank];       // outer output strides
 };
 
-__device__ inline void outer_offsets(const PermArgs &p, int64_t blk, int64_t &ta, int64_t &tb, int64_t &ibase,
-                                     int64_t &obase) {
-  ta = blk % p.ntile_a;
+__device__ inline void tile_origin(const PermArgs &p, int64_t blk, int TA, int TB, int64_t &a0, int64_t &b0,
+                                   int64_t &c0, int64_t &ib, int64_t &ob) {
+  int64_t t = blk % p.ntile_a;
   blk /= p.ntile_a;
-  tb = blk % p.ntile_b;
+  a0 = t * TA;
+  t = blk % p.ntile_b;
   blk /= p.ntile_b;
-  ibase = 0;
-  obase = 0;
+  b0 = t * TB;
+  t = blk % p.ntile_c;
+  blk /= p.ntile_c;
+  c0 = t * p.tc;
+  ib = a0 + b0 * p.isb + c0 * p.isc;
+  ob = a0 * p.osa + b0 * p.osb + c0 * p.osc;
   for (int k = 0; k < p.nouter; ++k) {
-    int64_t e = p.oe[k];
-    int64_t q = blk / e;
-    int64_t r = blk - q * e;
+    const int64_t e = p.oe[k];
+    const int64_t q = blk / e;
+    const int64_t r = blk - q * e;
     blk = q;
-    ibase += r * p.ois[k];
-    obase += r * p.oos[k];
+    ib += r * p.ois[k];
+    ob += r * p.oos[k];
   }
 }
 
-// Output fastest dim == input fastest dim (a): stream rows of a, rows indexed by b.
-template <typename T, int TA, int TB, int OP>
+template <typename T>
+constexpr int elems_per_block() {
+  return sizeof(T) > 8 ? 2048 : 4096;
+}
+
+// V consecutive elements moved as one 8- or 16-byte access.
+template <typename T, int V>
+struct Pack {
+  T v[V];
+};
+
+template <typename T, int V>
+__device__ inline Pack<T, V> ldv(const T *p) {
+  Pack<T, V> r;
+  if constexpr (sizeof(T) * V == 16) {
+    const uint4 u = *reinterpret_cast<const uint4 *>(p);
+    __builtin_memcpy(&r, &u, 16);
+  } else if constexpr (sizeof(T) * V == 8) {
+    const uint2 u = *reinterpret_cast<const uint2 *>(p);
+    __builtin_memcpy(&r, &u, 8);
+  } else {
+#pragma unroll
+    for (int t = 0; t < V; ++t) r.v[t] = p[t];
+  }
+  return r;
+}
+
+template <typename T, int V>
+__device__ inline void stv(T *p, const Pack<T, V> &r) {
+  if constexpr (sizeof(T) * V == 16) {
+    uint4 u;
+    __builtin_memcpy(&u, &r, 16);
+    *reinterpret_cast<uint4 *>(p) = u;
+  } else if constexpr (sizeof(T) * V == 8) {
+    uint2 u;
+    __builtin_memcpy(&u, &r, 8);
+    *reinterpret_cast<uint2 *>(p) = u;
+  } else {
+#pragma unroll
+    for (int t = 0; t < V; ++t) p[t] = r.v[t];
+  }
+}
+
+template <typename T, int V, int OP>
+__device__ inline void emit(T *o, Pack<T, V> x, typename Real<T>::type ar, typename Real<T>::type ai, int conj) {
+#pragma unroll
+  for (int t = 0; t < V; ++t) x.v[t] = Elem<T>::conj(x.v[t], conj);
+  if (OP == PERM_ADD) {
+    const Pack<T, V> d = ldv<T, V>(o);
+#pragma unroll
+    for (int t = 0; t < V; ++t) x.v[t] = Elem<T>::axpy(d.v[t], x.v[t], ar, ai);
+  }
+  stv<T, V>(o, x);
+}
+
+// Row mode: dim a is contiguous on both sides; no staging needed.  Accesses
+// are V elements wide along a (the host guarantees V | extent of a and
+// 16-byte aligned bases when V > 1).
+template <typename T, int TA, int TB, int V, int OP>
 __global__ __launch_bounds__(kThreads) void perm_rows_kernel(const T *__restrict__ src, T *__restrict__ dst,
                                                              PermArgs p, typename Real<T>::type ar,
                                                              typename Real<T>::type ai, int conj) {
-  int64_t ta, tb, ib, ob;
-  outer_offsets(p, blockIdx.x, ta, tb, ib, ob);
-  const int64_t a0 = ta * TA, b0 = tb * TB;
+  constexpr int EPB = elems_per_block<T>(), N = EPB / kThreads / V, AV = TA / V;
+  int64_t a0, b0, c0, ib, ob;
+  tile_origin(p, blockIdx.x, TA, TB, a0, b0, c0, ib, ob);
   const int na = (int)min<int64_t>(TA, p.ea - a0), nb = (int)min<int64_t>(TB, p.eb - b0);
-  ib += a0 + b0 * p.isb;
-  ob += a0 + b0 * p.osa;  // here osa holds the output stride of b
+  const int nc = (int)min<int64_t>(p.tc, p.ec - c0);
+  Pack<T, V> v[N];
 #pragma unroll
-  for (int e = threadIdx.x; e < TA * TB; e += kThreads) {
-    const int ia = e % TA, jb = e / TA;
-    if (ia < na && jb < nb) {
-      T v = Elem<T>::conj(src[ib + ia + jb * p.isb], conj);
-      T *o = dst + ob + ia + jb * p.osa;
-      if (OP == PERM_COPY)
-        *o = v;
-      else
-        *o = Elem<T>::axpy(*o, v, ar, ai);
-    }
+  for (int q = 0; q < N; ++q) {
+    const int u = threadIdx.x + q * kThreads;
+    const int ia = (u % AV) * V, jb = (u / AV) % TB, kc = u / (AV * TB);
+    if (ia < na && jb < nb && kc < nc) v[q] = ldv<T, V>(src + ib + ia + jb * p.isb + kc * p.isc);
+  }
+#pragma unroll
+  for (int q = 0; q < N; ++q) {
+    const int u = threadIdx.x + q * kThreads;
+    const int ia = (u % AV) * V, jb = (u / AV) % TB, kc = u / (AV * TB);
+    if (ia < na && jb < nb && kc < nc) emit<T, V, OP>(dst + ob + ia + jb * p.osb + kc * p.osc, v[q], ar, ai, conj);
   }
 }
 
-// General case: tile (a = input fastest) x (b = output fastest) through LDS.
-template <typename T, int TA, int TB, int OP>
+// Tile mode: stage a (a x b x c) tile through LDS (+1 pad on the a axis) so
+// the reads run along a and the writes along b, V elements per access on
+// both sides (the host guarantees V | extents of a and b when V > 1).
+template <typename T, int TA, int TB, int V, int OP>
 __global__ __launch_bounds__(kThreads) void perm_tile_kernel(const T *__restrict__ src, T *__restrict__ dst,
                                                              PermArgs p, typename Real<T>::type ar,
                                                              typename Real<T>::type ai, int conj) {
-  __shared__ T tile[TB][TA + 1];
-  int64_t ta, tb, ib, ob;
-  outer_offsets(p, blockIdx.x, ta, tb, ib, ob);
-  const int64_t a0 = ta * TA, b0 = tb * TB;
+  constexpr int EPB = elems_per_block<T>(), N = EPB / kThreads / V, TCMAX = EPB / (TA * TB);
+  constexpr int AV = TA / V, BV = TB / V;
+  __shared__ T tile[TCMAX][TB][TA + 1];
+  int64_t a0, b0, c0, ib, ob;
+  tile_origin(p, blockIdx.x, TA, TB, a0, b0, c0, ib, ob);
   const int na = (int)min<int64_t>(TA, p.ea - a0), nb = (int)min<int64_t>(TB, p.eb - b0);
-  ib += a0 + b0 * p.isb;
-  ob += b0 + a0 * p.osa;
+  const int nc = (int)min<int64_t>(p.tc, p.ec - c0);
+  Pack<T, V> v[N];
 #pragma unroll
-  for (int e = threadIdx.x; e < TA * TB; e += kThreads) {
-    const int ia = e % TA, jb = e / TA;
-    if (ia < na && jb < nb) tile[jb][ia] = src[ib + ia + jb * p.isb];
+  for (int q = 0; q < N; ++q) {
+    const int u = threadIdx.x + q * kThreads;
+    const int ia = (u % AV) * V, jb = (u / AV) % TB, kc = u / (AV * TB);
+    if (ia < na && jb < nb && kc < nc) v[q] = ldv<T, V>(src + ib + ia + jb * p.isb + kc * p.isc);
+  }
+#pragma unroll
+  for (int q = 0; q < N; ++q) {
+    const int u = threadIdx.x + q * kThreads;
+    const int ia = (u % AV) * V, jb = (u / AV) % TB, kc = u / (AV * TB);
+    if (ia < na && jb < nb && kc < nc)
+#pragma unroll
+      for (int t = 0; t < V; ++t) tile[kc][jb][ia + t] = v[q].v[t];
   }
   __syncthreads();
 #pragma unroll
-  for (int e = threadIdx.x; e < TA * TB; e += kThreads) {
-    const int jb = e % TB, ia = e / TB;
-    if (ia < na && jb < nb) {
-      T v = Elem<T>::conj(tile[jb][ia], conj);
-      T *o = dst + ob + jb + ia * p.osa;
-      if (OP == PERM_COPY)
-        *o = v;
-      else
-        *o = Elem<T>::axpy(*o, v, ar, ai);
+  for (int q = 0; q < N; ++q) {
+    const int u = threadIdx.x + q * kThreads;
+    const int jb = (u % BV) * V, ia = (u / BV) % TA, kc = u / (BV * TA);
+    if (ia < na && jb < nb && kc < nc) {
+      Pack<T, V> x;
+#pragma unroll
+      for (int t = 0; t < V; ++t) x.v[t] = tile[kc][jb + t][ia];
+      emit<T, V, OP>(dst + ob + jb + ia * p.osa + kc * p.osc, x, ar, ai, conj);
     }
   }
 }
 
-// Contiguous (identity after fusion) copy / add.
-template <typename T, int OP>
+// Contiguous (identity after fusion) copy / add: V-wide accesses, 4 in
+// flight per thread; the tail (n % (V*4*threads)) is handled element-wise.
+template <typename T, int V, int OP>
 __global__ __launch_bounds__(kThreads) void perm_flat_kernel(const T *__restrict__ src, T *__restrict__ dst,
                                                              int64_t n, typename Real<T>::type ar,
                                                              typename Real<T>::type ai, int conj) {
-  int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  constexpr int U = 4;
+  const int64_t nvec = n / V;
   const int64_t stride = (int64_t)gridDim.x * kThreads;
-  for (; i < n; i += stride) {
-    T v = Elem<T>::conj(src[i], conj);
-    if (OP == PERM_COPY)
-      dst[i] = v;
-    else
-      dst[i] = Elem<T>::axpy(dst[i], v, ar, ai);
+  int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  for (; i + (U - 1) * stride < nvec; i += U * stride) {
+    Pack<T, V> v[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) v[k] = ldv<T, V>(src + (i + k * stride) * V);
+#pragma unroll
+    for (int k = 0; k < U; ++k) emit<T, V, OP>(dst + (i + k * stride) * V, v[k], ar, ai, conj);
   }
+  for (; i < nvec; i += stride) emit<T, V, OP>(dst + i * V, ldv<T, V>(src + i * V), ar, ai, conj);
+  for (int64_t e = nvec * V + (int64_t)blockIdx.x * kThreads + threadIdx.x; e < n; e += stride)
+    emit<T, 1, OP>(dst + e, ldv<T, 1>(src + e), ar, ai, conj);
 }
 
 template <typename T>
@@ -235,15 +322,32 @@ __device__ inline double block_sum(double v) {
   return t;
 }
 
-// Stage 1: block b sums the fixed contiguous chunk [b*chunk, (b+1)*chunk).
-template <typename T>
-__global__ __launch_bounds__(kThreads) void norm2_stage1(const T *__restrict__ x, int64_t n, int64_t chunk,
+// Stage 1: grid-stride over V-wide packs with four independent accumulators
+// per thread; the grid size depends only on n, so the summation order (and
+// the result) is reproducible run to run.
+template <typename T, int V>
+__global__ __launch_bounds__(kThreads) void norm2_stage1(const T *__restrict__ x, int64_t n,
                                                          double *__restrict__ partials) {
-  const int64_t beg = (int64_t)blockIdx.x * chunk;
-  const int64_t end = min<int64_t>(n, beg + chunk);
-  double acc = 0.0;
-  for (int64_t i = beg + threadIdx.x; i < end; i += kThreads) acc += Elem<T>::sq(x[i]);
-  double t = block_sum(acc);
+  const int64_t nvec = n / V, stride = (int64_t)gridDim.x * kThreads;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  for (; i + 3 * stride < nvec; i += 4 * stride) {
+    Pack<T, V> v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = ldv<T, V>(x + (i + k * stride) * V);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int t = 0; t < V; ++t) acc[k] += Elem<T>::sq(v[k].v[t]);
+  }
+  for (; i < nvec; i += stride) {
+    const Pack<T, V> v = ldv<T, V>(x + i * V);
+#pragma unroll
+    for (int t = 0; t < V; ++t) acc[0] += Elem<T>::sq(v.v[t]);
+  }
+  if (blockIdx.x == 0)
+    for (int64_t e = nvec * V + threadIdx.x; e < n; e += kThreads) acc[0] += Elem<T>::sq(x[e]);
+  const double t = block_sum((acc[0] + acc[1]) + (acc[2] + acc[3]));
   if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
@@ -271,22 +375,32 @@ int grid_for(int64_t n) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(b, 8192));
 }
 
-template <typename T, int TA, int TB>
+template <typename T, int TA, int TB, int V>
 void launch_rows(const T *src, T *dst, const PermArgs &p, int op, typename Real<T>::type ar,
                  typename Real<T>::type ai, int conj, int64_t nblk, hipStream_t s) {
   if (op == PERM_COPY)
-    perm_rows_kernel<T, TA, TB, PERM_COPY><<<dim3((unsigned)nblk), dim3(kThreads), 0, s>>>(src, dst, p, ar, ai, conj);
+    perm_rows_kernel<T, TA, TB, V, PERM_COPY><<<dim3((unsigned)nblk), dim3(kThreads), 0, s>>>(src, dst, p, ar, ai,
+                                                                                              conj);
   else
-    perm_rows_kernel<T, TA, TB, PERM_ADD><<<dim3((unsigned)nblk), dim3(kThreads), 0, s>>>(src, dst, p, ar, ai, conj);
+    perm_rows_kernel<T, TA, TB, V, PERM_ADD><<<dim3((unsigned)nblk), dim3(kThreads), 0, s>>>(src, dst, p, ar, ai,
+                                                                                             conj);
 }
 
-template <typename T, int TA, int TB>
+template <typename T, int TA, int TB, int V>
 void launch_tile(const T *src, T *dst, const PermArgs &p, int op, typename Real<T>::type ar,
                  typename Real<T>::type ai, int conj, int64_t nblk, hipStream_t s) {
   if (op == PERM_COPY)
-    perm_tile_kernel<T, TA, TB, PERM_COPY><<<dim3((unsigned)nblk), dim3(kThreads), 0, s>>>(src, dst, p, ar, ai, conj);
+    perm_tile_kernel<T, TA, TB, V, PERM_COPY><<<dim3((unsigned)nblk), dim3(kThreads), 0, s>>>(src, dst, p, ar, ai,
+                                                                                              conj);
   else
-    perm_tile_kernel<T, TA, TB, PERM_ADD><<<dim3((unsigned)nblk), dim3(kThreads), 0, s>>>(src, dst, p, ar, ai, conj);
+    perm_tile_kernel<T, TA, TB, V, PERM_ADD><<<dim3((unsigned)nblk), dim3(kThreads), 0, s>>>(src, dst, p, ar, ai,
+                                                                                             conj);
+}
+
+// Widest access (elements) for type T: 16 bytes.
+template <typename T>
+constexpr int vec_width() {
+  return sizeof(T) >= 16 ? 1 : 16 / (int)sizeof(T);
 }
 
 // Fuses the permutation and launches the matching kernel.
@@ -341,12 +455,21 @@ int launch_perm_typed(const T *src, T *dst, int rank, const int *dims, const int
     }
   }
   const int r = (int)fe.size();
+  constexpr int VW = vec_width<T>();
+  const bool aligned = ((uintptr_t)src % 16 == 0) && ((uintptr_t)dst % 16 == 0);
   if (r <= 1) {
-    const int g = grid_for(vol);
-    if (op == PERM_COPY)
-      perm_flat_kernel<T, PERM_COPY><<<g, kThreads, 0, s>>>(src, dst, vol, ar, ai, conj);
-    else
-      perm_flat_kernel<T, PERM_ADD><<<g, kThreads, 0, s>>>(src, dst, vol, ar, ai, conj);
+    const int g = grid_for(vol / 4);
+    if (aligned) {
+      if (op == PERM_COPY)
+        perm_flat_kernel<T, VW, PERM_COPY><<<g, kThreads, 0, s>>>(src, dst, vol, ar, ai, conj);
+      else
+        perm_flat_kernel<T, VW, PERM_ADD><<<g, kThreads, 0, s>>>(src, dst, vol, ar, ai, conj);
+    } else {
+      if (op == PERM_COPY)
+        perm_flat_kernel<T, 1, PERM_COPY><<<g, kThreads, 0, s>>>(src, dst, vol, ar, ai, conj);
+      else
+        perm_flat_kernel<T, 1, PERM_ADD><<<g, kThreads, 0, s>>>(src, dst, vol, ar, ai, conj);
+    }
     return hipGetLastError() == hipSuccess ? 0 : -2;
   }
   int b = -1;
@@ -360,22 +483,31 @@ int launch_perm_typed(const T *src, T *dst, int rank, const int *dims, const int
       if (c < 0 || fos[k] < fos[c]) c = k;
     b = c;
   }
+  // third tile dim: the remaining dim with the smallest input stride
+  int c = -1;
+  for (int k = 1; k < r; ++k)
+    if (k != b && (c < 0 || fis[k] < fis[c])) c = k;
   PermArgs p;
   std::memset(&p, 0, sizeof(p));
   p.ea = fe[0];
+  p.osa = fos[0];
   p.eb = fe[b];
   p.isb = fis[b];
-  p.osa = rows ? fos[b] : fos[0];
+  p.osb = fos[b];
+  p.ec = c >= 0 ? fe[c] : 1;
+  p.isc = c >= 0 ? fis[c] : 0;
+  p.osc = c >= 0 ? fos[c] : 0;
   p.nouter = 0;
   int64_t nouter_vol = 1;
   for (int k = 1; k < r; ++k) {
-    if (k == b) continue;
+    if (k == b || k == c) continue;
     p.oe[p.nouter] = fe[k];
     p.ois[p.nouter] = fis[k];
     p.oos[p.nouter] = fos[k];
     p.nouter++;
     nouter_vol *= fe[k];
   }
+  constexpr int EPB = elems_per_block<T>();
   int ta, tb;
   if (rows) {
     if (p.ea >= 64) { ta = 64; tb = 16; }
@@ -388,19 +520,32 @@ int launch_perm_typed(const T *src, T *dst, int rank, const int *dims, const int
     else if (m >= 12) { ta = 16; tb = 16; }
     else { ta = 8; tb = 8; }
   }
+  p.tc = (int)std::min<int64_t>(p.ec, EPB / (ta * tb));
   p.ntile_a = (p.ea + ta - 1) / ta;
   p.ntile_b = (p.eb + tb - 1) / tb;
-  const int64_t nblk = p.ntile_a * p.ntile_b * nouter_vol;
+  p.ntile_c = (p.ec + p.tc - 1) / p.tc;
+  const int64_t nblk = p.ntile_a * p.ntile_b * p.ntile_c * nouter_vol;
   if (nblk <= 0 || nblk > 0x7fffffffLL) return -4;
+  // vector accesses need V | extent on each contiguous side (so every other
+  // stride is a multiple of V) and 16-byte aligned bodies
+  const bool vec = aligned && (p.ea % VW == 0) && (rows || p.eb % VW == 0) && VW > 1;
   if (rows) {
-    if (ta == 64) launch_rows<T, 64, 16>(src, dst, p, op, ar, ai, conj, nblk, s);
-    else if (ta == 32) launch_rows<T, 32, 32>(src, dst, p, op, ar, ai, conj, nblk, s);
-    else if (ta == 16) launch_rows<T, 16, 64>(src, dst, p, op, ar, ai, conj, nblk, s);
-    else launch_rows<T, 8, 128>(src, dst, p, op, ar, ai, conj, nblk, s);
+#define TAL_ROWS(A, B)                                                         \
+  (vec ? launch_rows<T, A, B, VW>(src, dst, p, op, ar, ai, conj, nblk, s)      \
+       : launch_rows<T, A, B, 1>(src, dst, p, op, ar, ai, conj, nblk, s))
+    if (ta == 64) TAL_ROWS(64, 16);
+    else if (ta == 32) TAL_ROWS(32, 32);
+    else if (ta == 16) TAL_ROWS(16, 64);
+    else TAL_ROWS(8, 128);
+#undef TAL_ROWS
   } else {
-    if (ta == 32) launch_tile<T, 32, 32>(src, dst, p, op, ar, ai, conj, nblk, s);
-    else if (ta == 16) launch_tile<T, 16, 16>(src, dst, p, op, ar, ai, conj, nblk, s);
-    else launch_tile<T, 8, 8>(src, dst, p, op, ar, ai, conj, nblk, s);
+#define TAL_TILE(A, B)                                                         \
+  (vec ? launch_tile<T, A, B, VW>(src, dst, p, op, ar, ai, conj, nblk, s)      \
+       : launch_tile<T, A, B, 1>(src, dst, p, op, ar, ai, conj, nblk, s))
+    if (ta == 32) TAL_TILE(32, 32);
+    else if (ta == 16) TAL_TILE(16, 16);
+    else TAL_TILE(8, 8);
+#undef TAL_TILE
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
@@ -471,12 +616,15 @@ size_t norm2_scratch_elems() { return kNormBlocks; }
 
 template <typename T>
 static int launch_norm2_typed(const T *x, size_t vol, double *partials, void *out, hipStream_t s) {
-  int64_t n = (int64_t)vol;
-  int64_t chunk = (n + kNormBlocks - 1) / kNormBlocks;
-  chunk = ((chunk + kThreads - 1) / kThreads) * kThreads;
-  if (chunk == 0) chunk = kThreads;
-  const int nb = (int)std::max<int64_t>(1, (n + chunk - 1) / chunk);
-  norm2_stage1<T><<<nb, kThreads, 0, s>>>(x, n, chunk, partials);
+  const int64_t n = (int64_t)vol;
+  constexpr int VW = vec_width<T>();
+  const bool aligned = ((uintptr_t)x % 16) == 0;
+  const int64_t packs = aligned ? n / VW : n;
+  const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(kNormBlocks, (packs + 4 * kThreads - 1) / (4 * kThreads)));
+  if (aligned)
+    norm2_stage1<T, VW><<<nb, kThreads, 0, s>>>(x, n, partials);
+  else
+    norm2_stage1<T, 1><<<nb, kThreads, 0, s>>>(x, n, partials);
   norm2_stage2<T><<<1, kThreads, 0, s>>>(partials, nb, (T *)out);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
