@@ -255,6 +255,201 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
       }
 }
 
+// ---------------------------------------------------------------------------
+// R4 GEMM as 3-term split-BF16 products (bf16x6): every FP32 operand value x
+// is split exactly into x = h + m + l with h = bf16(x), m = bf16(x - h),
+// l = bf16(x - h - m) (each residual is exact in FP32), and
+//   a*b ~ ah*bh + ah*bm + am*bh + ah*bl + am*bm + al*bh
+// is accumulated in FP32 by v_mfma_f32_16x16x32_bf16 (method of Henry, Tang,
+// Heinecke, "Leveraging the bfloat16 Artificial Intelligence Datatype For
+// Higher-Precision Computations", ARITH 2019).  The dropped terms are below
+// 2^-24 relative; with FP32 accumulation the normwise error at K = 9216 is a
+// few 1e-7 (cf. ~1e-6 for the FP32 MFMA path).  Tiles are 256 x 256: the
+// split costs VALU work per element staged, so a large tile keeps both that
+// and the L2 traffic per MFMA low.  FP32 tiles come from HBM as
+// for the FP32 kernel; the split happens once per element when the tile is
+// written to LDS (three BF16 planes, rows padded to 80 bytes so the 16-byte
+// fragment reads of 16 lanes hit all 64 banks once).  MFMA 16x16x32 bf16
+// layout: A lane l holds A[l%16][8*(l/16)..+7], B lane l holds
+// B[8*(l/16)..+7][l%16], accumulator register r of lane l holds
+// C[4*(l/16)+r][l%16].
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ inline void split3(float x, __bf16 &h, __bf16 &m, __bf16 &l) {
+  h = (__bf16)x;
+  const float r = x - (float)h;
+  m = (__bf16)r;
+  l = (__bf16)(r - (float)m);
+}
+
+// Block tile BM x BN with WM x WN waves (each wave (BM/WM) x (BN/WN)); BK = 32
+// (one MFMA depth).  Per K step each wave sums the five small products of a
+// 16x16 tile into a zero-started register and the dominant ah*bh product
+// into another, then adds both to the running FP32 accumulator, so the
+// accumulator is rounded once per K step instead of six times.
+template <int OPA, int OPB, int W, int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(2))) void gemm_bf16x6_kernel(
+    int64_t M, int64_t N, int64_t K, const float *__restrict__ A, int64_t lda, const float *__restrict__ B,
+    int64_t ldb, float *__restrict__ C, int64_t ldc, float alpha, int beta_one, int64_t gx, int64_t gy) {
+  constexpr int NT = 64 * WM * WN, BKS = 32, RS = BKS + 8;
+  constexpr int TM = BM / WM, TN = BN / WN, SM = TM / 16, SN = TN / 16;
+  constexpr int ACONT = (OPA == 0 ? BM : BKS), BCONT = (OPB == 0 ? BKS : BN);
+  constexpr int AVEC = BM * BKS / W / NT, BVEC = BN * BKS / W / NT;
+  constexpr int ATPL = ACONT / W, BTPL = BCONT / W;
+  constexpr int ALS = NT / ATPL, BLS = NT / BTPL;
+  static_assert(AVEC >= 1 && BVEC >= 1 && NT % ATPL == 0 && NT % BTPL == 0, "tile split");
+  __shared__ __attribute__((aligned(16))) __bf16 As[3][BM][RS];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[3][BN][RS];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+  int64_t bx, by;
+  tile_of(gx, gy, bx, by);
+  const int64_t m0 = bx * BM, n0 = by * BN;
+  const int a_c = (tid % ATPL) * W, a_l = tid / ATPL;
+  const int b_c = (tid % BTPL) * W, b_l = tid / BTPL;
+  Vec<float, W> ra[AVEC], rb[BVEC];
+
+  auto load_tiles = [&](int64_t k0) {
+#pragma unroll
+    for (int e = 0; e < AVEC; ++e) {
+      const int line = a_l + e * ALS;
+      const int64_t m = OPA == 0 ? m0 + a_c : m0 + line, k = OPA == 0 ? k0 + line : k0 + a_c;
+      const int64_t off = OPA == 0 ? (m + k * lda) : (k + m * lda);
+      if (m < M && k < K) {
+        ra[e] = ld<float, W>(A + off);
+      } else {
+#pragma unroll
+        for (int i = 0; i < W; ++i) ra[e].v[i] = 0.0f;
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < BVEC; ++e) {
+      const int line = b_l + e * BLS;
+      const int64_t n = OPB == 0 ? n0 + line : n0 + b_c, k = OPB == 0 ? k0 + b_c : k0 + line;
+      const int64_t off = OPB == 0 ? (k + n * ldb) : (n + k * ldb);
+      if (n < N && k < K) {
+        rb[e] = ld<float, W>(B + off);
+      } else {
+#pragma unroll
+        for (int i = 0; i < W; ++i) rb[e].v[i] = 0.0f;
+      }
+    }
+  };
+  // split each value into three BF16 planes; along K the W terms of a plane
+  // are contiguous and go out as one packed store
+  auto put = [&](__bf16 (*P)[BM > BN ? BM : BN][RS], const Vec<float, W> &v, int row, int col, bool kcont) {
+    __bf16 h[W], m[W], l[W];
+#pragma unroll
+    for (int i = 0; i < W; ++i) split3(v.v[i], h[i], m[i], l[i]);
+    if (kcont && W == 4) {
+      uint2 u;
+      __builtin_memcpy(&u, h, 8);
+      *reinterpret_cast<uint2 *>(&P[0][row][col]) = u;
+      __builtin_memcpy(&u, m, 8);
+      *reinterpret_cast<uint2 *>(&P[1][row][col]) = u;
+      __builtin_memcpy(&u, l, 8);
+      *reinterpret_cast<uint2 *>(&P[2][row][col]) = u;
+    } else {
+#pragma unroll
+      for (int i = 0; i < W; ++i) {
+        const int r = kcont ? row : row + i, c = kcont ? col + i : col;
+        P[0][r][c] = h[i];
+        P[1][r][c] = m[i];
+        P[2][r][c] = l[i];
+      }
+    }
+  };
+  auto store_tiles = [&]() {
+#pragma unroll
+    for (int e = 0; e < AVEC; ++e) {
+      const int line = a_l + e * ALS;
+      if (OPA == 1) put(reinterpret_cast<__bf16(*)[BM > BN ? BM : BN][RS]>(As), ra[e], line, a_c, true);
+      else put(reinterpret_cast<__bf16(*)[BM > BN ? BM : BN][RS]>(As), ra[e], a_c, line, false);
+    }
+#pragma unroll
+    for (int e = 0; e < BVEC; ++e) {
+      const int line = b_l + e * BLS;
+      if (OPB == 0) put(reinterpret_cast<__bf16(*)[BM > BN ? BM : BN][RS]>(Bs), rb[e], line, b_c, true);
+      else put(reinterpret_cast<__bf16(*)[BM > BN ? BM : BN][RS]>(Bs), rb[e], b_c, line, false);
+    }
+  };
+
+  f32x4 acc[SM][SN];
+#pragma unroll
+  for (int i = 0; i < SM; ++i)
+#pragma unroll
+    for (int j = 0; j < SN; ++j) acc[i][j] = f32x4{0, 0, 0, 0};
+
+  const int64_t nk = (K + BKS - 1) / BKS;
+  const int kof = 8 * (lane >> 4);
+  const f32x4 zero = {0, 0, 0, 0};
+  load_tiles(0);
+  for (int64_t kt = 0; kt < nk; ++kt) {
+    __syncthreads();
+    store_tiles();
+    __syncthreads();
+    if (kt + 1 < nk) load_tiles((kt + 1) * BKS);
+    bf16x8 fb[3][SN];
+#pragma unroll
+    for (int j = 0; j < SN; ++j)
+#pragma unroll
+      for (int t = 0; t < 3; ++t)
+        fb[t][j] = *reinterpret_cast<const bf16x8 *>(&Bs[t][wn * TN + j * 16 + (lane & 15)][kof]);
+#pragma unroll
+    for (int i = 0; i < SM; ++i) {
+      bf16x8 fa[3];
+#pragma unroll
+      for (int t = 0; t < 3; ++t)
+        fa[t] = *reinterpret_cast<const bf16x8 *>(&As[t][wm * TM + i * 16 + (lane & 15)][kof]);
+#pragma unroll
+      for (int j = 0; j < SN; ++j) {
+        f32x4 sm = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[2], fb[0][j], zero, 0, 0, 0);
+        sm = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[1], fb[1][j], sm, 0, 0, 0);
+        sm = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[0], fb[2][j], sm, 0, 0, 0);
+        sm = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[1], fb[0][j], sm, 0, 0, 0);
+        sm = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[0], fb[1][j], sm, 0, 0, 0);
+        const f32x4 hh = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[0], fb[0][j], zero, 0, 0, 0);
+        acc[i][j] += hh + sm;
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < SM; ++i)
+#pragma unroll
+    for (int j = 0; j < SN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t m = m0 + wm * TM + i * 16 + 4 * (lane >> 4) + r;
+        const int64_t n = n0 + wn * TN + j * 16 + (lane & 15);
+        if (m < M && n < N) {
+          float *c = C + m + n * ldc;
+          const float v = alpha * acc[i][j][r];
+          *c = beta_one ? *c + v : v;
+        }
+      }
+}
+
+template <int OPA, int OPB>
+int launch_bf16x6(int64_t M, int64_t N, int64_t K, const void *A, int64_t lda, const void *B, int64_t ldb, void *C,
+                  int64_t ldc, double ar, int beta_one, hipStream_t s) {
+  const int64_t acont = OPA == 0 ? M : K, bcont = OPB == 0 ? K : N;
+  const bool vec = ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0) && acont % 4 == 0 && bcont % 4 == 0 &&
+                   lda % 4 == 0 && ldb % 4 == 0;
+  if (vec) {  // 128 x 128 tiles, 4 waves of 64 x 64
+    const int64_t gx = (M + 127) / 128, gy = (N + 127) / 128, nblk = gx * gy;
+    if (nblk > 0x7fffffffLL) return -4;
+    gemm_bf16x6_kernel<OPA, OPB, 4, 128, 128, 2, 2><<<dim3((unsigned)nblk), dim3(256), 0, s>>>(
+        M, N, K, (const float *)A, lda, (const float *)B, ldb, (float *)C, ldc, (float)ar, beta_one, gx, gy);
+  } else {  // scalar accesses: 128 x 128 tiles, 4 waves of 64 x 64
+    const int64_t gx = (M + 127) / 128, gy = (N + 127) / 128, nblk = gx * gy;
+    if (nblk > 0x7fffffffLL) return -4;
+    gemm_bf16x6_kernel<OPA, OPB, 1, 128, 128, 2, 2><<<dim3((unsigned)nblk), dim3(256), 0, s>>>(
+        M, N, K, (const float *)A, lda, (const float *)B, ldb, (float *)C, ldc, (float)ar, beta_one, gx, gy);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
 template <typename S, int NC, int OPA, int OPB, int CJA, int CJB, int BM, int BN>
 int launch_typed(int64_t M, int64_t N, int64_t K, const void *A, int64_t lda, const void *B, int64_t ldb,
                  void *C, int64_t ldc, double ar, double ai, int beta_one, hipStream_t s) {
@@ -310,12 +505,11 @@ int op_code(char t) {
 
 }  // namespace
 
-bool gemm_split_bf16_available() { return false; }
+bool gemm_split_bf16_available() { return true; }
 
 int launch_gemm(int dk, int r4mode, char ta, char tb, int conja, int conjb, int64_t M, int64_t N, int64_t K,
                 double ar, double ai, const void *A, int64_t lda, const void *B, int64_t ldb, int beta_one, void *C,
                 int64_t ldc, hipStream_t s) {
-  (void)r4mode;
   const bool cplx = (dk == 14 || dk == 18);
   const int opa = op_code(ta), opb = op_code(tb);
   const int cja = (cplx && conja) ? 1 : 0, cjb = (cplx && conjb) ? 1 : 0;
@@ -324,6 +518,12 @@ int launch_gemm(int dk, int r4mode, char ta, char tb, int conja, int conjb, int6
   if (ldc < std::max<int64_t>(1, M)) return -1;
   if (lda < std::max<int64_t>(1, opa == 0 ? M : K)) return -1;
   if (ldb < std::max<int64_t>(1, opb == 0 ? K : N)) return -1;
+  if (dk == 4 && r4mode == R4_SPLIT_BF16) {
+    if (opa == 0 && opb == 0) return launch_bf16x6<0, 0>(M, N, K, A, lda, B, ldb, C, ldc, ar, beta_one, s);
+    if (opa == 0 && opb == 1) return launch_bf16x6<0, 1>(M, N, K, A, lda, B, ldb, C, ldc, ar, beta_one, s);
+    if (opa == 1 && opb == 0) return launch_bf16x6<1, 0>(M, N, K, A, lda, B, ldb, C, ldc, ar, beta_one, s);
+    return launch_bf16x6<1, 1>(M, N, K, A, lda, B, ldb, C, ldc, ar, beta_one, s);
+  }
   switch (dk) {
     case 8: return dispatch_ops<double, 1, 128, 128>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, s);
     case 4: return dispatch_ops<float, 1, 128, 128>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, s);

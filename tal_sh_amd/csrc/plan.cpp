@@ -230,10 +230,11 @@ int plan_contraction(const DigitalPattern &p, const int *ddims, const int *ldims
           if (sx == xt) { c.a.op = 'T'; }
           else if (sx == xn) { c.a.op = 'N'; }
           else {
+            // a copy is needed anyway: produce the K-contiguous form, which
+            // every GEMM kernel stages into LDS without scattering
             c.a.transpose = true;
-            const bool m_first = !sx.empty() && sx[0] < kContr;
-            c.a.op = m_first ? 'N' : 'T';
-            c.a.o2n = o2n_of(sx, m_first ? xn : xt);
+            c.a.op = 'T';
+            c.a.o2n = o2n_of(sx, xt);
           }
           // operand Y (B): 'N' form = K++N, 'T' form = N++K
           const std::vector<int> yn = cat(K, N), yt = cat(N, K);
@@ -241,9 +242,8 @@ int plan_contraction(const DigitalPattern &p, const int *ddims, const int *ldims
           else if (sy == yt) { c.b.op = 'T'; }
           else {
             c.b.transpose = true;
-            const bool n_first = !sy.empty() && sy[0] < kContr;
-            c.b.op = n_first ? 'T' : 'N';
-            c.b.o2n = o2n_of(sy, n_first ? yt : yn);
+            c.b.op = 'N';
+            c.b.o2n = o2n_of(sy, yn);
           }
           c.M = 1;
           for (int s : M) c.M *= ext_of[s];

@@ -196,6 +196,31 @@ def test_contract_random_all_kinds(talsh_gpu, oracle):
             assert relerr(dt.host_array(), want) <= tol, (dk, pat)
 
 
+def test_contract_split_bf16_r4(talsh_gpu, oracle):
+    """R4 with fast math (3-term split-BF16 MFMA, FP32 accumulation): normwise error <= 1e-6
+    against the FP64 oracle evaluated on the same FP32 inputs widened to FP64."""
+    ts = talsh_gpu
+    lib = ts.load()
+    assert lib.talshEnableFastMath(ts.DEV_NVIDIA_GPU, 0) == 0
+    assert lib.talshQueryFastMath(ts.DEV_NVIDIA_GPU, 0) == ts.YEP
+    try:
+        rng = random.Random(7)
+        cases = [("D(a,b,c,d)+=L(a,i,c,j)*R(j,b,i,d)", [24, 20, 24, 20], [24, 40, 24, 36], [36, 20, 40, 20])]
+        for _ in range(20):
+            pat, dd, ld, rd = random_contraction(rng, max_rank=4, ext_lo=1, ext_hi=13)
+            cases.append((pat, dd, ld, rd))
+        for t, (pat, dd, ld, rd) in enumerate(cases):
+            say("split-bf16", pat, dd, ld, rd)
+            L, R, D = tensor(R4, ld, t), tensor(R4, rd, t + 1), tensor(R4, dd, t + 2)
+            lt, rt, dt = on_gpu(ts, R4, ld, L), on_gpu(ts, R4, rd, R), on_gpu(ts, R4, dd, D)
+            ts.check(ts.contract(pat, dt, lt, rt, alpha=0.5), "contract")
+            want = oracle.contract(R8, pat, dd, D.astype(np.float64), ld, L.astype(np.float64), rd,
+                                   R.astype(np.float64), (0.5, 0.0), 1)
+            assert relerr(dt.host_array().astype(np.float64), want) <= 1e-6, pat
+    finally:
+        lib.talshDisableFastMath(ts.DEV_NVIDIA_GPU, 0)
+
+
 def test_contract_reference_shapes(talsh_gpu, oracle):
     """A sample of the reference benchmark shapes (tensor_contractions.txt), reduced in size."""
     import json
