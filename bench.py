@@ -177,6 +177,97 @@ def bench_transposes(oracle, steps):
             "add_gbs": add_gbs, "norm2_gbs": norm_gbs, "permutations": len(perms)}
 
 
+def batch_contraction(rng, lo=2, hi=24, max_vol=400000):
+    """Config 4 task: ranks 2-4 per operand, 1-2 contracted indices, extents uniform in lo..hi."""
+    while True:
+        rl, rr = rng.randint(2, 4), rng.randint(2, 4)
+        nc = rng.randint(1, min(2, rl, rr))
+        nl, nr = rl - nc, rr - nc
+        labels = [chr(ord("a") + i) for i in range(nl + nr + nc)]
+        ext = {x: rng.randint(lo, hi) for x in labels}
+        lu, ru, cc = labels[:nl], labels[nl:nl + nr], labels[nl + nr:]
+        d, l, r = lu + ru, lu + cc, ru + cc
+        rng.shuffle(d)
+        rng.shuffle(l)
+        rng.shuffle(r)
+        vol = lambda s: int(np.prod([ext[x] for x in s])) if s else 1
+        if max(vol(d), vol(l), vol(r)) > max_vol:
+            continue
+        pat = "D(%s)+=L(%s)*R(%s)" % (",".join(d), ",".join(l), ",".join(r))
+        return pat, [ext[x] for x in d], [ext[x] for x in l], [ext[x] for x in r]
+
+
+def bench_batch(oracle, dk, ntask=4096, reps=5, check_sample=128):
+    """Config 4: 4096 seeded small contractions; one persistent launch (graph replay)
+    versus one talshTensorContract per task (each its own transpose+GEMM launches)."""
+    import random
+
+    rng = random.Random(4096 + dk)
+    tasks = []
+    for t in range(ntask):
+        pat, dd, ld, rd = batch_contraction(rng)
+        lt, L = gpu_tensor(dk, ld, 7 * t + 1)
+        rt, R = gpu_tensor(dk, rd, 7 * t + 3)
+        dt = ts.Tensor(dk, dd, ts.gpu_flat(0))
+        tasks.append((pat, dd, ld, rd, L, R, lt, rt, dt))
+    lib = ts.load()
+    b = ts.Batch()
+    for (pat, dd, ld, rd, L, R, lt, rt, dt) in tasks:
+        ts.check(b.add(pat, dt, lt, rt, alpha=1.0, accumulative=0), "batch add")
+    t0 = time.perf_counter()
+    tk = ts.Task()
+    ts.check(b.execute(tk), "batch execute")  # plans, autotunes, captures
+    tk.wait()
+    tk.destruct()
+    first_s = time.perf_counter() - t0
+    st = b.stats()
+    flop = st["flops"]
+    lib.talshAmdDeviceSynchronize(0)
+    t0 = time.perf_counter()
+    gpu_ms = []
+    for _ in range(reps):
+        tk = ts.Task()
+        ts.check(b.execute(tk), "batch execute")
+        tk.wait()
+        gpu_ms.append(1e3 * tk.times()["mmul"])
+        tk.destruct()
+    lib.talshAmdDeviceSynchronize(0)
+    batched_s = (time.perf_counter() - t0) / reps
+    # correctness on a sample of tasks (each checked against the oracle)
+    srng = np.random.default_rng(9)
+    worst = 0.0
+    for i in srng.choice(ntask, size=min(check_sample, ntask), replace=False):
+        pat, dd, ld, rd, L, R, lt, rt, dt = tasks[i]
+        got = dt.host_array()
+        want = oracle.contract(dk, pat, dd, np.zeros(int(np.prod(dd)) if dd else 1, dtype=ts.NP_DTYPE[dk]),
+                               ld, L, rd, R, (1.0, 0.0), 0)
+        den = np.linalg.norm(want.astype(np.float64)) or 1.0
+        worst = max(worst, float(np.linalg.norm((got - want).astype(np.float64)) / den))
+        dt.place(ts.gpu_flat(0))
+    # one launch sequence per task: talshTensorContract, all queued, then waited
+    pertask = []
+    t0 = time.perf_counter()
+    for (pat, dd, ld, rd, L, R, lt, rt, dt) in tasks:
+        tk = ts.Task()
+        ts.check(ts.contract(pat, dt, lt, rt, alpha=1.0, accumulative=0, task=tk), "contract")
+        pertask.append(tk)
+    for tk in pertask:
+        tk.wait()
+        tk.destruct()
+    lib.talshAmdDeviceSynchronize(0)
+    single_s = time.perf_counter() - t0
+    b.destroy()
+    for (pat, dd, ld, rd, L, R, lt, rt, dt) in tasks:
+        for t in (lt, rt, dt):
+            t.destruct()
+    return {"tasks": ntask, "flop": flop, "batched_tflops": flop / batched_s / 1e12,
+            "batched_ms": 1e3 * batched_s, "batched_gpu_ms": float(np.median(gpu_ms)),
+            "per_task_tflops": flop / single_s / 1e12, "per_task_ms": 1e3 * single_s,
+            "speedup_vs_per_task": single_s / batched_s, "first_execution_s": first_s,
+            "tuned_classes": st["tuned_classes"], "tune_ms": st["tune_ms"], "work_items": st["work_items"],
+            "max_rel_err_sampled": worst}
+
+
 def cpu_baseline(budget_s=30.0):
     """TAL-SH's own CPU path on a bounded sample of the headline contraction.
 
@@ -218,7 +309,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only")
-    ap.add_argument("--only", default="", help="comma list of secondary parts to run (r4,bf16,c8,config1,transpose)")
+    ap.add_argument("--only", default="", help="comma list of parts to run (head,r4,bf16,c8,config1,transpose,batch)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1).", file=sys.stderr)
@@ -248,6 +339,9 @@ def main():
         secondary["r8_config1"] = bench_contraction(oracle, R8, "D(a,b,c,d)+=L(c,i,b,j)*R(d,j,a,i)", 32, 50, 5)
     if want("transpose"):
         secondary["transpose_config2"] = bench_transposes(oracle, 5)
+    if want("batch"):
+        secondary["batched_config4_r8"] = bench_batch(oracle, R8)
+        secondary["batched_config4_r4"] = bench_batch(oracle, R4)
     log("secondary", secondary)
     if not head:
         ts.shutdown()

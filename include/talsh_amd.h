@@ -48,6 +48,27 @@ int talsh_get_contr_ptrn_str2dig(const char *c_str, int *dig_ptrn, int *drank, i
 int talshAmdPlanContraction(const char *pattern, const int *ddims, const int *ldims, const int *rdims,
                             int accumulative, long long *mnk, int *flags, char *ops, int *perms, double *moved);
 
+/* ---- Batched contractions (one persistent launch, replayed as a hipGraph) ----
+ * A batch holds many contractions D (+)= alpha * L * R (same pattern syntax as
+ * talshTensorContract) on tensors resident on the execution GPU, all of one
+ * real data kind (R4 or R8).  Outputs must be distinct tensors and no output
+ * may be an input of another contraction in the batch.  The first execution
+ * plans the batch (gather tables, autotuned tile shapes, captured graph); an
+ * unchanged batch re-executes by replaying the graph.  Execution is
+ * asynchronous when talsh_task is given (talshTaskWait etc. apply), blocking
+ * otherwise. */
+typedef struct talshAmdBatch talshAmdBatch;
+int talshAmdBatchCreate(talshAmdBatch **batch);
+int talshAmdBatchAdd(talshAmdBatch *batch, const char *pattern, talsh_tens_t *dtens, talsh_tens_t *ltens,
+                     talsh_tens_t *rtens, double scale_real, double scale_imag, int accumulative);
+int talshAmdBatchSize(const talshAmdBatch *batch);
+int talshAmdBatchExecute(talshAmdBatch *batch, int dev_id, int dev_kind, talsh_task_t *talsh_task);
+/* FLOP per execution, work items (tiles), shape classes tuned by this batch,
+ * tuning time (ms) and number of executions so far. */
+int talshAmdBatchStats(const talshAmdBatch *batch, double *flops, long long *work_items, int *tuned_classes,
+                       double *tune_ms, int *executions);
+int talshAmdBatchDestroy(talshAmdBatch *batch);
+
 /* Copies a short build description into buf. */
 int talshAmdBuildInfo(char *buf, int len);
 

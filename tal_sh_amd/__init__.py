@@ -132,6 +132,12 @@ SIGNATURES = {
     "talshAmdSetNumStreams": (_I, [_I]),
     "talshAmdDeviceSynchronize": (_I, [_I]),
     "talshAmdBuildInfo": (_I, [ctypes.c_char_p, _I]),
+    "talshAmdBatchCreate": (_I, [ctypes.POINTER(_P)]),
+    "talshAmdBatchAdd": (_I, [_P, ctypes.c_char_p, _TP, _TP, _TP, _D, _D, _I]),
+    "talshAmdBatchSize": (_I, [_P]),
+    "talshAmdBatchExecute": (_I, [_P, _I, _I, _KP]),
+    "talshAmdBatchStats": (_I, [_P, _DP, ctypes.POINTER(ctypes.c_longlong), _IP, _DP, _IP]),
+    "talshAmdBatchDestroy": (_I, [_P]),
 }
 
 _lib = None
@@ -278,6 +284,34 @@ class Task:
 
     def destruct(self):
         return self.lib.talshTaskDestruct(self.ptr)
+
+
+class Batch:
+    """talshAmdBatch handle: many contractions, one persistent launch, graph replays."""
+
+    def __init__(self):
+        self.lib = load()
+        self.h = ctypes.c_void_p()
+        check(self.lib.talshAmdBatchCreate(ctypes.byref(self.h)), "talshAmdBatchCreate")
+
+    def add(self, pattern, d, l, r, alpha=1.0, accumulative=YEP):
+        return self.lib.talshAmdBatchAdd(self.h, pattern.encode(), d.ptr, l.ptr, r.ptr, float(alpha), 0.0,
+                                         accumulative)
+
+    def execute(self, task=None, gpu=0):
+        return self.lib.talshAmdBatchExecute(self.h, gpu, DEV_NVIDIA_GPU, task.ptr if task else None)
+
+    def stats(self):
+        f, w, c, t, e = ctypes.c_double(), ctypes.c_longlong(), ctypes.c_int(), ctypes.c_double(), ctypes.c_int()
+        check(self.lib.talshAmdBatchStats(self.h, ctypes.byref(f), ctypes.byref(w), ctypes.byref(c),
+                                          ctypes.byref(t), ctypes.byref(e)), "talshAmdBatchStats")
+        return {"flops": f.value, "work_items": w.value, "tuned_classes": c.value, "tune_ms": t.value,
+                "executions": e.value}
+
+    def destroy(self):
+        if self.h:
+            check(self.lib.talshAmdBatchDestroy(self.h), "talshAmdBatchDestroy")
+            self.h = ctypes.c_void_p()
 
 
 def contract(pattern, d, l, r, alpha=1.0, dev_id=DEV_DEFAULT, dev_kind=DEV_DEFAULT, copy_ctrl=None,

@@ -1,0 +1,113 @@
+"""Batched contractions (talshAmdBatch*): one persistent launch replayed as a hipGraph.
+
+Every task of a batch is checked against the oracle; graph replays are
+checked one by one with ordinary (eager) library work between them, and the
+batched result must equal, to rounding, what talshTensorContract gives for
+the same task alone.
+"""
+import random
+
+import numpy as np
+import pytest
+
+from tal_common import R4, R8, tensor
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def relerr(got, want):
+    den = np.linalg.norm(want.astype(np.float64))
+    return np.linalg.norm((got - want).astype(np.float64)) / (den if den > 0 else 1.0)
+
+
+def batch_contraction(rng, lo=2, hi=24):
+    """ranks 2-4 per operand, 1-2 contracted indices, extents uniform in lo..hi."""
+    while True:
+        rl, rr = rng.randint(2, 4), rng.randint(2, 4)
+        nc = rng.randint(1, min(2, rl, rr))
+        nl, nr = rl - nc, rr - nc
+        labels = [chr(ord("a") + i) for i in range(nl + nr + nc)]
+        ext = {x: rng.randint(lo, hi) for x in labels}
+        lu, ru, cc = labels[:nl], labels[nl:nl + nr], labels[nl + nr:]
+        d = lu + ru
+        rng.shuffle(d)
+        l = lu + cc
+        rng.shuffle(l)
+        r = ru + cc
+        rng.shuffle(r)
+        vol = lambda s: int(np.prod([ext[x] for x in s])) if s else 1
+        if max(vol(d), vol(l), vol(r)) > 400000:
+            continue
+        pat = "D(%s)+=L(%s)*R(%s)" % (",".join(d), ",".join(l), ",".join(r))
+        return pat, [ext[x] for x in d], [ext[x] for x in l], [ext[x] for x in r]
+
+
+def upload(ts, dk, dims, data):
+    t = ts.Tensor(dk, dims, ts.gpu_flat(0), init_val=complex(data[0]) if not len(dims) else 0.0)
+    if len(dims):
+        h = ts.Tensor(dk, dims, 0, data=data)
+        lab = ",".join("i%d" % k for k in range(len(dims)))
+        ts.check(ts.copy("D(%s)=L(%s)" % (lab, lab), t, h), "upload")
+        h.destruct()
+    return t
+
+
+@pytest.mark.parametrize("dk", (R8, R4))
+def test_batch_matches_oracle_and_replays(talsh_gpu, oracle, dk):
+    ts = talsh_gpu
+    rng = random.Random(2024 + dk)
+    ntask = 160
+    tasks = []
+    for t in range(ntask):
+        pat, dd, ld, rd = batch_contraction(rng)
+        L, R, D = tensor(dk, ld, 3 * t), tensor(dk, rd, 3 * t + 1), tensor(dk, dd, 3 * t + 2)
+        acc = t % 2
+        tasks.append((pat, dd, ld, rd, L, R, D, acc, upload(ts, dk, ld, L), upload(ts, dk, rd, R),
+                      upload(ts, dk, dd, D)))
+    b = ts.Batch()
+    for (pat, dd, ld, rd, L, R, D, acc, lt, rt, dt) in tasks:
+        assert b.add(pat, dt, lt, rt, alpha=0.5, accumulative=acc) == 0
+    tol = 1e-12 if dk == R8 else 1e-5
+    wants = [oracle.contract(dk, pat, dd, D, ld, L, rd, R, (0.5, 0.0), acc)
+             for (pat, dd, ld, rd, L, R, D, acc, lt, rt, dt) in tasks]
+    for rep in range(3):
+        say("batch execute", dk, "replay", rep)
+        tk = ts.Task()
+        assert b.execute(tk) == 0
+        assert tk.wait() == ts.TALSH_TASK_COMPLETED
+        assert tk.t.flops > 0
+        tk.destruct()
+        for i, (pat, dd, ld, rd, L, R, D, acc, lt, rt, dt) in enumerate(tasks):
+            got = dt.host_array()
+            # accumulative tasks add alpha*L*R once per execution
+            want = wants[i] + rep * (wants[i] - D) if acc else wants[i]
+            assert relerr(got, want) <= tol, (rep, i, pat)
+            # eager library work between replays: put the output back on the GPU
+            dt.place(ts.gpu_flat(0))
+    # same results as talshTensorContract on each task alone (non-accumulative ones)
+    for (pat, dd, ld, rd, L, R, D, acc, lt, rt, dt) in tasks[:40]:
+        if acc:
+            continue
+        alone = upload(ts, dk, dd, D)
+        ts.check(ts.contract(pat, alone, lt, rt, alpha=0.5, accumulative=0), "contract")
+        assert relerr(dt.host_array(), alone.host_array()) <= tol
+        dt.place(ts.gpu_flat(0))
+    st = b.stats()
+    assert st["executions"] == 3 and st["work_items"] > 0
+    b.destroy()
+
+
+def test_batch_rejects_aliased_outputs(talsh_gpu):
+    ts = talsh_gpu
+    d = upload(ts, R8, [4, 4], np.zeros(16))
+    l = upload(ts, R8, [4, 3], tensor(R8, [4, 3], 1))
+    r = upload(ts, R8, [3, 4], tensor(R8, [3, 4], 2))
+    b = ts.Batch()
+    for _ in range(2):
+        assert b.add("D(a,b)+=L(a,i)*R(i,b)", d, l, r) == 0
+    assert b.execute() == ts.TALSH_INVALID_ARGS
+    b.destroy()
