@@ -268,6 +268,254 @@ def bench_batch(oracle, dk, ntask=4096, reps=5, check_sample=128):
             "max_rel_err_sampled": worst}
 
 
+SERVE_MIX = ((R8, 0.6), (R4, 0.3), (C8, 0.1))
+SERVE_ELEM = {R4: 4, R8: 8, C8: 16}
+
+
+class PinnedBuffer:
+    """Pinned host memory taken from the library's host arena (body of a rank-2 R8 Host tensor)."""
+
+    def __init__(self, nbytes):
+        cols = 1 << 20
+        rows = max(1, (nbytes + 8 * cols - 1) // (8 * cols))
+        self.t = ts.Tensor(R8, [cols, rows], 0, noinit=True)
+        self.addr = self.t.host_address()
+        self.nbytes = 8 * cols * rows
+
+    def view(self, dtype, offset_bytes, count):
+        buf = (ctypes.c_char * (count * np.dtype(dtype).itemsize)).from_address(self.addr + offset_bytes)
+        return np.frombuffer(buf, dtype=dtype)
+
+    def fill_uniform(self, dtype, seed):
+        """uniform[-1,1) values of `dtype` over the whole buffer, written in place."""
+        rng = np.random.default_rng(seed)
+        n = self.nbytes // np.dtype(dtype).itemsize
+        step = 1 << 26
+        for s in range(0, n, step):
+            v = self.view(dtype, s * np.dtype(dtype).itemsize, min(step, n - s))
+            rng.random(dtype=dtype, out=v)
+            v *= 2
+            v -= 1
+
+
+class OutputRing:
+    """Output bodies for requests in flight, carved from one pinned region (first fit from a moving head)."""
+
+    def __init__(self, nbytes):
+        self.size, self.head, self.live = nbytes, 0, {}
+
+    def alloc(self, n):
+        n = (n + 255) // 256 * 256
+        p = self.head if self.head + n <= self.size else 0
+        for _ in range(len(self.live) + 2):
+            hit = [o + m for o, m in self.live.items() if p < o + m and o < p + n]
+            if not hit:
+                self.live[p] = n
+                self.head = p + n
+                return p
+            p = max(hit)
+            if p + n > self.size:
+                p = 0
+        return None
+
+    def free(self, p):
+        self.live.pop(p, None)
+
+
+def serve_stream(ntask, seed=5, ntemplates=48, reuse=0.3, vmin=1e3, vmax=1e8):
+    """Seeded request stream (the first n requests are the same for every n).
+
+    Each request: (pattern, ddims, ldims, rdims, dk, flop, template index or -1, L offset, R offset);
+    70 % bring both operands from host memory, 30 % contract a resident template L with a new R."""
+    import random
+    from tal_common import serving_request
+
+    rng = random.Random(seed)
+    kinds, weights = [k for k, _ in SERVE_MIX], [w for _, w in SERVE_MIX]
+
+    def fresh():
+        pat, dd, ld, rd, flop = serving_request(rng, vmin, vmax)
+        dk = rng.choices(kinds, weights)[0]
+        return pat, dd, ld, rd, dk, flop * (4.0 if dk == C8 else 1.0)
+
+    templates = [fresh() for _ in range(ntemplates)]
+    out = []
+    for _ in range(ntask):
+        if rng.random() < reuse:
+            ti = rng.randrange(ntemplates)
+            pat, dd, ld, rd, dk, flop = templates[ti]
+        else:
+            ti = -1
+            pat, dd, ld, rd, dk, flop = fresh()
+        out.append((pat, dd, ld, rd, dk, flop, ti, rng.random(), rng.random()))
+    return templates, out
+
+
+def bench_serve(oracle, ntask=4000, rate=None, budget=200 * 10 ** 9, max_inflight=32, ring_bytes=24 << 30,
+                check_max=24):
+    """Config 5: Poisson stream of contraction requests through talshAmdServer.
+
+    Operands come from pinned host memory (views into two seeded input buffers:
+    float64 for R8/C8, float32 for R4) or, for reuse requests, are resident
+    template tensors on the GPU; results return to pinned host memory (COPY_TTT).
+    Three runs over the same stream: one request at a time (submit, wait), a
+    closed loop keeping 2*max_inflight requests outstanding (capacity), and
+    Poisson arrivals at `rate` requests/s (default 0.8 x measured capacity).
+    Latency = completion observed - scheduled arrival."""
+    import random
+
+    templates, stream = serve_stream(ntask)
+    lib = ts.load()
+    free0 = lib.talshDeviceBufferFreeSize(0, ts.DEV_HOST)
+    in64 = PinnedBuffer(5 << 29)  # largest operand: 1.3e8 C8 elements = 2.08 GB
+    in32 = PinnedBuffer(5 << 27)  # largest R4 operand: 520 MB
+    ring = PinnedBuffer(ring_bytes)
+    if free0 - lib.talshDeviceBufferFreeSize(0, ts.DEV_HOST) < in64.nbytes + in32.nbytes + ring.nbytes:
+        raise RuntimeError("host buffers did not come from the pinned host arena (talshInit host_buf_size)")
+    in64.fill_uniform(np.float64, 1)
+    in32.fill_uniform(np.float32, 2)
+
+    def operand_view(dk, dims, frac):
+        vol = int(np.prod(dims))
+        buf, dt = (in32, np.float32) if dk == R4 else (in64, np.float64)
+        n = vol * (2 if dk == C8 else 1)
+        slots = (buf.nbytes // np.dtype(dt).itemsize - n) // 64
+        assert slots >= 0, (dk, dims)
+        off = 64 * int(frac * slots) * np.dtype(dt).itemsize
+        return buf.addr + off, (buf, dt, off, n)
+
+    def host_data(dk, info):
+        buf, dt, off, n = info
+        a = buf.view(dt, off, n).copy()
+        return a.view(np.complex128) if dk == C8 else a
+
+    # resident templates (left operands on the GPU)
+    tpl = []
+    for i, (pat, dd, ld, rd, dk, flop) in enumerate(templates):
+        addr, info = operand_view(dk, ld, (i + 0.5) / len(templates))
+        h = ts.Tensor(dk, ld, 0, ext=addr)
+        g = ts.Tensor(dk, ld, ts.gpu_flat(0))
+        lab = ",".join("i%d" % k for k in range(len(ld)))
+        ts.check(ts.copy("D(%s)=L(%s)" % (lab, lab), g, h), "template upload")
+        h.destruct()
+        tpl.append((g, info))
+    resident_bytes = sum(int(np.prod(t[2])) * SERVE_ELEM[t[4]] for t in templates)
+    ts.load().talshAmdDeviceSynchronize(0)
+
+    def make(i):
+        pat, dd, ld, rd, dk, flop, ti, fl, fr = stream[i]
+        if ti >= 0:
+            lt, linfo, lbytes = tpl[ti][0], tpl[ti][1], 0
+        else:
+            addr, linfo = operand_view(dk, ld, fl)
+            lt, lbytes = ts.Tensor(dk, ld, 0, ext=addr), int(np.prod(ld)) * SERVE_ELEM[dk]
+        addr, rinfo = operand_view(dk, rd, fr)
+        rt = ts.Tensor(dk, rd, 0, ext=addr)
+        dbytes = int(np.prod(dd)) * SERVE_ELEM[dk]
+        return pat, dk, lt, rt, linfo, rinfo, dbytes, lbytes + int(np.prod(rd)) * SERVE_ELEM[dk] + dbytes
+
+    want_check = set(i for i in range(ntask) if stream[i][5] <= 3e8)
+    want_check = set(sorted(want_check, key=lambda i: (i * 2654435761) % 1000003)[:check_max])
+    checked = {}
+
+    def run(n, arrivals=None, outstanding=None, collect=False):
+        rg = OutputRing(ring.nbytes)
+        srv = ts.Server(gpu=0, hbm_budget=budget, max_inflight=max_inflight)
+        live = {}
+        lat, flop, pcie = [], 0.0, 0
+        i = 0
+        t0 = time.perf_counter()
+        while i < n or live:
+            now = time.perf_counter() - t0
+            due = i < n and (arrivals[i] <= now if arrivals is not None else len(live) < outstanding)
+            off = rg.alloc(int(np.prod(stream[i][1])) * SERVE_ELEM[stream[i][4]]) if due else None
+            if off is not None:  # (a full output ring holds the request back: its wait counts as latency)
+                pat, dk, lt, rt, linfo, rinfo, dbytes, moved = make(i)
+                dt = ts.Tensor(dk, stream[i][1], 0, ext=ring.addr + off)
+                sched = arrivals[i] if arrivals is not None else now
+                ts.check(srv.submit(i, pat, dt, lt, rt, alpha=1.0, copy_ctrl=ts.coh("T", "T", "T"),
+                                    accumulative=0), "server submit")
+                live[i] = (sched, dt, lt, rt, off, linfo, rinfo, moved)
+                i += 1
+                continue
+            for tag, _, st in srv.progress():
+                if st != ts.TALSH_TASK_COMPLETED:
+                    raise RuntimeError("request %d failed with status %d" % (tag, st))
+                done = time.perf_counter() - t0
+                sched, dt, lt, rt, off, linfo, rinfo, moved = live.pop(tag)
+                lat.append(done - sched)
+                flop += stream[tag][5]
+                pcie += moved
+                if collect and tag in want_check:
+                    checked[tag] = (dt.host_array(), linfo, rinfo)
+                dt.destruct()
+                rt.destruct()
+                if stream[tag][6] < 0:
+                    lt.destruct()
+                rg.free(off)
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+        srv.destroy()
+        lat = np.array(lat)
+        return {"requests": n, "wall_s": wall, "requests_per_s": n / wall, "tflops": flop / wall / 1e12,
+                "p50_latency_ms": 1e3 * float(np.percentile(lat, 50)),
+                "p99_latency_ms": 1e3 * float(np.percentile(lat, 99)),
+                "mean_latency_ms": 1e3 * float(lat.mean()), "host_link_gbs": pcie / wall / 1e9,
+                "server_p99_queue_wait_ms": 1e3 * st["p99_queue_wait_s"],
+                "peak_transient_gb": st["peak_transient_bytes"] / 1e9, "deferred": int(st["deferred"])}
+
+    def serial(n):
+        t0 = time.perf_counter()
+        flop = 0.0
+        for i in range(n):
+            pat, dk, lt, rt, linfo, rinfo, dbytes, moved = make(i)
+            dt = ts.Tensor(dk, stream[i][1], 0, ext=ring.addr)
+            tk = ts.Task()
+            ts.check(ts.contract(pat, dt, lt, rt, copy_ctrl=ts.coh("T", "T", "T"), accumulative=0, task=tk),
+                     "contract")
+            tk.wait()
+            tk.destruct()
+            flop += stream[i][5]
+            for x in (dt, rt) + ((lt,) if stream[i][6] < 0 else ()):
+                x.destruct()
+        wall = time.perf_counter() - t0
+        return {"requests": n, "wall_s": wall, "requests_per_s": n / wall, "tflops": flop / wall / 1e12}
+
+    nser = min(ntask, 300)
+    log("serve: one at a time", nser)
+    one = serial(nser)
+    ncap = min(ntask, 1500)
+    log("serve: closed loop", ncap)
+    cap = run(ncap, outstanding=2 * max_inflight)
+    if rate is None:
+        rate = 0.8 * cap["requests_per_s"]
+    arng = random.Random(77)
+    arrivals, t = [], 0.0
+    for _ in range(ntask):
+        t += arng.expovariate(rate)
+        arrivals.append(t)
+    log("serve: poisson", ntask, "at", rate, "req/s")
+    pois = run(ntask, arrivals=arrivals, collect=True)
+    errs = {}
+    for tag, (got, linfo, rinfo) in checked.items():
+        pat, dd, ld, rd, dk, flop, ti, fl, fr = stream[tag]
+        L, R = host_data(dk, linfo), host_data(dk, rinfo)
+        want = oracle.contract(dk, pat, dd, np.zeros(int(np.prod(dd)), dtype=ts.NP_DTYPE[dk]), ld, L, rd, R,
+                               (1.0, 0.0), 0)
+        den = np.linalg.norm(want.astype(np.complex128)) or 1.0
+        e = float(np.linalg.norm((got - want).astype(np.complex128)) / den)
+        key = {R4: "r4", R8: "r8", C8: "c8"}[dk]
+        errs[key] = max(errs.get(key, 0.0), e)
+    for g, _ in tpl:
+        g.destruct()
+    total_flop = sum(s[5] for s in stream)
+    return {"requests": ntask, "rate_req_per_s": rate, "budget_gb": budget / 1e9, "max_inflight": max_inflight,
+            "stream_tflop": total_flop / 1e12, "resident_template_gb": resident_bytes / 1e9,
+            "poisson": pois, "closed_loop": cap, "one_at_a_time": one,
+            "overlap_speedup": cap["requests_per_s"] / one["requests_per_s"],
+            "checked_requests": len(checked), "max_rel_err_sampled": errs}
+
+
 def cpu_baseline(budget_s=30.0):
     """TAL-SH's own CPU path on a bounded sample of the headline contraction.
 
@@ -302,6 +550,11 @@ def cpu_baseline(budget_s=30.0):
                       "1.74e11 FLOP), median of %d runs, %.2f s each, OMP threads %d" % (len(times), t, cores)}
 
 
+def check_init(err):
+    if err:
+        raise RuntimeError("talshInit failed: %d" % err)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -309,7 +562,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only")
-    ap.add_argument("--only", default="", help="comma list of parts to run (head,r4,bf16,c8,config1,transpose,batch)")
+    ap.add_argument("--only", default="",
+                    help="comma list of parts to run (head,r4,bf16,c8,config1,transpose,batch,serve)")
+    ap.add_argument("--serve-requests", type=int, default=4000, help="serving stream length (full: 20000)")
+    ap.add_argument("--serve-rate", type=float, default=None, help="arrivals per second (default 0.8 x capacity)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1).", file=sys.stderr)
@@ -317,9 +573,7 @@ def main():
     from tal_common import Oracle
 
     oracle = Oracle()
-    err = ts.init(host_buf_size=4 << 30, dev_buffer_size=48 << 30)
-    if err:
-        raise RuntimeError("talshInit failed: %d" % err)
+    check_init(ts.init(host_buf_size=4 << 30, dev_buffer_size=48 << 30))
     pat3 = "D(a,b,c,d)+=L(a,i,c,j)*R(j,b,i,d)"
     only = set(x for x in args.only.split(",") if x)
     want = lambda part: (not args.quick and not only) or part in only
@@ -342,12 +596,16 @@ def main():
     if want("batch"):
         secondary["batched_config4_r8"] = bench_batch(oracle, R8)
         secondary["batched_config4_r4"] = bench_batch(oracle, R4)
+    ts.shutdown()
+    if want("serve"):
+        # 200 GB admission budget: the device pool holds it plus the resident templates
+        check_init(ts.init(host_buf_size=29 << 30, dev_buffer_size=224 << 30))
+        secondary["serving_config5"] = bench_serve(oracle, ntask=args.serve_requests, rate=args.serve_rate)
+        ts.shutdown()
     log("secondary", secondary)
     if not head:
-        ts.shutdown()
         print(json.dumps({"secondary": secondary}), flush=True)
         return
-    ts.shutdown()
     cpu = None if args.no_cpu else cpu_baseline()
     gemm_s = head["gemm_ms"] * 1e-3
     achieved = head["flop_per_step"] / gemm_s / 1e12

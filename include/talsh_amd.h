@@ -69,6 +69,45 @@ int talshAmdBatchStats(const talshAmdBatch *batch, double *flops, long long *wor
                        double *tune_ms, int *executions);
 int talshAmdBatchDestroy(talshAmdBatch *batch);
 
+/* ---- Serving: continuous admission of contraction requests ----
+ * Requests are talshTensorContract operations submitted at any time.  They
+ * are admitted onto the GPU, in arrival order (a waiting head may be
+ * overtaken by smaller requests at most 64 times), whenever fewer than
+ * max_inflight requests are running and the request's transient device
+ * memory (operands it has to bring to the GPU + transposed temporaries) fits
+ * in what is left of hbm_budget.  Admitted requests run asynchronously and
+ * overlap on the GPU's streams; talshAmdServerProgress reports completions
+ * (tag, latency from submission, task status) and admits more work; it
+ * reports at most max_out completions per call (the rest on later calls).
+ * A request that cannot run even on an idle device is reported with status
+ * TALSH_TASK_ERROR.  Submit returns DEVICE_UNABLE for a request whose
+ * transient need exceeds the whole budget.
+ * Operands already resident on the GPU are used in place and cost no budget.
+ * The caller keeps every tensor of a request alive and untouched until its
+ * completion is reported. */
+typedef struct talshAmdServer talshAmdServer;
+typedef struct {
+  const char *pattern;
+  talsh_tens_t *dtens;
+  talsh_tens_t *ltens;
+  talsh_tens_t *rtens;
+  double scale_real;
+  double scale_imag;
+  int copy_ctrl;    /* as for talshTensorContract */
+  int accumulative; /* YEP / NOPE */
+  long long tag;    /* caller's id, returned on completion */
+} talshAmdRequest;
+int talshAmdServerCreate(talshAmdServer **server, int gpu, size_t hbm_budget, int max_inflight);
+int talshAmdServerSubmit(talshAmdServer *server, const talshAmdRequest *request);
+int talshAmdServerProgress(talshAmdServer *server, int max_out, long long *tags, double *latency_s, int *status,
+                           int *nout);
+int talshAmdServerPending(const talshAmdServer *server, int *queued, int *inflight);
+/* out[12] = {submitted, completed, failed, deferred, p50 latency s, p99 latency s, mean latency s,
+ *            p50 queue wait s, p99 queue wait s, TFLOP/s over first arrival..last completion,
+ *            peak transient bytes, budget bytes} */
+int talshAmdServerStats(const talshAmdServer *server, double *out, int nout);
+int talshAmdServerDestroy(talshAmdServer *server);
+
 /* Copies a short build description into buf. */
 int talshAmdBuildInfo(char *buf, int len);
 

@@ -138,6 +138,12 @@ SIGNATURES = {
     "talshAmdBatchExecute": (_I, [_P, _I, _I, _KP]),
     "talshAmdBatchStats": (_I, [_P, _DP, ctypes.POINTER(ctypes.c_longlong), _IP, _DP, _IP]),
     "talshAmdBatchDestroy": (_I, [_P]),
+    "talshAmdServerCreate": (_I, [ctypes.POINTER(_P), _I, _S, _I]),
+    "talshAmdServerSubmit": (_I, [_P, _P]),
+    "talshAmdServerProgress": (_I, [_P, _I, ctypes.POINTER(ctypes.c_longlong), _DP, _IP, _IP]),
+    "talshAmdServerPending": (_I, [_P, _IP, _IP]),
+    "talshAmdServerStats": (_I, [_P, _DP, _I]),
+    "talshAmdServerDestroy": (_I, [_P]),
 }
 
 _lib = None
@@ -180,6 +186,11 @@ def shutdown():
     return load().talshShutdown()
 
 
+# talsh_tens_init_i that leaves the body as allocated
+_INIT_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)
+_NOINIT = _INIT_FN(lambda data, shape, signature: 0)
+
+
 def gpu_flat(n=0):
     return load().talshFlatDevId(DEV_NVIDIA_GPU, n)
 
@@ -187,7 +198,8 @@ def gpu_flat(n=0):
 class Tensor:
     """Owns one talsh_tens_t (caller-allocated struct, as in the reference tests)."""
 
-    def __init__(self, dk, dims, dev_id=0, init_val=0.0, data=None, ext=None):
+    def __init__(self, dk, dims, dev_id=0, init_val=0.0, data=None, ext=None, noinit=False):
+        """ext: external body (numpy array or raw address); noinit: leave a new Host body uninitialised."""
         self.lib = load()
         self.t = TalshTens()
         self.dk = dk
@@ -195,8 +207,9 @@ class Tensor:
         check(self.lib.talshTensorClean(ctypes.byref(self.t)), "talshTensorClean")
         d = (ctypes.c_int * max(1, len(dims)))(*dims)
         iv = complex(init_val)
-        err = self.lib.talshTensorConstruct(ctypes.byref(self.t), dk, len(dims), d, dev_id,
-                                            ext.ctypes.data if ext is not None else None, -1, None,
+        addr = ext if isinstance(ext, int) else (ext.ctypes.data if ext is not None else None)
+        init = ctypes.cast(_NOINIT, ctypes.c_void_p) if noinit else None
+        err = self.lib.talshTensorConstruct(ctypes.byref(self.t), dk, len(dims), d, dev_id, addr, -1, init,
                                             iv.real, iv.imag)
         self._ext = ext
         if err != 0:
@@ -216,6 +229,13 @@ class Tensor:
         a = np.ascontiguousarray(np.asarray(data).reshape(-1), dtype=NP_DTYPE[self.dk])
         assert a.size == self.volume()
         check(self.lib.talshTensorImportData(self.ptr, self.dk, a.ctypes.data), "talshTensorImportData")
+
+    def host_address(self):
+        """Address of the Host image body (talshTensorGetBodyAccess)."""
+        body = ctypes.c_void_p()
+        check(self.lib.talshTensorGetBodyAccess(self.ptr, ctypes.byref(body), self.dk, 0, DEV_HOST),
+              "talshTensorGetBodyAccess")
+        return body.value
 
     def host_array(self):
         """Copy of the Host image (moves the tensor to Host first if needed)."""
@@ -312,6 +332,66 @@ class Batch:
         if self.h:
             check(self.lib.talshAmdBatchDestroy(self.h), "talshAmdBatchDestroy")
             self.h = ctypes.c_void_p()
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("pattern", ctypes.c_char_p), ("dtens", _TP), ("ltens", _TP), ("rtens", _TP),
+                ("scale_real", ctypes.c_double), ("scale_imag", ctypes.c_double), ("copy_ctrl", ctypes.c_int),
+                ("accumulative", ctypes.c_int), ("tag", ctypes.c_longlong)]
+
+
+SERVER_STATS = ("submitted", "completed", "failed", "deferred", "p50_latency_s", "p99_latency_s",
+                "mean_latency_s", "p50_queue_wait_s", "p99_queue_wait_s", "tflops", "peak_transient_bytes",
+                "budget_bytes")
+
+
+class Server:
+    """talshAmdServer handle: continuous admission of contraction requests."""
+
+    def __init__(self, gpu=0, hbm_budget=200 << 30, max_inflight=32):
+        self.lib = load()
+        self.h = ctypes.c_void_p()
+        check(self.lib.talshAmdServerCreate(ctypes.byref(self.h), gpu, hbm_budget, max_inflight), "ServerCreate")
+        self._keep = {}
+        n = 4096
+        self._tags = (ctypes.c_longlong * n)()
+        self._lat = (ctypes.c_double * n)()
+        self._st = (ctypes.c_int * n)()
+
+    def submit(self, tag, pattern, d, l, r, alpha=1.0, copy_ctrl=None, accumulative=YEP):
+        a = complex(alpha)
+        pb = pattern.encode()
+        req = Request(pb, ctypes.pointer(d.t), ctypes.pointer(l.t), ctypes.pointer(r.t), a.real, a.imag,
+                      coh("T", "T", "T") if copy_ctrl is None else copy_ctrl, accumulative, tag)
+        err = self.lib.talshAmdServerSubmit(self.h, ctypes.byref(req))
+        if err == 0:
+            self._keep[tag] = (pb, d, l, r)
+        return err
+
+    def progress(self):
+        n = ctypes.c_int()
+        check(self.lib.talshAmdServerProgress(self.h, len(self._tags), self._tags, self._lat, self._st,
+                                              ctypes.byref(n)), "ServerProgress")
+        out = [(self._tags[i], self._lat[i], self._st[i]) for i in range(n.value)]
+        for tag, _, _ in out:
+            self._keep.pop(tag, None)
+        return out
+
+    def pending(self):
+        q, f = ctypes.c_int(), ctypes.c_int()
+        self.lib.talshAmdServerPending(self.h, ctypes.byref(q), ctypes.byref(f))
+        return q.value, f.value
+
+    def stats(self):
+        v = (ctypes.c_double * 12)()
+        check(self.lib.talshAmdServerStats(self.h, v, 12), "ServerStats")
+        return dict(zip(SERVER_STATS, list(v)))
+
+    def destroy(self):
+        if self.h:
+            check(self.lib.talshAmdServerDestroy(self.h), "ServerDestroy")
+            self.h = ctypes.c_void_p()
+            self._keep.clear()
 
 
 def contract(pattern, d, l, r, alpha=1.0, dev_id=DEV_DEFAULT, dev_kind=DEV_DEFAULT, copy_ctrl=None,

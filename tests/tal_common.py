@@ -216,3 +216,55 @@ class Reference:
                                  ctypes.c_double(alpha[0]), ctypes.c_double(alpha[1]))
         assert err == 0, err
         return d
+
+
+def _split_volume(rng, vol, n, lo=2):
+    """n extents (each >= lo) whose product is close to vol."""
+    if n == 0:
+        return []
+    dims = []
+    rem = float(vol)
+    for i in range(n - 1):
+        base = rem ** (1.0 / (n - i))
+        e = max(lo, int(round(base * 2.0 ** rng.uniform(-0.5, 0.5))))
+        dims.append(e)
+        rem = max(1.0, rem / e)
+    dims.append(max(lo, int(round(rem))))
+    return dims
+
+
+def serving_request(rng, vmin=1e3, vmax=1e8):
+    """One serving task: operand volumes log-uniform in [vmin, vmax] elements,
+    operand ranks 2-6, D rank 2-6, 1+ contracted indices, D volume <= vmax.
+    Returns (pattern, ddims, ldims, rdims, flop)."""
+    import math
+    while True:
+        rl, rr = rng.randint(2, 6), rng.randint(2, 6)
+        cs = [c for c in range(1, min(rl, rr) + 1) if 2 <= rl + rr - 2 * c <= 6 and c < max(rl, rr)]
+        if not cs:
+            continue
+        c = rng.choice(cs)
+        nl, nr = rl - c, rr - c
+        vl = math.exp(rng.uniform(math.log(vmin), math.log(vmax)))
+        vr = math.exp(rng.uniform(math.log(vmin), math.log(vmax)))
+        kmin = max(2.0 ** c, math.sqrt(vl * vr / vmax))
+        kmax = min(vl / 2.0 ** nl, vr / 2.0 ** nr)
+        if kmin > kmax:
+            continue
+        K = math.exp(rng.uniform(math.log(kmin), math.log(kmax)))
+        kd = _split_volume(rng, K, c)
+        Kv = float(np.prod(kd))
+        md = _split_volume(rng, vl / Kv, nl)
+        nd = _split_volume(rng, vr / Kv, nr)
+        M, N = float(np.prod(md)) if md else 1.0, float(np.prod(nd)) if nd else 1.0
+        if M * Kv > 1.3 * vmax or N * Kv > 1.3 * vmax or M * N > 1.3 * vmax:
+            continue
+        labels = [chr(ord("a") + i) for i in range(nl + nr + c)]
+        lu, ru, cc = labels[:nl], labels[nl:nl + nr], labels[nl + nr:]
+        ext = dict(zip(lu + ru + cc, md + nd + kd))
+        d, l, r = lu + ru, lu + cc, ru + cc
+        rng.shuffle(d)
+        rng.shuffle(l)
+        rng.shuffle(r)
+        pat = "D(%s)+=L(%s)*R(%s)" % (",".join(d), ",".join(l), ",".join(r))
+        return pat, [ext[x] for x in d], [ext[x] for x in l], [ext[x] for x in r], 2.0 * M * N * Kv
