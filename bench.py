@@ -566,6 +566,8 @@ def main():
                     help="comma list of parts to run (head,r4,bf16,c8,config1,transpose,batch,serve)")
     ap.add_argument("--serve-requests", type=int, default=4000, help="serving stream length (full: 20000)")
     ap.add_argument("--serve-rate", type=float, default=None, help="arrivals per second (default 0.8 x capacity)")
+    ap.add_argument("--serve-streams", type=int, default=None, help="streams per GPU for the serving run")
+    ap.add_argument("--serve-inflight", type=int, default=32, help="requests in flight (server slots)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1).", file=sys.stderr)
@@ -599,8 +601,9 @@ def main():
     ts.shutdown()
     if want("serve"):
         # 200 GB admission budget: the device pool holds it plus the resident templates
-        check_init(ts.init(host_buf_size=29 << 30, dev_buffer_size=224 << 30))
-        secondary["serving_config5"] = bench_serve(oracle, ntask=args.serve_requests, rate=args.serve_rate)
+        check_init(ts.init(host_buf_size=29 << 30, dev_buffer_size=224 << 30, nstreams=args.serve_streams))
+        secondary["serving_config5"] = bench_serve(oracle, ntask=args.serve_requests, rate=args.serve_rate,
+                                                     max_inflight=args.serve_inflight)
         ts.shutdown()
     log("secondary", secondary)
     if not head:

@@ -93,6 +93,7 @@ class Op {
       if (e) return e;
       temps_.push_back(p);
       temp_handles_.push_back(h);
+      temp_bytes_.push_back(b);
     }
     return TALSH_SUCCESS;
   }
@@ -102,7 +103,15 @@ class Op {
   int begin(int data_kind, int coh_ctrl) {
     Device &d = rt().gpus[g_];
     (void)hipSetDevice(d.hip_id);
-    task_ = task_new(g_);
+    // stream choice weight: host-link bytes, plus device bytes at the HBM/link speed ratio
+    double weight = 0.0;
+    for (auto &o : ops_) {
+      const bool xfer_in = !o.resident && o.read;
+      const bool xfer_out = !o.resident && o.output && (o.coh == COPY_T || o.coh == COPY_K);
+      weight += (double)o.bytes * ((xfer_in ? 1.0 : 0.0) + (xfer_out ? 1.0 : 0.0) + 1.0 / 64.0);
+    }
+    for (size_t i = 0; i < temps_.size(); ++i) weight += 2.0 * (double)temp_bytes_[i] / 64.0;
+    task_ = task_new(g_, weight);
     s_ = task_->stream;
     (void)hipEventRecord(task_->ev_start->ev, s_);
     for (auto &o : ops_) {
@@ -264,6 +273,7 @@ class Op {
   std::vector<Operand> ops_;
   std::vector<void *> temps_;
   std::vector<int> temp_handles_;
+  std::vector<size_t> temp_bytes_;
   Task *task_ = nullptr;
   hipStream_t s_ = nullptr;
   bool committed_ = false;
@@ -598,7 +608,7 @@ int talshTensorPlace(talsh_tens_t *tens, int dev_id, int dev_kind, void *dev_mem
   }
   Device &d = rt().gpus[g];
   (void)hipSetDevice(d.hip_id);
-  Task *task = task_new(g);
+  Task *task = task_new(g, (double)bytes);
   hipStream_t s = task->stream;
   void *sp = tens->dev_rsc[src].gmem_p;
   (void)hipEventRecord(task->ev_start->ev, s);

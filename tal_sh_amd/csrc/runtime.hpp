@@ -8,6 +8,7 @@
 #include <memory>
 #include <mutex>
 #include <unordered_map>
+#include <deque>
 #include <vector>
 
 #include "kernels.hpp"
@@ -42,7 +43,8 @@ struct Device {
   size_t mem_total = 0;
   Arena arena;
   std::vector<hipStream_t> streams;
-  unsigned rr = 0;
+  // per stream: (completion event, weight) of tasks not yet seen complete, oldest first
+  std::vector<std::deque<std::pair<EvRef, double>>> backlog;
   std::vector<hipEvent_t> free_events;
   // statistics (talsh_stats_t of the reference, device_algebra.h:183-193)
   long long tasks_submitted = 0, tasks_completed = 0, tasks_deferred = 0, tasks_failed = 0;
@@ -106,7 +108,9 @@ int image_find(const talsh_tens_t *t, int flat_dev, int dk, bool need_avail);
 int tensor_healthy(const talsh_tens_t *t);
 
 // Task lifecycle.
-Task *task_new(int g);
+// New task on the stream of GPU g with the least outstanding work; `weight`
+// estimates the task's own work in host-link-equivalent bytes.
+Task *task_new(int g, double weight = 0.0);
 void task_free(Task *t);
 int task_finalize_if_done(talsh_task_t *tt, bool block);
 
