@@ -430,6 +430,197 @@ __global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(2)
       }
 }
 
+// ---------------------------------------------------------------------------
+// Pre-split bf16x6 GEMM.  split_planes_kernel writes the three BF16 planes of
+// an operand once into a workspace (row r, column k of a plane at r*Kp + k;
+// rows padded to the GEMM tile, K padded to 32, zeros in the padding), so the
+// GEMM's inner loop only moves bytes and issues MFMAs: no split arithmetic
+// (which the on-the-fly kernel above repeats for every C tile that reads an
+// element) and no bounds checks.  The dominant product ah*bh goes to one
+// accumulator and the five small ones to a second, so the main accumulator is
+// rounded once per K step of 32 without any VALU adds; the two are summed in
+// the epilogue.
+//
+// Block tile 256 x 128, 8 waves of 64 x 64 (two per SIMD), BK = 32.  LDS
+// holds two stages of both operands (2 x (48 + 24) KB); a plane row is 64 bytes
+// (four 16-byte chunks) and chunk c of row r is stored at slot c ^ ((r >> 2) & 3),
+// so the 16-byte fragment reads of 16 consecutive rows cover all 64 banks.
+// The next K step is loaded into registers while the current one is multiplied;
+// one barrier per K step.
+__device__ inline void split3_finite(float x, __bf16 &h, __bf16 &m, __bf16 &l) {
+  h = (__bf16)x;
+  if (!__builtin_isfinite(x)) {  // Inf/NaN carried by the leading term alone
+    m = l = (__bf16)0.0f;
+    return;
+  }
+  const float r = x - (float)h;
+  m = (__bf16)r;
+  l = (__bf16)(r - (float)m);
+}
+
+// KC = 1: source element (r, k) at X[r*ld + k]; KC = 0: at X[r + k*ld].
+// One thread per 8 consecutive k of one row (KC = 1: lanes along k, KC = 0:
+// lanes along r, so source reads are coalesced either way).
+template <int KC>
+__global__ __launch_bounds__(256) void split_planes_kernel(const float *__restrict__ X, int64_t ld, int64_t R,
+                                                           int64_t K, int64_t Rp, int64_t Kp,
+                                                           __bf16 *__restrict__ P, int64_t plane, int vec) {
+  const int64_t kcn = Kp / 8;
+  const int64_t item = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (item >= Rp * kcn) return;
+  const int64_t r = KC ? item / kcn : item % Rp, kc = KC ? item % kcn : item / Rp;
+  const int64_t k0 = kc * 8;
+  float v[8];
+  if (KC && vec && r < R && k0 + 8 <= K) {
+    const float4 a = *reinterpret_cast<const float4 *>(X + r * ld + k0);
+    const float4 b = *reinterpret_cast<const float4 *>(X + r * ld + k0 + 4);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+  } else {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int64_t k = k0 + i;
+      v[i] = (r < R && k < K) ? (KC ? X[r * ld + k] : X[r + k * ld]) : 0.0f;
+    }
+  }
+  __bf16 h[8], m[8], l[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) split3_finite(v[i], h[i], m[i], l[i]);
+  uint4 u;
+  const int64_t off = r * Kp + k0;
+  __builtin_memcpy(&u, h, 16);
+  *reinterpret_cast<uint4 *>(P + off) = u;
+  __builtin_memcpy(&u, m, 16);
+  *reinterpret_cast<uint4 *>(P + plane + off) = u;
+  __builtin_memcpy(&u, l, 16);
+  *reinterpret_cast<uint4 *>(P + 2 * plane + off) = u;
+}
+
+constexpr int PBM = 256, PBN = 128, PBK = 32;
+
+__device__ inline int pslot(int r, int c) { return r * 4 + (c ^ ((r >> 2) & 3)); }
+
+__global__ __launch_bounds__(512) void gemm_bf16x6p_kernel(int64_t M, int64_t N, int64_t Kp,
+                                                           const uint4 *__restrict__ Ap, int64_t aplane,
+                                                           const uint4 *__restrict__ Bp, int64_t bplane,
+                                                           float *__restrict__ C, int64_t ldc, float alpha,
+                                                           int beta_one, int64_t gx, int64_t gy) {
+  // aplane / bplane / row strides in 16-byte chunks
+  __shared__ uint4 As[2][3][PBM * 4];
+  __shared__ uint4 Bs[2][3][PBN * 4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 3, wn = wave >> 2;
+  int64_t bx, by;
+  tile_of(gx, gy, bx, by);
+  const int64_t kpc = Kp / 8;  // chunks per plane row
+  // thread tid loads chunk (tid & 3) of row (tid >> 2) [+128] of each plane
+  const int lr = tid >> 2, lc = tid & 3;
+  const uint4 *ag = Ap + (bx * PBM + lr) * kpc + lc;
+  const uint4 *bg = Bp + (by * PBN + lr) * kpc + lc;
+  const int ls = pslot(lr, lc);
+  const int64_t a1 = 128 * kpc;
+  // prefetch registers: A planes t = 0,1,2 rows lr and lr+128; B planes t = 0,1,2 row lr
+  uint4 a0, a1r, a2, a3, a4, a5, b0, b1, b2;
+#define TAL_PLOAD(kc)                                                                  \
+  a0 = ag[(kc)];                                                                       \
+  a1r = ag[a1 + (kc)];                                                                 \
+  a2 = ag[aplane + (kc)];                                                              \
+  a3 = ag[aplane + a1 + (kc)];                                                         \
+  a4 = ag[2 * aplane + (kc)];                                                          \
+  a5 = ag[2 * aplane + a1 + (kc)];                                                     \
+  b0 = bg[(kc)];                                                                       \
+  b1 = bg[bplane + (kc)];                                                              \
+  b2 = bg[2 * bplane + (kc)];
+#define TAL_PSTORE(st)                                                                 \
+  As[st][0][ls] = a0;                                                                  \
+  As[st][0][ls + 512] = a1r;                                                           \
+  As[st][1][ls] = a2;                                                                  \
+  As[st][1][ls + 512] = a3;                                                            \
+  As[st][2][ls] = a4;                                                                  \
+  As[st][2][ls + 512] = a5;                                                            \
+  Bs[st][0][ls] = b0;                                                                  \
+  Bs[st][1][ls] = b1;                                                                  \
+  Bs[st][2][ls] = b2;
+
+  f32x4 big[4][4], small[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) big[i][j] = small[i][j] = f32x4{0, 0, 0, 0};
+
+  // fragment slot of row 16*x + (lane & 15), chunk lane >> 4
+  const int fs = pslot(lane & 15, lane >> 4);
+  const int64_t nk = Kp / PBK;
+  TAL_PLOAD(0)
+  TAL_PSTORE(0)
+  __syncthreads();
+  for (int64_t kt = 0; kt < nk; ++kt) {
+    const int cur = (int)(kt & 1);
+    if (kt + 1 < nk) {
+      TAL_PLOAD((kt + 1) * (PBK / 8))
+    }
+    bf16x8 fb[3][4];
+#pragma unroll
+    for (int t = 0; t < 3; ++t)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        fb[t][j] = __builtin_bit_cast(bf16x8, Bs[cur][t][(wn * 64 + j * 16) * 4 + fs]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      bf16x8 fa[3];
+#pragma unroll
+      for (int t = 0; t < 3; ++t) fa[t] = __builtin_bit_cast(bf16x8, As[cur][t][(wm * 64 + i * 16) * 4 + fs]);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) small[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[2], fb[0][j], small[i][j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) small[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[1], fb[1][j], small[i][j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) small[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[0], fb[2][j], small[i][j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) small[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[1], fb[0][j], small[i][j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) small[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[0], fb[1][j], small[i][j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) big[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[0], fb[0][j], big[i][j], 0, 0, 0);
+    }
+    if (kt + 1 < nk) {
+      TAL_PSTORE(cur ^ 1)
+    }
+    __syncthreads();
+  }
+#undef TAL_PLOAD
+#undef TAL_PSTORE
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t m = bx * PBM + wm * 64 + i * 16 + 4 * (lane >> 4) + r;
+        const int64_t n = by * PBN + wn * 64 + j * 16 + (lane & 15);
+        if (m < M && n < N) {
+          float *c = C + m + n * ldc;
+          const float v = alpha * (big[i][j][r] + small[i][j][r]);
+          *c = beta_one ? *c + v : v;
+        }
+      }
+}
+
+int64_t round_up(int64_t x, int64_t q) { return (x + q - 1) / q * q; }
+
+// Splits op(X) (rows R, depth K) into planes at P; returns 0 or a launch error.
+int split_operand(bool kcont, const float *X, int64_t ld, int64_t R, int64_t K, int64_t Rp, int64_t Kp, __bf16 *P,
+                  hipStream_t s) {
+  const int64_t items = Rp * (Kp / 8), nblk = (items + 255) / 256;
+  if (nblk > 0x7fffffffLL) return -4;
+  const int vec = ((uintptr_t)X % 16 == 0) && ld % 4 == 0;
+  if (kcont)
+    split_planes_kernel<1><<<dim3((unsigned)nblk), dim3(256), 0, s>>>(X, ld, R, K, Rp, Kp, P, Rp * Kp, vec);
+  else
+    split_planes_kernel<0><<<dim3((unsigned)nblk), dim3(256), 0, s>>>(X, ld, R, K, Rp, Kp, P, Rp * Kp, vec);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
 template <int OPA, int OPB>
 int launch_bf16x6(int64_t M, int64_t N, int64_t K, const void *A, int64_t lda, const void *B, int64_t ldb, void *C,
                   int64_t ldc, double ar, int beta_one, hipStream_t s) {
@@ -506,6 +697,34 @@ int op_code(char t) {
 }  // namespace
 
 bool gemm_split_bf16_available() { return true; }
+
+size_t gemm_split_workspace(int64_t M, int64_t N, int64_t K) {
+  const int64_t Mp = round_up(std::max<int64_t>(M, 1), PBM), Np = round_up(std::max<int64_t>(N, 1), PBN);
+  const int64_t Kp = round_up(std::max<int64_t>(K, 1), PBK);
+  return (size_t)(round_up(6 * Mp * Kp, 256) + 6 * Np * Kp);
+}
+
+int launch_gemm_split(char ta, char tb, int64_t M, int64_t N, int64_t K, double ar, const void *A, int64_t lda,
+                      const void *B, int64_t ldb, int beta_one, void *C, int64_t ldc, void *ws, hipStream_t s) {
+  const int opa = op_code(ta), opb = op_code(tb);
+  if (opa < 0 || opb < 0 || M < 0 || N < 0 || K < 0 || !A || !B || !C || !ws || (uintptr_t)ws % 16) return -1;
+  if (M == 0 || N == 0) return 0;
+  if (ldc < std::max<int64_t>(1, M)) return -1;
+  if (lda < std::max<int64_t>(1, opa == 0 ? M : K)) return -1;
+  if (ldb < std::max<int64_t>(1, opb == 0 ? K : N)) return -1;
+  const int64_t Mp = round_up(M, PBM), Np = round_up(N, PBN), Kp = round_up(std::max<int64_t>(K, 1), PBK);
+  __bf16 *pa = (__bf16 *)ws;
+  __bf16 *pb = (__bf16 *)((char *)ws + round_up(6 * Mp * Kp, 256));
+  // A 'T' (KxM) and B 'N' (KxN) already have K contiguous per row of the plane
+  if (int e = split_operand(opa == 1, (const float *)A, lda, M, K, Mp, Kp, pa, s)) return e;
+  if (int e = split_operand(opb == 0, (const float *)B, ldb, N, K, Np, Kp, pb, s)) return e;
+  const int64_t gx = Mp / PBM, gy = Np / PBN;
+  if (gx * gy > 0x7fffffffLL) return -4;
+  gemm_bf16x6p_kernel<<<dim3((unsigned)(gx * gy)), dim3(512), 0, s>>>(
+      M, N, Kp, (const uint4 *)pa, Mp * Kp / 8, (const uint4 *)pb, Np * Kp / 8, (float *)C, ldc, (float)ar,
+      beta_one, gx, gy);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
 
 int launch_gemm(int dk, int r4mode, char ta, char tb, int conja, int conjb, int64_t M, int64_t N, int64_t K,
                 double ar, double ai, const void *A, int64_t lda, const void *B, int64_t ldb, int beta_one, void *C,

@@ -498,6 +498,10 @@ int talshTensorContract(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *lt
   if (plan.a.transpose) { ta = (int)temps.size(); temps.push_back(op.operand(xi).bytes); }
   if (plan.b.transpose) { tb = (int)temps.size(); temps.push_back(op.operand(yi).bytes); }
   if (plan.d_transpose) { td = (int)temps.size(); temps.push_back(op.operand(0).bytes); }
+  // split-BF16 R4: workspace for the BF16 planes of both GEMM operands
+  const bool split = (dk == R4 && rt().r4mode == R4_SPLIT_BF16);
+  int tw = -1;
+  if (split) { tw = (int)temps.size(); temps.push_back(gemm_split_workspace(plan.M, plan.N, plan.K)); }
   if (int e = op.acquire(temps)) return deferred(g, e);
   if (int e = op.begin(dk, copy_ctrl)) return e;
   hipStream_t s = op.stream();
@@ -528,10 +532,15 @@ int talshTensorContract(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *lt
   if (p.conj & 1) { lconj ^= 1; rconj ^= 1; }
   if (!cplx) lconj = rconj = 0;
   op.mark_gemm(true);
-  if (launch_gemm(dk, rt().r4mode, plan.a.op, plan.b.op, plan.swap ? rconj : lconj, plan.swap ? lconj : rconj,
-                  plan.M, plan.N, plan.K, scale_real, cplx ? scale_imag : 0.0, A, plan.a.ld, B, plan.b.ld,
-                  acc ? 1 : 0, C, std::max<int64_t>(1, plan.M), s))
+  if (split) {
+    if (launch_gemm_split(plan.a.op, plan.b.op, plan.M, plan.N, plan.K, scale_real, A, plan.a.ld, B, plan.b.ld,
+                          acc ? 1 : 0, C, std::max<int64_t>(1, plan.M), op.temp(tw), s))
+      return TALSH_FAILURE;
+  } else if (launch_gemm(dk, R4_FP32, plan.a.op, plan.b.op, plan.swap ? rconj : lconj, plan.swap ? lconj : rconj,
+                         plan.M, plan.N, plan.K, scale_real, cplx ? scale_imag : 0.0, A, plan.a.ld, B, plan.b.ld,
+                         acc ? 1 : 0, C, std::max<int64_t>(1, plan.M), s)) {
     return TALSH_FAILURE;
+  }
   op.mark_gemm(false);
   if (td >= 0) {
     if (launch_permute(dk, C, D.dptr, (int)plan.dprime_dims.size(), plan.dprime_dims.data(),

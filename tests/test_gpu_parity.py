@@ -205,7 +205,12 @@ def test_contract_split_bf16_r4(talsh_gpu, oracle):
     assert lib.talshQueryFastMath(ts.DEV_NVIDIA_GPU, 0) == ts.YEP
     try:
         rng = random.Random(7)
-        cases = [("D(a,b,c,d)+=L(a,i,c,j)*R(j,b,i,d)", [24, 20, 24, 20], [24, 40, 24, 36], [36, 20, 40, 20])]
+        cases = [("D(a,b,c,d)+=L(a,i,c,j)*R(j,b,i,d)", [24, 20, 24, 20], [24, 40, 24, 36], [36, 20, 40, 20]),
+                 # all four matrix forms of the GEMM operands, extents off the 256/128/32 tile grid
+                 ("D(a,b)+=L(a,k)*R(k,b)", [300, 200], [300, 100], [100, 200]),
+                 ("D(a,b)+=L(k,a)*R(b,k)", [257, 129], [45, 257], [129, 45]),
+                 ("D(a,b)+=L(a,k)*R(b,k)", [513, 260], [513, 77], [260, 77]),
+                 ("D(a,b)+=L(k,a)*R(k,b)", [31, 700], [300, 31], [300, 700])]
         for _ in range(20):
             pat, dd, ld, rd = random_contraction(rng, max_rank=4, ext_lo=1, ext_hi=13)
             cases.append((pat, dd, ld, rd))
