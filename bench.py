@@ -22,6 +22,11 @@ import time
 
 import numpy as np
 
+# The serving run overlaps requests on 16 streams, one hardware queue each;
+# HIP reads this when it starts, so it is set before the library is loaded.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
+
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 sys.path.insert(0, os.path.join(REPO, "tests"))
@@ -566,7 +571,7 @@ def main():
                     help="comma list of parts to run (head,r4,bf16,c8,config1,transpose,batch,serve)")
     ap.add_argument("--serve-requests", type=int, default=4000, help="serving stream length (full: 20000)")
     ap.add_argument("--serve-rate", type=float, default=None, help="arrivals per second (default 0.8 x capacity)")
-    ap.add_argument("--serve-streams", type=int, default=None, help="streams per GPU for the serving run")
+    ap.add_argument("--serve-streams", type=int, default=16, help="streams per GPU for the serving run")
     ap.add_argument("--serve-inflight", type=int, default=32, help="requests in flight (server slots)")
     args = ap.parse_args()
     if args.gpus != 1:
