@@ -111,6 +111,7 @@ struct PermArgs {
   int64_t osa, osb, osc;       // output strides of a, b, c
   int64_t ntile_a, ntile_b, ntile_c;
   int tc;                      // tile size along c (<= TCMAX of the kernel)
+  int tb_log;                  // row mode: log2 of the tile size along b
   int64_t oe[kMaxRank];        // outer extents
   int64_t ois[kMaxRank];       // outer input strides
   int64_t oos[kMaxRank];       // outer output strides
@@ -139,9 +140,14 @@ __device__ inline void tile_origin(const PermArgs &p, int64_t blk, int TA, int T
   }
 }
 
+// Elements moved by one block.  Small blocks win on MI355X: with 128-byte
+// runs (extent-16 R8 dims) 1024-element blocks beat 4096-element ones by up
+// to 25 % on the slowest permutations of the 16^6 sweep (more blocks resident
+// per CU, shorter tail), and a flat copy peaks with one 16-byte vector per
+// thread.
 template <typename T>
 constexpr int elems_per_block() {
-  return sizeof(T) > 8 ? 2048 : 4096;
+  return 1024;
 }
 
 // V consecutive elements moved as one 8- or 16-byte access.
@@ -196,27 +202,30 @@ __device__ inline void emit(T *o, Pack<T, V> x, typename Real<T>::type ar, typen
 
 // Row mode: dim a is contiguous on both sides; no staging needed.  Accesses
 // are V elements wide along a (the host guarantees V | extent of a and
-// 16-byte aligned bases when V > 1).
-template <typename T, int TA, int TB, int V, int OP>
+// 16-byte aligned bases when V > 1).  The tile size along b is a power of two
+// chosen at launch (p.tb_log) to fit the extent of b, so a small b does not
+// leave most of the block idle; c takes the rest of the block.
+template <typename T, int TA, int V, int OP>
 __global__ __launch_bounds__(kThreads) void perm_rows_kernel(const T *__restrict__ src, T *__restrict__ dst,
                                                              PermArgs p, typename Real<T>::type ar,
                                                              typename Real<T>::type ai, int conj) {
   constexpr int EPB = elems_per_block<T>(), N = EPB / kThreads / V, AV = TA / V;
+  const int tbl = p.tb_log, tb = 1 << tbl;
   int64_t a0, b0, c0, ib, ob;
-  tile_origin(p, blockIdx.x, TA, TB, a0, b0, c0, ib, ob);
-  const int na = (int)min<int64_t>(TA, p.ea - a0), nb = (int)min<int64_t>(TB, p.eb - b0);
+  tile_origin(p, blockIdx.x, TA, tb, a0, b0, c0, ib, ob);
+  const int na = (int)min<int64_t>(TA, p.ea - a0), nb = (int)min<int64_t>(tb, p.eb - b0);
   const int nc = (int)min<int64_t>(p.tc, p.ec - c0);
   Pack<T, V> v[N];
 #pragma unroll
   for (int q = 0; q < N; ++q) {
     const int u = threadIdx.x + q * kThreads;
-    const int ia = (u % AV) * V, jb = (u / AV) % TB, kc = u / (AV * TB);
+    const int ia = (u % AV) * V, jb = (u / AV) & (tb - 1), kc = (u / AV) >> tbl;
     if (ia < na && jb < nb && kc < nc) v[q] = ldv<T, V>(src + ib + ia + jb * p.isb + kc * p.isc);
   }
 #pragma unroll
   for (int q = 0; q < N; ++q) {
     const int u = threadIdx.x + q * kThreads;
-    const int ia = (u % AV) * V, jb = (u / AV) % TB, kc = u / (AV * TB);
+    const int ia = (u % AV) * V, jb = (u / AV) & (tb - 1), kc = (u / AV) >> tbl;
     if (ia < na && jb < nb && kc < nc) emit<T, V, OP>(dst + ob + ia + jb * p.osb + kc * p.osc, v[q], ar, ai, conj);
   }
 }
@@ -270,7 +279,7 @@ template <typename T, int V, int OP>
 __global__ __launch_bounds__(kThreads) void perm_flat_kernel(const T *__restrict__ src, T *__restrict__ dst,
                                                              int64_t n, typename Real<T>::type ar,
                                                              typename Real<T>::type ai, int conj) {
-  constexpr int U = 4;
+  constexpr int U = 1;
   const int64_t nvec = n / V;
   const int64_t stride = (int64_t)gridDim.x * kThreads;
   int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
@@ -372,18 +381,18 @@ __global__ __launch_bounds__(kThreads) void norm2_stage2(const double *__restric
 
 int grid_for(int64_t n) {
   int64_t b = (n + kThreads - 1) / kThreads;
-  return (int)std::max<int64_t>(1, std::min<int64_t>(b, 8192));
+  return (int)std::max<int64_t>(1, std::min<int64_t>(b, 1 << 22));
 }
 
-template <typename T, int TA, int TB, int V>
+template <typename T, int TA, int V>
 void launch_rows(const T *src, T *dst, const PermArgs &p, int op, typename Real<T>::type ar,
                  typename Real<T>::type ai, int conj, int64_t nblk, hipStream_t s) {
   if (op == PERM_COPY)
-    perm_rows_kernel<T, TA, TB, V, PERM_COPY><<<dim3((unsigned)nblk), dim3(kThreads), 0, s>>>(src, dst, p, ar, ai,
-                                                                                              conj);
+    perm_rows_kernel<T, TA, V, PERM_COPY><<<dim3((unsigned)nblk), dim3(kThreads), 0, s>>>(src, dst, p, ar, ai,
+                                                                                          conj);
   else
-    perm_rows_kernel<T, TA, TB, V, PERM_ADD><<<dim3((unsigned)nblk), dim3(kThreads), 0, s>>>(src, dst, p, ar, ai,
-                                                                                             conj);
+    perm_rows_kernel<T, TA, V, PERM_ADD><<<dim3((unsigned)nblk), dim3(kThreads), 0, s>>>(src, dst, p, ar, ai,
+                                                                                         conj);
 }
 
 template <typename T, int TA, int TB, int V>
@@ -458,7 +467,7 @@ int launch_perm_typed(const T *src, T *dst, int rank, const int *dims, const int
   constexpr int VW = vec_width<T>();
   const bool aligned = ((uintptr_t)src % 16 == 0) && ((uintptr_t)dst % 16 == 0);
   if (r <= 1) {
-    const int g = grid_for(vol / 4);
+    const int g = grid_for(vol / VW);
     if (aligned) {
       if (op == PERM_COPY)
         perm_flat_kernel<T, VW, PERM_COPY><<<g, kThreads, 0, s>>>(src, dst, vol, ar, ai, conj);
@@ -510,10 +519,11 @@ int launch_perm_typed(const T *src, T *dst, int rank, const int *dims, const int
   constexpr int EPB = elems_per_block<T>();
   int ta, tb;
   if (rows) {
-    if (p.ea >= 64) { ta = 64; tb = 16; }
-    else if (p.ea >= 32) { ta = 32; tb = 32; }
-    else if (p.ea >= 16) { ta = 16; tb = 64; }
-    else { ta = 8; tb = 128; }
+    ta = p.ea >= 64 ? 64 : p.ea >= 32 ? 32 : p.ea >= 16 ? 16 : 8;
+    // b: the smallest power of two covering its extent, within the block
+    p.tb_log = 0;
+    while ((1 << p.tb_log) < p.eb && (ta << (p.tb_log + 1)) <= EPB) ++p.tb_log;
+    tb = 1 << p.tb_log;
   } else {
     const int64_t m = std::min(p.ea, p.eb);
     if (m >= 24) { ta = 32; tb = 32; }
@@ -530,13 +540,13 @@ int launch_perm_typed(const T *src, T *dst, int rank, const int *dims, const int
   // stride is a multiple of V) and 16-byte aligned bodies
   const bool vec = aligned && (p.ea % VW == 0) && (rows || p.eb % VW == 0) && VW > 1;
   if (rows) {
-#define TAL_ROWS(A, B)                                                         \
-  (vec ? launch_rows<T, A, B, VW>(src, dst, p, op, ar, ai, conj, nblk, s)      \
-       : launch_rows<T, A, B, 1>(src, dst, p, op, ar, ai, conj, nblk, s))
-    if (ta == 64) TAL_ROWS(64, 16);
-    else if (ta == 32) TAL_ROWS(32, 32);
-    else if (ta == 16) TAL_ROWS(16, 64);
-    else TAL_ROWS(8, 128);
+#define TAL_ROWS(A)                                                            \
+  (vec ? launch_rows<T, A, VW>(src, dst, p, op, ar, ai, conj, nblk, s)         \
+       : launch_rows<T, A, 1>(src, dst, p, op, ar, ai, conj, nblk, s))
+    if (ta == 64) TAL_ROWS(64);
+    else if (ta == 32) TAL_ROWS(32);
+    else if (ta == 16) TAL_ROWS(16);
+    else TAL_ROWS(8);
 #undef TAL_ROWS
   } else {
 #define TAL_TILE(A, B)                                                         \
@@ -562,6 +572,113 @@ T make_val(double vr, double vi) {
   }
 }
 
+// ------------------------------------------------------------ slice / insert
+// A block of a dlf tensor ("slice", extents S) is copied out of or into a
+// larger dlf tensor (extents B) at offsets O.  Both keep the dimension order,
+// so after the host drops unit extents and fuses dimensions the slice spans
+// fully, the problem is a set of rows: a run of L elements contiguous on both
+// sides (L = 1 when the first remaining dimension is strided in the big
+// tensor), enumerated by the remaining row dimensions.  A block covers
+// `ry` rows x (rx * V) run elements; each thread decomposes its row index
+// once and moves V elements per access.
+struct SliceArgs {
+  int nrd;           // row dimensions
+  int rx_log;        // log2 of threads along the run
+  int64_t L;         // run length (elements)
+  int64_t nrows, nchunk;
+  int64_t src0, dst0;
+  int64_t re[kMaxRank], rss[kMaxRank], rds[kMaxRank];  // row extents, source / destination strides
+};
+
+template <typename T, int V, int OP>
+__global__ __launch_bounds__(kThreads) void slice_kernel(const T *__restrict__ src, T *__restrict__ dst, SliceArgs p) {
+  const int rx = 1 << p.rx_log, ry = kThreads >> p.rx_log;
+  const int tx = threadIdx.x & (rx - 1), ty = threadIdx.x >> p.rx_log;
+  int64_t blk = blockIdx.x;
+  const int64_t chunk = blk % p.nchunk;
+  blk /= p.nchunk;
+  int64_t r = blk * ry + ty;
+  const int64_t e = (chunk * rx + tx) * V;
+  if (r >= p.nrows || e >= p.L) return;
+  int64_t so = p.src0 + e, dof = p.dst0 + e;
+  for (int k = 0; k < p.nrd; ++k) {
+    const int64_t q = r / p.re[k];
+    const int64_t i = r - q * p.re[k];
+    r = q;
+    so += i * p.rss[k];
+    dof += i * p.rds[k];
+  }
+  emit<T, V, OP>(dst + dof, ldv<T, V>(src + so), (typename Real<T>::type)1, (typename Real<T>::type)0, 0);
+}
+
+template <typename T>
+int launch_slice_typed(int mode, const T *src, T *dst, int rank, const int *sdims, const int *bdims,
+                       const int *offs, int accumulative, hipStream_t s) {
+  // strides of both tensors; dims with slice extent 1 only move the base
+  int64_t sst[kMaxRank], bst[kMaxRank], vol = 1, bvol = 1, base = 0;
+  for (int k = 0; k < rank; ++k) {
+    if (sdims[k] < 1 || bdims[k] < 1 || offs[k] < 0 || (int64_t)offs[k] + sdims[k] > bdims[k]) return -1;
+    sst[k] = vol;
+    bst[k] = bvol;
+    vol *= sdims[k];
+    bvol *= bdims[k];
+    base += (int64_t)offs[k] * bst[k];
+  }
+  int64_t ext[kMaxRank], xs[kMaxRank], xb[kMaxRank];  // fused dims: extent, slice stride, big stride
+  int n = 0;
+  for (int k = 0; k < rank; ++k) {
+    if (sdims[k] == 1) continue;
+    if (n > 0 && xs[n - 1] * ext[n - 1] == sst[k] && xb[n - 1] * ext[n - 1] == bst[k]) {
+      ext[n - 1] *= sdims[k];
+      continue;
+    }
+    ext[n] = sdims[k];
+    xs[n] = sst[k];
+    xb[n] = bst[k];
+    ++n;
+  }
+  SliceArgs p;
+  int first = 0;
+  p.L = 1;
+  if (n > 0 && xs[0] == 1 && xb[0] == 1) {
+    p.L = ext[0];
+    first = 1;
+  }
+  p.nrd = n - first;
+  p.nrows = 1;
+  for (int k = first; k < n; ++k) {
+    p.re[k - first] = ext[k];
+    p.rss[k - first] = mode == 0 ? xb[k] : xs[k];
+    p.rds[k - first] = mode == 0 ? xs[k] : xb[k];
+    p.nrows *= ext[k];
+  }
+  p.src0 = mode == 0 ? base : 0;
+  p.dst0 = mode == 0 ? 0 : base;
+  // vector width: every run start and both bases must be whole vectors
+  constexpr int VW = vec_width<T>();
+  bool vec = VW > 1 && p.L % VW == 0 && p.src0 % VW == 0 && p.dst0 % VW == 0 && (uintptr_t)src % 16 == 0 &&
+             (uintptr_t)dst % 16 == 0;
+  for (int k = 0; k < p.nrd && vec; ++k) vec = p.rss[k] % VW == 0 && p.rds[k] % VW == 0;
+  const int V = vec ? VW : 1;
+  const int64_t nvec = p.L / V;
+  int rx_log = 0;
+  while (rx_log < 8 && ((int64_t)1 << rx_log) < nvec) ++rx_log;
+  p.rx_log = rx_log;
+  const int64_t rx = (int64_t)1 << rx_log, ry = kThreads / rx;
+  p.nchunk = (nvec + rx - 1) / rx;
+  const int64_t nblk = p.nchunk * ((p.nrows + ry - 1) / ry);
+  if (nblk > 0x7fffffffLL) return -4;
+#define TAL_SLICE_LAUNCH(VV)                                                                                   \
+  (accumulative ? slice_kernel<T, VV, PERM_ADD><<<dim3((unsigned)nblk), dim3(kThreads), 0, s>>>(src, dst, p)   \
+                : slice_kernel<T, VV, PERM_COPY><<<dim3((unsigned)nblk), dim3(kThreads), 0, s>>>(src, dst, p))
+  if (V > 1)
+    TAL_SLICE_LAUNCH(VW);
+  else
+    TAL_SLICE_LAUNCH(1);
+#undef TAL_SLICE_LAUNCH
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
 }  // namespace
 
 int launch_permute(int dk, const void *src, void *dst, int rank, const int *dims, const int *o2n, int op,
@@ -580,6 +697,20 @@ int launch_permute(int dk, const void *src, void *dst, int rank, const int *dims
       return launch_perm_typed<float2>((const float2 *)src, (float2 *)dst, rank, dims, o2n, op, ar, ai, conj, s);
     case 18:
       return launch_perm_typed<double2>((const double2 *)src, (double2 *)dst, rank, dims, o2n, op, ar, ai, conj, s);
+    default: return -1;
+  }
+}
+
+int launch_slice(int dk, int mode, const void *src, void *dst, int rank, const int *sdims, const int *bdims,
+                 const int *offsets, int accumulative, hipStream_t s) {
+  if (rank < 0 || rank > kMaxRank || (rank > 0 && (!sdims || !bdims || !offsets)) || !src || !dst ||
+      (mode != 0 && mode != 1))
+    return -1;
+  switch (dk) {
+    case 4: return launch_slice_typed(mode, (const float *)src, (float *)dst, rank, sdims, bdims, offsets, accumulative, s);
+    case 8: return launch_slice_typed(mode, (const double *)src, (double *)dst, rank, sdims, bdims, offsets, accumulative, s);
+    case 14: return launch_slice_typed(mode, (const float2 *)src, (float2 *)dst, rank, sdims, bdims, offsets, accumulative, s);
+    case 18: return launch_slice_typed(mode, (const double2 *)src, (double2 *)dst, rank, sdims, bdims, offsets, accumulative, s);
     default: return -1;
   }
 }
