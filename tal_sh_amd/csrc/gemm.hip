@@ -443,8 +443,10 @@ __global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(2)
 //
 // Block tile 256 x 128, 8 waves of 64 x 64 (two per SIMD), BK = 32.  LDS
 // holds two stages of both operands (2 x (48 + 24) KB); a plane row is 64 bytes
-// (four 16-byte chunks) and chunk c of row r is stored at slot c ^ ((r >> 2) & 3),
-// so the 16-byte fragment reads of 16 consecutive rows cover all 64 banks.
+// (four 16-byte chunks) and chunk c of row r is stored at slot
+// c ^ (((r >> 1) ^ (r >> 3)) & 3), so the 16-byte fragment reads are free of
+// bank conflicts whether the LDS serves them 8 lanes over 32 banks or 16 lanes
+// over 64 banks per cycle.
 // The next K step is loaded into registers while the current one is multiplied;
 // one barrier per K step.
 __device__ inline void split3_finite(float x, __bf16 &h, __bf16 &m, __bf16 &l) {
@@ -498,7 +500,8 @@ __global__ __launch_bounds__(256) void split_planes_kernel(const float *__restri
 
 constexpr int PBM = 256, PBN = 128, PBK = 32;
 
-__device__ inline int pslot(int r, int c) { return r * 4 + (c ^ ((r >> 2) & 3)); }
+__device__ inline int pswz(int r) { return ((r >> 1) ^ (r >> 3)) & 3; }
+__device__ inline int pslot(int r, int c) { return r * 4 + (c ^ pswz(r)); }
 
 __global__ __launch_bounds__(512) void gemm_bf16x6p_kernel(int64_t M, int64_t N, int64_t Kp,
                                                            const uint4 *__restrict__ Ap, int64_t aplane,
@@ -515,22 +518,25 @@ __global__ __launch_bounds__(512) void gemm_bf16x6p_kernel(int64_t M, int64_t N,
   const int64_t kpc = Kp / 8;  // chunks per plane row
   // thread tid loads chunk (tid & 3) of row (tid >> 2) [+128] of each plane
   const int lr = tid >> 2, lc = tid & 3;
-  const uint4 *ag = Ap + (bx * PBM + lr) * kpc + lc;
-  const uint4 *bg = Bp + (by * PBN + lr) * kpc + lc;
+  // block-uniform bases (scalar registers) plus one 32-bit per-thread byte offset
+  const char *ag = reinterpret_cast<const char *>(Ap + bx * PBM * kpc);
+  const char *bg = reinterpret_cast<const char *>(Bp + by * PBN * kpc);
+  const uint32_t voff = (uint32_t)((lr * kpc + lc) * 16);
   const int ls = pslot(lr, lc);
   const int64_t a1 = 128 * kpc;
   // prefetch registers: A planes t = 0,1,2 rows lr and lr+128; B planes t = 0,1,2 row lr
   uint4 a0, a1r, a2, a3, a4, a5, b0, b1, b2;
+#define TAL_PG(base, chunk) (*reinterpret_cast<const uint4 *>((base) + (chunk) * 16 + voff))
 #define TAL_PLOAD(kc)                                                                  \
-  a0 = ag[(kc)];                                                                       \
-  a1r = ag[a1 + (kc)];                                                                 \
-  a2 = ag[aplane + (kc)];                                                              \
-  a3 = ag[aplane + a1 + (kc)];                                                         \
-  a4 = ag[2 * aplane + (kc)];                                                          \
-  a5 = ag[2 * aplane + a1 + (kc)];                                                     \
-  b0 = bg[(kc)];                                                                       \
-  b1 = bg[bplane + (kc)];                                                              \
-  b2 = bg[2 * bplane + (kc)];
+  a0 = TAL_PG(ag, (kc));                                                               \
+  a1r = TAL_PG(ag, a1 + (kc));                                                         \
+  a2 = TAL_PG(ag, aplane + (kc));                                                      \
+  a3 = TAL_PG(ag, aplane + a1 + (kc));                                                 \
+  a4 = TAL_PG(ag, 2 * aplane + (kc));                                                  \
+  a5 = TAL_PG(ag, 2 * aplane + a1 + (kc));                                             \
+  b0 = TAL_PG(bg, (kc));                                                               \
+  b1 = TAL_PG(bg, bplane + (kc));                                                      \
+  b2 = TAL_PG(bg, 2 * bplane + (kc));
 #define TAL_PSTORE(st)                                                                 \
   As[st][0][ls] = a0;                                                                  \
   As[st][0][ls + 512] = a1r;                                                           \
@@ -548,8 +554,9 @@ __global__ __launch_bounds__(512) void gemm_bf16x6p_kernel(int64_t M, int64_t N,
 #pragma unroll
     for (int j = 0; j < 4; ++j) big[i][j] = small[i][j] = f32x4{0, 0, 0, 0};
 
-  // fragment slot of row 16*x + (lane & 15), chunk lane >> 4
-  const int fs = pslot(lane & 15, lane >> 4);
+  // fragment slot of row 16*x + (lane & 15), chunk lane >> 4: the swizzle of
+  // that row is pswz(lane & 15), with bit 1 flipped when x is odd
+  const int fs[2] = {pslot(lane & 15, lane >> 4), (lane & 15) * 4 + ((lane >> 4) ^ pswz(lane & 15) ^ 2)};
   const int64_t nk = Kp / PBK;
   TAL_PLOAD(0)
   TAL_PSTORE(0)
@@ -564,12 +571,12 @@ __global__ __launch_bounds__(512) void gemm_bf16x6p_kernel(int64_t M, int64_t N,
     for (int t = 0; t < 3; ++t)
 #pragma unroll
       for (int j = 0; j < 4; ++j)
-        fb[t][j] = __builtin_bit_cast(bf16x8, Bs[cur][t][(wn * 64 + j * 16) * 4 + fs]);
+        fb[t][j] = __builtin_bit_cast(bf16x8, Bs[cur][t][(wn * 64 + j * 16) * 4 + fs[j & 1]]);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       bf16x8 fa[3];
 #pragma unroll
-      for (int t = 0; t < 3; ++t) fa[t] = __builtin_bit_cast(bf16x8, As[cur][t][(wm * 64 + i * 16) * 4 + fs]);
+      for (int t = 0; t < 3; ++t) fa[t] = __builtin_bit_cast(bf16x8, As[cur][t][(wm * 64 + i * 16) * 4 + fs[i & 1]]);
 #pragma unroll
       for (int j = 0; j < 4; ++j) small[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[2], fb[0][j], small[i][j], 0, 0, 0);
 #pragma unroll
@@ -588,6 +595,7 @@ __global__ __launch_bounds__(512) void gemm_bf16x6p_kernel(int64_t M, int64_t N,
     }
     __syncthreads();
   }
+#undef TAL_PG
 #undef TAL_PLOAD
 #undef TAL_PSTORE
 #pragma unroll
