@@ -129,6 +129,22 @@ size_t talshDeviceBufferFreeSize_(int dev_num, int dev_kind);
 double talshDeviceGetFlops(int dev_kind TALSH_DEFAULT(DEV_DEFAULT), int dev_id TALSH_DEFAULT(DEV_DEFAULT));
 int talshStats(int dev_id TALSH_DEFAULT(-1), int dev_kind TALSH_DEFAULT(DEV_NULL));
 int talshStats_(int dev_id, int dev_kind);
+/* Fast math (talsh.h:167-175): selects the split-BF16 MFMA GEMM for R4/C4 contractions. */
+int talshEnableFastMath(int dev_kind TALSH_DEFAULT(DEV_DEFAULT), int dev_id TALSH_DEFAULT(DEV_DEFAULT));
+int talshDisableFastMath(int dev_kind TALSH_DEFAULT(DEV_DEFAULT), int dev_id TALSH_DEFAULT(DEV_DEFAULT));
+int talshQueryFastMath(int dev_kind TALSH_DEFAULT(DEV_DEFAULT), int dev_id TALSH_DEFAULT(DEV_DEFAULT));
+/* Execution device for given operands (talsh.h:192-194): the device already holding the most operand data;
+ * operations run on GPUs only, so the Host is never returned while a GPU is initialised. */
+int talshDetermineOptimalDevice(const talsh_tens_t *tens0, const talsh_tens_t *tens1 TALSH_DEFAULT(NULL),
+                                const talsh_tens_t *tens2 TALSH_DEFAULT(NULL));
+/* Base pointer of a device's buffer (talsh.h:212-214): the GPU pool region or the pinned Host buffer. */
+void *talshDeviceBufferBasePtr(int dev_num, int dev_kind TALSH_DEFAULT(DEV_NULL));
+void *talshDeviceBufferBasePtr_(int dev_num, int dev_kind);
+/* Logging switches (talsh.h:219-225): buffer allocations / tensor operations printed to stdout. */
+void talshMemManagerLogStart(void);
+void talshMemManagerLogFinish(void);
+void talshTensorOpLogStart(void);
+void talshTensorOpLogFinish(void);
 
 /* ---- tensor blocks (talsh.h:232-311) ---- */
 int talshTensorCreate(talsh_tens_t **tens_block);
@@ -164,6 +180,8 @@ int talshTensorGetBodyAccessConst(const talsh_tens_t *tens_block, const void **b
 int talshTensorGetScalar(talsh_tens_t *tens_block, double *scalar_real, double *scalar_imag);
 void talshTensorPrint(const talsh_tens_t *tens_block);
 void talshTensorPrintInfo(const talsh_tens_t *tens_block);
+/* Elements of the Host image with absolute value >= thresh, with their multi-indices (talsh.h:311). */
+void talshTensorPrintBody(const talsh_tens_t *tens_block, double thresh);
 
 /* ---- tasks (talsh.h:332-372) ---- */
 int talshTaskCreate(talsh_task_t **talsh_task);
@@ -208,6 +226,20 @@ int talshTensorScale(talsh_tens_t *dtens, double val_real, double val_imag, int 
                      talsh_task_t *talsh_task TALSH_DEFAULT(NULL));
 int talshTensorScale_(talsh_tens_t *dtens, double val_real, double val_imag, int dev_id, int dev_kind,
                       int copy_ctrl, talsh_task_t *talsh_task);
+/* Slice: dtens (+)= block of ltens at 0-based offsets; insert: block of dtens at offsets (+)= ltens
+ * (talsh.h:472-493; both tensors keep their dimension order, ranks equal). */
+int talshTensorSlice(talsh_tens_t *dtens, talsh_tens_t *ltens, const int *offsets,
+                     int dev_id TALSH_DEFAULT(DEV_DEFAULT), int dev_kind TALSH_DEFAULT(DEV_DEFAULT),
+                     int copy_ctrl TALSH_DEFAULT(COPY_MT), int accumulative TALSH_DEFAULT(NOPE),
+                     talsh_task_t *talsh_task TALSH_DEFAULT(NULL));
+int talshTensorSlice_(talsh_tens_t *dtens, talsh_tens_t *ltens, const int *offsets, int dev_id, int dev_kind,
+                      int copy_ctrl, int accumulative, talsh_task_t *talsh_task);
+int talshTensorInsert(talsh_tens_t *dtens, talsh_tens_t *ltens, const int *offsets,
+                      int dev_id TALSH_DEFAULT(DEV_DEFAULT), int dev_kind TALSH_DEFAULT(DEV_DEFAULT),
+                      int copy_ctrl TALSH_DEFAULT(COPY_MT), int accumulative TALSH_DEFAULT(NOPE),
+                      talsh_task_t *talsh_task TALSH_DEFAULT(NULL));
+int talshTensorInsert_(talsh_tens_t *dtens, talsh_tens_t *ltens, const int *offsets, int dev_id, int dev_kind,
+                       int copy_ctrl, int accumulative, talsh_task_t *talsh_task);
 int talshTensorCopy(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens,
                     int dev_id TALSH_DEFAULT(DEV_DEFAULT), int dev_kind TALSH_DEFAULT(DEV_DEFAULT),
                     int copy_ctrl TALSH_DEFAULT(COPY_MT), talsh_task_t *talsh_task TALSH_DEFAULT(NULL));
@@ -231,6 +263,22 @@ int talshTensorContract_(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *l
 int talshTensorNorm2(talsh_tens_t *dtens, talsh_tens_t *ltens, int dev_id TALSH_DEFAULT(DEV_DEFAULT),
                      int dev_kind TALSH_DEFAULT(DEV_DEFAULT), int copy_ctrl TALSH_DEFAULT(COPY_TT),
                      talsh_task_t *talsh_task TALSH_DEFAULT(NULL));
+
+/* ---- decompositions (talsh.h:540-592): outside the contraction path; these validate their arguments and
+ * return TALSH_NOT_IMPLEMENTED (the reference computes them on the Host with LAPACK). ---- */
+int talshTensorDecomposeSVD(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, talsh_tens_t *rtens,
+                            talsh_tens_t *stens, const char absorb TALSH_DEFAULT('N'),
+                            int dev_id TALSH_DEFAULT(DEV_DEFAULT), int dev_kind TALSH_DEFAULT(DEV_DEFAULT));
+int talshTensorDecomposeSVDL(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, talsh_tens_t *rtens,
+                             int dev_id TALSH_DEFAULT(DEV_DEFAULT), int dev_kind TALSH_DEFAULT(DEV_DEFAULT));
+int talshTensorDecomposeSVDR(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, talsh_tens_t *rtens,
+                             int dev_id TALSH_DEFAULT(DEV_DEFAULT), int dev_kind TALSH_DEFAULT(DEV_DEFAULT));
+int talshTensorDecomposeSVDLR(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, talsh_tens_t *rtens,
+                              int dev_id TALSH_DEFAULT(DEV_DEFAULT), int dev_kind TALSH_DEFAULT(DEV_DEFAULT));
+int talshTensorOrthogonalizeSVD(const char *cptrn, talsh_tens_t *dtens, int dev_id TALSH_DEFAULT(DEV_DEFAULT),
+                                int dev_kind TALSH_DEFAULT(DEV_DEFAULT));
+int talshTensorOrthogonalizeMGS(talsh_tens_t *dtens, int num_iso_dims, int *iso_dims,
+                                int dev_id TALSH_DEFAULT(DEV_DEFAULT), int dev_kind TALSH_DEFAULT(DEV_DEFAULT));
 
 /* ---- debugging (talsh.h:593) ---- */
 double talshTensorImageNorm1_cpu(const talsh_tens_t *talsh_tens);

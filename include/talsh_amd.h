@@ -16,13 +16,10 @@
 extern "C" {
 #endif
 
-/* Fast math (talsh.h:167-174 of the reference): on this build "fast math"
- * selects the 3-term split-BF16 MFMA GEMM for R4/C4 contractions (FP32
- * accumulation); disabled, R4/C4 use the FP32 MFMA GEMM.  R8/C8 always use
- * the FP64 MFMA GEMM. */
-int talshEnableFastMath(int dev_kind, int dev_id);
-int talshDisableFastMath(int dev_kind, int dev_id);
-int talshQueryFastMath(int dev_kind, int dev_id);
+/* Fast math (talshEnableFastMath etc., declared in talsh.h): on this build
+ * "fast math" selects the 3-term split-BF16 MFMA GEMM for R4 contractions
+ * (FP32 accumulation); disabled, R4/C4 use the FP32 MFMA GEMM.  R8/C8
+ * always use the FP64 MFMA GEMM. */
 
 /* Size in bytes of the per-GPU buffer pool taken at talshInit (default: 80 %
  * of the free device memory).  Must be called before talshInit. */

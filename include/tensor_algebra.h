@@ -182,4 +182,17 @@ typedef struct {
 typedef int (*talsh_tens_init_i)(const talsh_tens_data_t *tens_data, const talsh_tens_shape_t *tens_shape,
                                  const talsh_tens_signature_t *tens_signature);
 
+#ifdef __cplusplus
+extern "C" {
+#endif
+/* Digital -> symbolic contraction pattern (tensor_algebra.h:265-266, the Fortran routine
+ * tensor_algebra_cpu.F90:5533-5611): "D(a,b,..)+=L(..)*R(..)" with uncontracted indices named a, b, ... in
+ * D order and contracted ones after them; conj_bits 1/2/4 put '+' after D/L/R.  cptrn_sym receives cpl
+ * characters followed by a NUL; ierr = 1 for negative ranks. */
+void get_contr_pattern_sym(const int *rank_left, const int *rank_right, const int *conj_bits, const int *cptrn_dig,
+                           char *cptrn_sym, int *cpl, int *ierr);
+#ifdef __cplusplus
+}
+#endif
+
 #endif /* TENSOR_ALGEBRA_H_ */

@@ -9,6 +9,9 @@
 //   talshTensorContract  talshc.cpp:5073 (Host branch :5212-5289)
 //   talshTensorCopy      talshc.cpp:4530
 //   talshTensorAdd       talshc.cpp:4795
+//   talshTensorSlice     talshc.cpp:4008 (Host: tensor_block_slice, tensor_algebra_cpu.F90:2315)
+//   talshTensorInsert    talshc.cpp:4269 (Host: tensor_block_insert, tensor_algebra_cpu.F90:2466)
+//   get_contr_pattern_sym         tensor_algebra_cpu.F90:5533-5611
 //   talsh_get_contr_ptrn_str2dig  talshf.F90:619-667
 //   get_contr_permutations        tensor_algebra_cpu.F90:5613-5755
 //   tensBlck_norm2_host           tensor_algebra_gpu_nvidia.cu:3269-3299
@@ -122,6 +125,38 @@ int ref_scale(int dk, int rank, const int *dims, void *d, double vr, double vi) 
   err = talshTensorScale(&td, vr, vi, 0, DEV_HOST, COPY_M, NULL);
   talshTensorDestruct(&td);
   return err;
+}
+
+// s (+)= b[offsets + i] (slice) or b[offsets + i] (+)= s (insert).
+static int slice_or_insert(int insert, int dk, int rank, const int *sdims, void *s, const int *bdims, void *b,
+                           const int *offsets, int accumulative) {
+  talsh_tens_t ts, tb;
+  int err = wrap(&ts, dk, rank, sdims, s);
+  if (err) return err;
+  err = wrap(&tb, dk, rank, bdims, b);
+  if (err) { talshTensorDestruct(&ts); return err; }
+  if (insert)
+    err = talshTensorInsert(&tb, &ts, offsets, 0, DEV_HOST, COPY_MT, accumulative ? YEP : NOPE, NULL);
+  else
+    err = talshTensorSlice(&ts, &tb, offsets, 0, DEV_HOST, COPY_MT, accumulative ? YEP : NOPE, NULL);
+  talshTensorDestruct(&tb);
+  talshTensorDestruct(&ts);
+  return err;
+}
+int ref_slice(int dk, int rank, const int *sdims, void *s, const int *bdims, void *b, const int *offsets,
+              int accumulative) {
+  return slice_or_insert(0, dk, rank, sdims, s, bdims, b, offsets, accumulative);
+}
+int ref_insert(int dk, int rank, const int *sdims, void *s, const int *bdims, void *b, const int *offsets,
+               int accumulative) {
+  return slice_or_insert(1, dk, rank, sdims, s, bdims, b, offsets, accumulative);
+}
+// Digital -> symbolic contraction pattern; returns ierr, the string in buf.
+int ref_pattern_sym(int lrank, int rrank, int conj_bits, const int *dig, char *buf, int *cpl) {
+  int ierr = 0;
+  get_contr_pattern_sym(&lrank, &rrank, &conj_bits, dig, buf, cpl, &ierr);
+  if (ierr == 0) buf[*cpl] = 0;
+  return ierr;
 }
 
 // Squared 2-norm via the reference's host reduction (R4/R8 only).

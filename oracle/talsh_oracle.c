@@ -599,3 +599,64 @@ void orc_fill_uniform(uint64_t seed, size_t n, double *out) {
     out[i] = 2.0 * ((double)(z >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
   }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Slice / insert: walk the block's elements in dlf order with a multi-index
+ * odometer and map each to its position in the big tensor,
+ *   pos = sum_k (offsets[k] + i_k) * prod_{m<k} bdims[m]
+ * (tensor_block_slice_dlf_r8 / tensor_block_insert_dlf_r8,
+ * tensor_algebra_cpu.F90:6900-6974 and the insert counterparts). */
+static int block_positions(int rank, const int *sdims, const int *bdims, const int *offsets, size_t *vol) {
+  *vol = 1;
+  for (int k = 0; k < rank; ++k) {
+    if (sdims[k] < 1 || bdims[k] < 1 || offsets[k] < 0 || offsets[k] + sdims[k] > bdims[k]) return 1;
+    *vol *= (size_t)sdims[k];
+  }
+  return 0;
+}
+
+static int slice_insert(int insert, int dk, int rank, const int *sdims, void *s, const int *bdims, void *b,
+                        const int *offsets, int accumulative) {
+  size_t vol;
+  if (rank < 0 || rank > 56 || block_positions(rank, sdims, bdims, offsets, &vol)) return 1;
+  const int esz = (dk == 4) ? 4 : (dk == 8 || dk == 14) ? 8 : (dk == 18) ? 16 : 0;
+  const int ncomp = (dk == 14 || dk == 18) ? 2 : 1;
+  if (!esz) return 2;
+  int idx[56] = {0};
+  size_t bstride[56];
+  size_t st = 1;
+  for (int k = 0; k < rank; ++k) {
+    bstride[k] = st;
+    st *= (size_t)bdims[k];
+  }
+  for (size_t l = 0; l < vol; ++l) {
+    size_t pos = 0;
+    for (int k = 0; k < rank; ++k) pos += (size_t)(offsets[k] + idx[k]) * bstride[k];
+    for (int c = 0; c < ncomp; ++c) {
+      if (dk == 4 || dk == 14) {
+        float *dst = insert ? (float *)b + pos * ncomp + c : (float *)s + l * ncomp + c;
+        const float src = insert ? ((const float *)s)[l * ncomp + c] : ((const float *)b)[pos * ncomp + c];
+        *dst = accumulative ? *dst + src : src;
+      } else {
+        double *dst = insert ? (double *)b + pos * ncomp + c : (double *)s + l * ncomp + c;
+        const double src = insert ? ((const double *)s)[l * ncomp + c] : ((const double *)b)[pos * ncomp + c];
+        *dst = accumulative ? *dst + src : src;
+      }
+    }
+    for (int k = 0; k < rank; ++k) { /* odometer, first index fastest */
+      if (++idx[k] < sdims[k]) break;
+      idx[k] = 0;
+    }
+  }
+  return 0;
+}
+
+int orc_slice(int dk, int rank, const int *sdims, void *s, const int *bdims, const void *b, const int *offsets,
+              int accumulative) {
+  return slice_insert(0, dk, rank, sdims, s, bdims, (void *)b, offsets, accumulative);
+}
+
+int orc_insert(int dk, int rank, const int *sdims, const void *s, const int *bdims, void *b, const int *offsets,
+               int accumulative) {
+  return slice_insert(1, dk, rank, sdims, (void *)s, bdims, b, offsets, accumulative);
+}

@@ -20,6 +20,8 @@
  *                          loop nests, and the reference contraction
  *   orc_norm2              exact sums of small integers + reference norm2
  *   orc_init / orc_scale   reference init/scale, bit-exact
+ *   orc_slice / orc_insert closed-form index map (values that encode their
+ *                          own position) + reference slice/insert, bit-exact
  */
 #ifndef TALSH_ORACLE_H_
 #define TALSH_ORACLE_H_
@@ -60,6 +62,16 @@ double orc_norm2(int dk, size_t vol, const void *p);
 
 void orc_init(int dk, size_t vol, void *d, double vr, double vi);
 void orc_scale(int dk, size_t vol, void *d, double vr, double vi);
+
+/* Slice (tensor_block_slice_dlf_*, tensor_algebra_cpu.F90:6821-7053):
+ *   s[i] = b[offsets + i]  (s[i] = s[i] + b[offsets + i] when accumulative);
+ * insert (tensor_block_insert_dlf_*): b[offsets + i] = s[i] (+=).  s has
+ * extents sdims, b has extents bdims, both dlf, 0-based offsets.  Returns
+ * nonzero for a block that does not fit. */
+int orc_slice(int dk, int rank, const int *sdims, void *s, const int *bdims, const void *b, const int *offsets,
+              int accumulative);
+int orc_insert(int dk, int rank, const int *sdims, const void *s, const int *bdims, void *b, const int *offsets,
+               int accumulative);
 
 /* Seeded uniform[-1,1] stream (splitmix64), shared with the Python tests. */
 void orc_fill_uniform(uint64_t seed, size_t n, double *out);

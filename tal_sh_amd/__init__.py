@@ -124,6 +124,11 @@ SIGNATURES = {
     "talshTensorAdd": (_I, [ctypes.c_char_p, _TP, _TP, _D, _D, _I, _I, _I, _KP]),
     "talshTensorContract": (_I, [ctypes.c_char_p, _TP, _TP, _TP, _D, _D, _I, _I, _I, _I, _KP]),
     "talshTensorNorm2": (_I, [_TP, _TP, _I, _I, _I, _KP]),
+    "talshTensorSlice": (_I, [_TP, _TP, _IP, _I, _I, _I, _I, _KP]),
+    "talshTensorInsert": (_I, [_TP, _TP, _IP, _I, _I, _I, _I, _KP]),
+    "talshDetermineOptimalDevice": (_I, [_TP, _TP, _TP]),
+    "talshDeviceBufferBasePtr": (_P, [_I, _I]),
+    "get_contr_pattern_sym": (None, [_IP, _IP, _IP, _IP, ctypes.c_char_p, _IP, _IP]),
     "talshTensorImageNorm1_cpu": (_D, [_TP]),
     "talshEnableFastMath": (_I, [_I, _I]),
     "talshDisableFastMath": (_I, [_I, _I]),
@@ -400,6 +405,33 @@ def contract(pattern, d, l, r, alpha=1.0, dev_id=DEV_DEFAULT, dev_kind=DEV_DEFAU
     a = complex(alpha)
     return load().talshTensorContract(pattern.encode(), d.ptr, l.ptr, r.ptr, a.real, a.imag, dev_id, dev_kind,
                                       copy_ctrl, accumulative, task.ptr if task else None)
+
+
+def slice(d, l, offsets, dev_id=DEV_DEFAULT, dev_kind=DEV_DEFAULT, copy_ctrl=None, accumulative=NOPE, task=None):
+    """talshTensorSlice: d (+)= block of l at 0-based offsets."""
+    copy_ctrl = coh("M", "T") if copy_ctrl is None else copy_ctrl
+    off = (ctypes.c_int * max(1, len(offsets)))(*offsets)
+    return load().talshTensorSlice(d.ptr, l.ptr, off, dev_id, dev_kind, copy_ctrl, accumulative,
+                                   task.ptr if task else None)
+
+
+def insert(d, l, offsets, dev_id=DEV_DEFAULT, dev_kind=DEV_DEFAULT, copy_ctrl=None, accumulative=NOPE, task=None):
+    """talshTensorInsert: block of d at 0-based offsets (+)= l."""
+    copy_ctrl = coh("M", "T") if copy_ctrl is None else copy_ctrl
+    off = (ctypes.c_int * max(1, len(offsets)))(*offsets)
+    return load().talshTensorInsert(d.ptr, l.ptr, off, dev_id, dev_kind, copy_ctrl, accumulative,
+                                    task.ptr if task else None)
+
+
+def pattern_sym(lrank, rrank, conj, dig):
+    """get_contr_pattern_sym: digital -> symbolic contraction pattern; returns (ierr, string)."""
+    buf = ctypes.create_string_buffer(4096)
+    cpl, ierr = ctypes.c_int(), ctypes.c_int()
+    lr, rr, cj = ctypes.c_int(lrank), ctypes.c_int(rrank), ctypes.c_int(conj)
+    d = (ctypes.c_int * max(1, len(dig)))(*dig)
+    load().get_contr_pattern_sym(ctypes.byref(lr), ctypes.byref(rr), ctypes.byref(cj), d, buf, ctypes.byref(cpl),
+                                 ctypes.byref(ierr))
+    return ierr.value, buf.raw[:cpl.value].decode() if ierr.value == 0 else ""
 
 
 def copy(pattern, d, l, dev_id=DEV_DEFAULT, dev_kind=DEV_DEFAULT, copy_ctrl=None, task=None):

@@ -18,6 +18,12 @@ enum PermOp { PERM_COPY = 0, PERM_ADD = 1 };
 int launch_permute(int dk, const void *src, void *dst, int rank, const int *dims, const int *o2n, int op,
                    double ar, double ai, int conj, hipStream_t s);
 
+// Slice (mode 0): small = big[offsets + i] (small += ... when accumulative);
+// insert (mode 1): big[offsets + i] = small (+=).  small has extents sdims,
+// big has extents bdims, both dlf with the same dimension order.  Bit-exact.
+int launch_slice(int dk, int mode, const void *src, void *dst, int rank, const int *sdims, const int *bdims,
+                 const int *offsets, int accumulative, hipStream_t s);
+
 // d = val (all elements)
 int launch_init(int dk, void *d, size_t vol, double vr, double vi, hipStream_t s);
 // d = d * alpha

@@ -12,7 +12,9 @@
 // gpu_tensor_block_{init,scale,copy,add,contract_dlf,place}
 // (tensor_algebra_gpu_nvidia.cu:4089-7188); coherence letters as in
 // DOC/TALSH_manual.txt:719-750 (D discard, M move, T temporary, K keep).
+#include <cstdio>
 #include <cstring>
+#include <string>
 
 #include "plan.hpp"
 #include "runtime.hpp"
@@ -41,7 +43,7 @@ struct Operand {
 
 class Op {
  public:
-  Op(int g, talsh_task_t *user_task) : g_(g), user_(user_task) {}
+  Op(int g, talsh_task_t *user_task, const char *name) : g_(g), user_(user_task), name_(name) {}
   ~Op() {
     if (!committed_) rollback();
   }
@@ -214,6 +216,9 @@ class Op {
     }
     tt->flops = task_->flops;
     tt->data_vol = task_->bytes;
+    if (rt().log_ops)
+      std::printf("#TALSH OP: %s on GPU %d: %d operands, %.6g flop, %.6g bytes moved\n", name_, g_, nargs,
+                  task_->flops, task_->bytes);
     task_ = nullptr;
     if (user_) return TALSH_SUCCESS;
     const int st = task_finalize_if_done(tt, true);
@@ -270,6 +275,7 @@ class Op {
 
   int g_;
   talsh_task_t *user_;
+  const char *name_;
   std::vector<Operand> ops_;
   std::vector<void *> temps_;
   std::vector<int> temp_handles_;
@@ -349,7 +355,7 @@ int talshTensorInit(talsh_tens_t *dtens, double val_real, double val_imag, int d
   if (int e = check_task(talsh_task)) return e;
   int g;
   if (int e = exec_gpu(dev_id, dev_kind, {dtens}, &g)) return e;
-  Op op(g, talsh_task);
+  Op op(g, talsh_task, __func__);
   if (int e = op.add(dtens, coh_letter(copy_ctrl, 1, 0), true, false)) return e;
   if (int e = op.acquire({})) return deferred(g, e);
   Operand &d = op.operand(0);
@@ -370,7 +376,7 @@ int talshTensorScale(talsh_tens_t *dtens, double val_real, double val_imag, int 
   if (int e = check_task(talsh_task)) return e;
   int g;
   if (int e = exec_gpu(dev_id, dev_kind, {dtens}, &g)) return e;
-  Op op(g, talsh_task);
+  Op op(g, talsh_task, __func__);
   if (int e = op.add(dtens, coh_letter(copy_ctrl, 1, 0), true, true)) return e;
   if (int e = op.acquire({})) return deferred(g, e);
   Operand &d = op.operand(0);
@@ -413,7 +419,7 @@ int talshTensorCopy(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens,
   if (int e = unary_perm(cptrn, dtens, ltens, &o2n, &conj)) return e;
   int g;
   if (int e = exec_gpu(dev_id, dev_kind, {dtens, ltens}, &g)) return e;
-  Op op(g, talsh_task);
+  Op op(g, talsh_task, __func__);
   if (int e = op.add(dtens, coh_letter(copy_ctrl, 2, 0), true, false)) return e;
   if (int e = op.add(ltens, coh_letter(copy_ctrl, 2, 1), false, true)) return e;
   if (op.operand(0).dk != op.operand(1).dk) return TALSH_INVALID_ARGS;
@@ -443,7 +449,7 @@ int talshTensorAdd(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, 
   if (int e = unary_perm(cptrn, dtens, ltens, &o2n, &conj)) return e;
   int g;
   if (int e = exec_gpu(dev_id, dev_kind, {dtens, ltens}, &g)) return e;
-  Op op(g, talsh_task);
+  Op op(g, talsh_task, __func__);
   if (int e = op.add(dtens, coh_letter(copy_ctrl, 2, 0), true, true)) return e;
   if (int e = op.add(ltens, coh_letter(copy_ctrl, 2, 1), false, true)) return e;
   if (op.operand(0).dk != op.operand(1).dk) return TALSH_INVALID_ARGS;
@@ -461,6 +467,103 @@ int talshTensorAdd(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, 
 int talshTensorAdd_(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, double scale_real,
                     double scale_imag, int dev_id, int dev_kind, int copy_ctrl, talsh_task_t *talsh_task) {
   return talshTensorAdd(cptrn, dtens, ltens, scale_real, scale_imag, dev_id, dev_kind, copy_ctrl, talsh_task);
+}
+
+// Slice (mode 0: dtens (+)= ltens[offsets + i]) and insert (mode 1:
+// dtens[offsets + i] (+)= ltens): talshTensorSlice / talshTensorInsert
+// (talshc.cpp:4008-4524) with the reference's CPU semantics
+// (tensor_algebra_cpu.F90:2315-2626, 6821-7053); its GPU back ends
+// (tensor_algebra_gpu_nvidia.cu:4788-4804) are unimplemented.  An insert
+// writes only part of dtens, so dtens is always read (uploaded if needed).
+static int slice_insert(int mode, talsh_tens_t *dtens, talsh_tens_t *ltens, const int *offsets, int dev_id,
+                        int dev_kind, int copy_ctrl, int accumulative, talsh_task_t *talsh_task, const char *name) {
+  std::lock_guard<std::recursive_mutex> lk(rt().mu);
+  if (!rt().on) return TALSH_NOT_INITIALIZED;
+  if (!dtens || !ltens) return TALSH_INVALID_ARGS;
+  if (int e = check_task(talsh_task)) return e;
+  if (talshTensorIsEmpty(dtens) == YEP || talshTensorIsEmpty(ltens) == YEP) return TALSH_OBJECT_IS_EMPTY;
+  const int rank = dtens->shape_p->num_dim;
+  if (ltens->shape_p->num_dim != rank || (rank > 0 && !offsets)) return TALSH_INVALID_ARGS;
+  const talsh_tens_t *small = mode == 0 ? dtens : ltens, *big = mode == 0 ? ltens : dtens;
+  for (int k = 0; k < rank; ++k)
+    if (offsets[k] < 0 || offsets[k] + small->shape_p->dims[k] > big->shape_p->dims[k]) return TALSH_INVALID_ARGS;
+  const bool acc = accumulative != NOPE;
+  int g;
+  if (int e = exec_gpu(dev_id, dev_kind, {dtens, ltens}, &g)) return e;
+  Op op(g, talsh_task, name);
+  if (int e = op.add(dtens, coh_letter(copy_ctrl, 2, 0), true, acc || mode == 1)) return e;
+  if (int e = op.add(ltens, coh_letter(copy_ctrl, 2, 1), false, true)) return e;
+  if (op.operand(0).dk != op.operand(1).dk) return TALSH_INVALID_ARGS;
+  if (int e = op.acquire({})) return deferred(g, e);
+  const int dk = op.operand(0).dk;
+  if (int e = op.begin(dk, copy_ctrl)) return e;
+  std::vector<int> sd = dims_of(small), bd = dims_of(big);
+  if (launch_slice(dk, mode, op.operand(1).dptr, op.operand(0).dptr, rank, sd.data(), bd.data(), offsets,
+                   acc ? 1 : 0, op.stream()))
+    return TALSH_FAILURE;
+  const double vol = (double)shape_volume(small->shape_p);
+  if (int e = op.end(acc ? ((dk == C4 || dk == C8) ? 2.0 : 1.0) * vol : 0.0)) return e;
+  return op.publish(2);
+}
+
+int talshTensorSlice(talsh_tens_t *dtens, talsh_tens_t *ltens, const int *offsets, int dev_id, int dev_kind,
+                     int copy_ctrl, int accumulative, talsh_task_t *talsh_task) {
+  return slice_insert(0, dtens, ltens, offsets, dev_id, dev_kind, copy_ctrl, accumulative, talsh_task, __func__);
+}
+int talshTensorSlice_(talsh_tens_t *dtens, talsh_tens_t *ltens, const int *offsets, int dev_id, int dev_kind,
+                      int copy_ctrl, int accumulative, talsh_task_t *talsh_task) {
+  return talshTensorSlice(dtens, ltens, offsets, dev_id, dev_kind, copy_ctrl, accumulative, talsh_task);
+}
+int talshTensorInsert(talsh_tens_t *dtens, talsh_tens_t *ltens, const int *offsets, int dev_id, int dev_kind,
+                      int copy_ctrl, int accumulative, talsh_task_t *talsh_task) {
+  return slice_insert(1, dtens, ltens, offsets, dev_id, dev_kind, copy_ctrl, accumulative, talsh_task, __func__);
+}
+int talshTensorInsert_(talsh_tens_t *dtens, talsh_tens_t *ltens, const int *offsets, int dev_id, int dev_kind,
+                       int copy_ctrl, int accumulative, talsh_task_t *talsh_task) {
+  return talshTensorInsert(dtens, ltens, offsets, dev_id, dev_kind, copy_ctrl, accumulative, talsh_task);
+}
+
+// Decompositions (talshc.cpp:5650-6290) are Host LAPACK work outside the
+// contraction path: arguments are checked as the reference does, then
+// TALSH_NOT_IMPLEMENTED is returned.
+static int decomposition_args(const char *cptrn, std::initializer_list<const talsh_tens_t *> tens) {
+  if (!rt().on) return TALSH_NOT_INITIALIZED;
+  if (!cptrn) return TALSH_INVALID_ARGS;
+  for (const talsh_tens_t *t : tens) {
+    if (!t) return TALSH_INVALID_ARGS;
+    if (talshTensorIsEmpty(t) == YEP) return TALSH_OBJECT_IS_EMPTY;
+  }
+  return TALSH_NOT_IMPLEMENTED;
+}
+int talshTensorDecomposeSVD(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, talsh_tens_t *rtens,
+                            talsh_tens_t *stens, const char absorb, int dev_id, int dev_kind) {
+  (void)absorb, (void)dev_id, (void)dev_kind;
+  if (!stens) return TALSH_INVALID_ARGS;
+  return decomposition_args(cptrn, {dtens, ltens, rtens});
+}
+int talshTensorDecomposeSVDL(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, talsh_tens_t *rtens,
+                             int dev_id, int dev_kind) {
+  (void)dev_id, (void)dev_kind;
+  return decomposition_args(cptrn, {dtens, ltens, rtens});
+}
+int talshTensorDecomposeSVDR(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, talsh_tens_t *rtens,
+                             int dev_id, int dev_kind) {
+  (void)dev_id, (void)dev_kind;
+  return decomposition_args(cptrn, {dtens, ltens, rtens});
+}
+int talshTensorDecomposeSVDLR(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, talsh_tens_t *rtens,
+                              int dev_id, int dev_kind) {
+  (void)dev_id, (void)dev_kind;
+  return decomposition_args(cptrn, {dtens, ltens, rtens});
+}
+int talshTensorOrthogonalizeSVD(const char *cptrn, talsh_tens_t *dtens, int dev_id, int dev_kind) {
+  (void)dev_id, (void)dev_kind;
+  return decomposition_args(cptrn, {dtens});
+}
+int talshTensorOrthogonalizeMGS(talsh_tens_t *dtens, int num_iso_dims, int *iso_dims, int dev_id, int dev_kind) {
+  (void)dev_id, (void)dev_kind;
+  if (num_iso_dims < 0 || (num_iso_dims > 0 && !iso_dims)) return TALSH_INVALID_ARGS;
+  return decomposition_args("", {dtens});
 }
 
 int talshTensorContract(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, talsh_tens_t *rtens,
@@ -483,7 +586,7 @@ int talshTensorContract(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *lt
   if (pe != 0) return TALSH_INVALID_ARGS;
   int g;
   if (int e = exec_gpu(dev_id, dev_kind, {dtens, ltens, rtens}, &g)) return e;
-  Op op(g, talsh_task);
+  Op op(g, talsh_task, __func__);
   const bool acc = accumulative != NOPE;
   if (int e = op.add(dtens, coh_letter(copy_ctrl, 3, 0), true, acc)) return e;
   if (int e = op.add(ltens, coh_letter(copy_ctrl, 3, 1), false, true)) return e;
@@ -568,7 +671,7 @@ int talshTensorNorm2(talsh_tens_t *dtens, talsh_tens_t *ltens, int dev_id, int d
   if (shape_volume(dtens->shape_p) != 1) return TALSH_INVALID_ARGS;
   int g;
   if (int e = exec_gpu(dev_id, dev_kind, {ltens, dtens}, &g)) return e;
-  Op op(g, talsh_task);
+  Op op(g, talsh_task, __func__);
   if (int e = op.add(dtens, coh_letter(copy_ctrl, 2, 0), true, false)) return e;
   if (int e = op.add(ltens, coh_letter(copy_ctrl, 2, 1), false, true)) return e;
   const int dk = op.operand(1).dk;
@@ -780,6 +883,62 @@ int talsh_get_contr_ptrn_str2dig(const char *c_str, int *dig_ptrn, int *drank, i
   *conj_bits = p.conj;
   for (int i = 0; i < p.lrank + p.rrank; ++i) dig_ptrn[i] = p.dig[i];
   return 0;
+}
+
+// Digital -> symbolic pattern (tensor_algebra.h:265-266): D gets the
+// uncontracted labels a, b, ... in order, each contracted index of L a new
+// label after them, and R reuses L's label for the index it contracts with.
+void get_contr_pattern_sym(const int *rank_left, const int *rank_right, const int *conj_bits, const int *cptrn_dig,
+                           char *cptrn_sym, int *cpl, int *ierr) {
+  if (!rank_left || !rank_right || !conj_bits || !cptrn_sym || !cpl || !ierr) return;
+  *cpl = 0;
+  *ierr = 0;
+  const int lr = *rank_left, rr = *rank_right, cj = *conj_bits;
+  if (lr < 0 || rr < 0 || lr > MAX_TENSOR_RANK || rr > MAX_TENSOR_RANK || (lr + rr > 0 && !cptrn_dig)) {
+    *ierr = 1;
+    return;
+  }
+  std::string out;
+  int nu = 0;
+  for (int i = 0; i < lr + rr; ++i) nu += cptrn_dig[i] > 0;
+  auto tensor = [&](const char *name, int bit, const std::vector<char> &labels) {
+    out += name;
+    if (cj & bit) out += '+';
+    out += '(';
+    for (size_t i = 0; i < labels.size(); ++i) {
+      if (i) out += ',';
+      out += labels[i];
+    }
+    out += ')';
+  };
+  std::vector<char> dl, ll, rl, lcon(lr > 0 ? lr : 1, '?');
+  for (int i = 0; i < nu; ++i) dl.push_back((char)('a' + i));
+  char next = (char)('a' + nu);
+  for (int i = 0; i < lr; ++i) {
+    const int d = cptrn_dig[i];
+    if (d > 0) {
+      ll.push_back((char)('a' + d - 1));
+    } else {
+      lcon[i] = next++;
+      ll.push_back(lcon[i]);
+    }
+  }
+  for (int i = 0; i < rr; ++i) {
+    const int d = cptrn_dig[lr + i];
+    if (d > 0) rl.push_back((char)('a' + d - 1));
+    else if (-d >= 1 && -d <= lr) rl.push_back(lcon[-d - 1]);
+    else {
+      *ierr = 1;
+      return;
+    }
+  }
+  tensor("D", 1, dl);
+  out += "+=";
+  tensor("L", 2, ll);
+  out += '*';
+  tensor("R", 4, rl);
+  std::memcpy(cptrn_sym, out.c_str(), out.size() + 1);
+  *cpl = (int)out.size();
 }
 
 // Contraction plan chosen for a pattern and shapes (see talsh_amd.h).

@@ -20,6 +20,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <set>
 
 namespace tal {
 
@@ -134,11 +136,18 @@ void hazard_drain(const void *p) {
 }
 
 // ------------------------------------------------------------------ memory
-int dev_alloc(int g, size_t bytes, void **p, int *handle) { return rt().gpus[g].arena.alloc(bytes, p, handle); }
+int dev_alloc(int g, size_t bytes, void **p, int *handle) {
+  const int e = rt().gpus[g].arena.alloc(bytes, p, handle);
+  if (rt().log_mem)
+    std::printf("#TALSH MEM: GPU %d allocate %zu bytes -> %s %p (entry %d)\n", g, bytes, e ? "failed" : "ok",
+                e ? nullptr : *p, e ? -1 : *handle);
+  return e;
+}
 
 void dev_free(int g, void *p, int handle) {
   hazard_drain(p);
   rt().gpus[g].arena.release(handle);
+  if (rt().log_mem) std::printf("#TALSH MEM: GPU %d release %p (entry %d)\n", g, p, handle);
 }
 
 int host_alloc(size_t bytes, void **p, int *handle) {
@@ -469,6 +478,51 @@ size_t talshDeviceBufferSize(int dev_num, int dev_kind) {
   return 0;
 }
 size_t talshDeviceBufferSize_(int dev_num, int dev_kind) { return talshDeviceBufferSize(dev_num, dev_kind); }
+
+void *talshDeviceBufferBasePtr(int dev_num, int dev_kind) {
+  const int id = resolve_flat(dev_num, dev_kind);
+  const int g = gpu_index(id);
+  if (g >= 0 && rt().gpus[g].on) return rt().gpus[g].base;
+  if (id == DEV_HOST) return rt().host_base;
+  return nullptr;
+}
+void *talshDeviceBufferBasePtr_(int dev_num, int dev_kind) { return talshDeviceBufferBasePtr(dev_num, dev_kind); }
+
+// Execution device for up to three operands (talshc.cpp:366-488): the device
+// that already holds available images of the most operand elements, so the
+// fewest elements have to move.  Every operation of this library runs on a
+// GPU, so when the Host wins (or nothing is resident anywhere) the least busy
+// GPU is returned instead; the Host only when no GPU is initialised.
+int talshDetermineOptimalDevice(const talsh_tens_t *tens0, const talsh_tens_t *tens1, const talsh_tens_t *tens2) {
+  std::lock_guard<std::recursive_mutex> lk(rt().mu);
+  if (!tens0) return DEV_NULL;
+  std::map<int, double> resident;  // flat device id -> elements with an available image there
+  for (const talsh_tens_t *t : {tens0, tens1, tens2}) {
+    if (!t || tensor_healthy(t) != YEP || !t->shape_p) continue;
+    const double v = (double)shape_volume(t->shape_p);
+    std::set<int> seen;
+    for (int i = 0; i < t->ndev; ++i)
+      if (t->avail[i] == YEP && seen.insert(t->dev_rsc[i].dev_id).second) resident[t->dev_rsc[i].dev_id] += v;
+  }
+  int best = DEV_NULL;
+  double best_v = -1.0;
+  for (const auto &kv : resident)
+    if (gpu_index(kv.first) >= 0 && rt().gpus[gpu_index(kv.first)].on && kv.second > best_v) {
+      best = kv.first;
+      best_v = kv.second;
+    }
+  if (best != DEV_NULL) return best;
+  const int g = talshDeviceBusyLeast(DEV_NVIDIA_GPU);
+  if (g >= 0) return talshFlatDevId(DEV_NVIDIA_GPU, g);
+  return talshFlatDevId(DEV_HOST, 0);
+}
+
+// Logging switches (talshc.cpp:1057-1075): buffer-pool allocations and tensor
+// operations are printed to stdout while on.
+void talshMemManagerLogStart(void) { rt().log_mem = true; }
+void talshMemManagerLogFinish(void) { rt().log_mem = false; }
+void talshTensorOpLogStart(void) { rt().log_ops = true; }
+void talshTensorOpLogFinish(void) { rt().log_ops = false; }
 
 size_t talshDeviceTensorSize(int dev_num, int dev_kind) {
   // no per-tensor cap beyond the buffer itself (the reference caps at 1/6, mem_manager.cpp:626-631)
@@ -819,6 +873,45 @@ void talshTensorPrintInfo(const talsh_tens_t *t) {
   for (int i = 0; i < t->ndev; ++i)
     std::printf(" Image %d: device %d, data kind %d, available %d, ptr %p\n", i, t->dev_rsc[i].dev_id,
                 t->data_kind[i], t->avail[i], t->dev_rsc[i].gmem_p);
+}
+
+// Elements of the Host image with |x| >= thresh and their multi-indices
+// (talshc.cpp:1753-1850): one line "value i1 i2 ..." per element.
+void talshTensorPrintBody(const talsh_tens_t *t, double thresh) {
+  std::printf("#MESSAGE: Printing tensor body:");
+  if (!t || !t->shape_p || t->shape_p->num_dim < 0) {
+    std::printf(" Empty tensor\n");
+    return;
+  }
+  const int i = image_find(t, DEV_HOST, NO_TYPE, true);
+  if (i < 0) {
+    std::printf(" No Host image\n");
+    return;
+  }
+  hazard_drain(t->dev_rsc[i].gmem_p);
+  const int nd = t->shape_p->num_dim;
+  const int *dims = t->shape_p->dims;
+  const size_t v = shape_volume(t->shape_p);
+  const void *p = t->dev_rsc[i].gmem_p;
+  for (size_t l = 0; l < v; ++l) {
+    double re = 0, im = 0;
+    switch (t->data_kind[i]) {
+      case R4: re = ((const float *)p)[l]; break;
+      case R8: re = ((const double *)p)[l]; break;
+      case C4: re = ((const float *)p)[2 * l]; im = ((const float *)p)[2 * l + 1]; break;
+      case C8: re = ((const double *)p)[2 * l]; im = ((const double *)p)[2 * l + 1]; break;
+    }
+    const bool cplx = t->data_kind[i] == C4 || t->data_kind[i] == C8;
+    if (std::hypot(re, im) < thresh) continue;
+    if (cplx) std::printf("\n(%E,%E)", re, im);
+    else std::printf("\n%E", re);
+    size_t q = l;
+    for (int k = 0; k < nd; ++k) {
+      std::printf(" %zu", q % (size_t)dims[k]);
+      q /= (size_t)dims[k];
+    }
+  }
+  std::printf("\n#END OF MESSAGE\n");
 }
 
 double talshTensorImageNorm1_cpu(const talsh_tens_t *t) {

@@ -61,6 +61,7 @@ struct Runtime {
   int r4mode = R4_FP32;
   size_t dev_buffer_request = 0;  // 0: take a fraction of the free memory
   int nstreams = 4;
+  bool log_mem = false, log_ops = false;  // talshMemManagerLogStart / talshTensorOpLogStart
   std::unordered_map<const void *, Hazard> hazards;
   std::recursive_mutex mu;
 };

@@ -13,9 +13,12 @@ Outputs (all data; no reference source is stored):
                     tensor_contractions.txt, edge cases and random patterns.
   contr_perms.json  get_contr_permutations (tensor_algebra_cpu.F90:5613-5755)
                     outputs for those digital patterns, all four GEMM forms.
-  numeric.npz       small seeded copies/adds/contractions/norms/scales
-                    computed by the reference CPU path (inputs are regenerated
-                    from the seeds recorded in numeric.json).
+                    Each parsed pattern also carries "sym", the reference's
+                    digital -> symbolic form (get_contr_pattern_sym,
+                    tensor_algebra_cpu.F90:5533-5611).
+  numeric.npz       small seeded copies/adds/contractions/norms/scales and
+                    slices/inserts computed by the reference CPU path (inputs
+                    are regenerated from the seeds recorded in numeric.json).
 """
 import json
 import os
@@ -105,6 +108,9 @@ def main():
         row = {"pattern": p, "err": err}
         if err == 0:
             row.update({"drank": dr, "lrank": lr, "rrank": rr, "conj": cj, "dig": dig})
+            serr, sym = ref.pattern_sym(lr, rr, cj, dig)
+            if serr == 0:
+                row["sym"] = sym
         pat_rows.append(row)
     with open(os.path.join(HERE, "patterns.json"), "w") as f:
         json.dump({"patterns": pat_rows, "tensor_contractions_txt": shapes}, f, separators=(",", ":"))
@@ -170,6 +176,23 @@ def main():
         key = "c%d" % len(cases)
         arrays[key] = ref.scale(dk, dims, tensor(dk, dims, 88 + dk), alpha)
         cases.append({"key": key, "op": "scale", "dk": dk, "ldims": dims, "seed_l": 88 + dk, "alpha": list(alpha)})
+    # slices and inserts (talshTensorSlice / talshTensorInsert on the Host)
+    srng = random.Random(41)
+    for dk in (R4, R8, C4, C8):
+        for t in range(6):
+            rank = srng.randint(0, 5)
+            bdims = [srng.randint(1, 6) for _ in range(rank)]
+            sdims = [srng.randint(1, b) for b in bdims]
+            offsets = [srng.randint(0, b - x) for b, x in zip(bdims, sdims)]
+            seed_b, seed_s = 3000 + 10 * t + dk, 4000 + 10 * t + dk
+            B, S = tensor(dk, bdims, seed_b), tensor(dk, sdims, seed_s)
+            for op in ("slice", "insert"):
+                acc = t % 2
+                key = "c%d" % len(cases)
+                fn = ref.slice if op == "slice" else ref.insert
+                arrays[key] = fn(dk, sdims, S, bdims, B, offsets, acc)
+                cases.append({"key": key, "op": op, "dk": dk, "sdims": sdims, "bdims": bdims, "offsets": offsets,
+                              "seed_b": seed_b, "seed_s": seed_s, "accumulative": acc})
     np.savez_compressed(os.path.join(HERE, "numeric.npz"), **arrays)
     with open(os.path.join(HERE, "numeric.json"), "w") as f:
         json.dump(cases, f, separators=(",", ":"))

@@ -153,6 +153,24 @@ class Oracle:
                            ctypes.c_double(alpha[0]), ctypes.c_double(alpha[1]))
         return a
 
+    def slice(self, dk, sdims, s, bdims, b, offsets, accumulative=0):
+        """s (+)= b[offsets + i]; returns the new s."""
+        s = s.copy()
+        b = np.ascontiguousarray(b)
+        err = self.lib.orc_slice(dk, len(sdims), _intp(sdims), s.ctypes.data_as(ctypes.c_void_p), _intp(bdims),
+                                 b.ctypes.data_as(ctypes.c_void_p), _intp(offsets), accumulative)
+        assert err == 0, err
+        return s
+
+    def insert(self, dk, sdims, s, bdims, b, offsets, accumulative=0):
+        """b[offsets + i] (+)= s; returns the new b."""
+        b = b.copy()
+        s = np.ascontiguousarray(s)
+        err = self.lib.orc_insert(dk, len(sdims), _intp(sdims), s.ctypes.data_as(ctypes.c_void_p), _intp(bdims),
+                                  b.ctypes.data_as(ctypes.c_void_p), _intp(offsets), accumulative)
+        assert err == 0, err
+        return b
+
 
 class Reference:
     """ctypes view of oracle/_ref/libtalsh_ref.so (TAL-SH's own CPU path)."""
@@ -216,6 +234,26 @@ class Reference:
                                  ctypes.c_double(alpha[0]), ctypes.c_double(alpha[1]))
         assert err == 0, err
         return d
+
+    def slice(self, dk, sdims, s, bdims, b, offsets, accumulative=0):
+        s, b = s.copy(), b.copy()
+        err = self.lib.ref_slice(dk, len(sdims), _intp(sdims), s.ctypes.data_as(ctypes.c_void_p), _intp(bdims),
+                                 b.ctypes.data_as(ctypes.c_void_p), _intp(offsets), accumulative)
+        assert err == 0, err
+        return s
+
+    def insert(self, dk, sdims, s, bdims, b, offsets, accumulative=0):
+        s, b = s.copy(), b.copy()
+        err = self.lib.ref_insert(dk, len(sdims), _intp(sdims), s.ctypes.data_as(ctypes.c_void_p), _intp(bdims),
+                                  b.ctypes.data_as(ctypes.c_void_p), _intp(offsets), accumulative)
+        assert err == 0, err
+        return b
+
+    def pattern_sym(self, lrank, rrank, conj, dig):
+        buf = ctypes.create_string_buffer(4096)
+        cpl = ctypes.c_int()
+        err = self.lib.ref_pattern_sym(lrank, rrank, conj, _intp(dig), buf, ctypes.byref(cpl))
+        return err, buf.value.decode() if err == 0 else ""
 
 
 def _split_volume(rng, vol, n, lo=2):

@@ -75,6 +75,27 @@ def test_product_parser_matches_reference_golden():
             row["drank"], row["lrank"], row["rrank"], row["conj"], row["dig"]), row["pattern"]
 
 
+def test_product_pattern_sym_matches_reference_golden():
+    """get_contr_pattern_sym (digital -> symbolic) against the reference routine's
+    outputs, and (for ordinary contractions) parsing its output gives back the
+    same digital pattern."""
+    rows = [r for r in json.load(open(os.path.join(GOLDEN, "patterns.json")))["patterns"] if "sym" in r]
+    assert len(rows) > 400
+    lib = ts.load()
+    dig = (ctypes.c_int * 128)()
+    dr, lr, rr, cj = (ctypes.c_int() for _ in range(4))
+    for row in rows:
+        err, sym = ts.pattern_sym(row["lrank"], row["rrank"], row["conj"], row["dig"])
+        assert err == 0 and sym == row["sym"], (row["pattern"], sym, row["sym"])
+        pos = [d for d in row["dig"] if d > 0]
+        if len(pos) != len(set(pos)):
+            continue  # hyper-contraction: the routine names D's indices per occurrence, as the reference does
+        assert lib.talsh_get_contr_ptrn_str2dig(sym.encode(), dig, ctypes.byref(dr), ctypes.byref(lr),
+                                                ctypes.byref(rr), ctypes.byref(cj)) == 0
+        assert list(dig[:lr.value + rr.value]) == row["dig"] and cj.value == row["conj"]
+    assert ts.pattern_sym(-1, 2, 0, [1, 2])[0] == 1
+
+
 def plan(pattern, dd, ld, rd, acc=1):
     lib = ts.load()
     mnk = (ctypes.c_longlong * 3)()

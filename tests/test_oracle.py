@@ -76,8 +76,14 @@ def test_numeric_golden(oracle):
     for c in cases:
         want = arrays[c["key"]]
         dk = c["dk"]
-        L, R, D = _inputs(c)
         seen.add((c["op"], dk))
+        if c["op"] in ("slice", "insert"):
+            B, S = tensor(dk, c["bdims"], c["seed_b"]), tensor(dk, c["sdims"], c["seed_s"])
+            fn = oracle.slice if c["op"] == "slice" else oracle.insert
+            got = fn(dk, c["sdims"], S, c["bdims"], B, c["offsets"], c["accumulative"])
+            assert got.tobytes() == want.astype(NP_DTYPE[dk]).tobytes(), c
+            continue
+        L, R, D = _inputs(c)
         if c["op"] in ("copy", "add"):
             err, dr, lr, rr, cj, dig = oracle.parse(c["pattern"])
             o2n = dig
@@ -98,7 +104,7 @@ def test_numeric_golden(oracle):
         elif c["op"] == "scale":
             got = oracle.scale(dk, L, tuple(c["alpha"]))
             assert got.tobytes() == want.astype(NP_DTYPE[dk]).tobytes(), c
-    for op in ("copy", "add", "contract", "scale"):
+    for op in ("copy", "add", "contract", "scale", "slice", "insert"):
         for dk in (R4, R8, C4, C8):
             assert (op, dk) in seen
     assert ("norm2", R8) in seen and ("norm2", R4) in seen
@@ -119,6 +125,35 @@ def test_copy_closed_form(oracle):
             axes[t - 1] = k
         want = np.transpose(A, axes).reshape(-1, order="F")
         assert np.array_equal(got, want)
+
+
+def test_slice_insert_closed_form(oracle):
+    """Big-tensor values encode their own dlf position, so a slice must hold
+    sum_k (offset_k + i_k) * stride_k at slice position i; an insert followed by
+    the same slice returns the inserted block; an accumulative slice adds."""
+    rng = random.Random(9)
+    for trial in range(60):
+        rank = rng.randint(0, 5)
+        bd = [rng.randint(1, 6) for _ in range(rank)]
+        sd = [rng.randint(1, b) for b in bd]
+        off = [rng.randint(0, b - x) for b, x in zip(bd, sd)]
+        big = np.arange(int(np.prod(bd)) if bd else 1, dtype=np.float64)
+        sl = oracle.slice(R8, sd, np.zeros(int(np.prod(sd)) if sd else 1), bd, big, off)
+        strides = np.cumprod([1] + bd[:-1]) if bd else []
+        want = np.zeros(sl.size)
+        for lin in range(sl.size):
+            q, pos = lin, 0
+            for k in range(rank):
+                pos += (off[k] + q % sd[k]) * strides[k]
+                q //= sd[k]
+            want[lin] = pos
+        assert np.array_equal(sl, want), (sd, bd, off)
+        blk = -1.0 - np.arange(sl.size, dtype=np.float64)
+        big2 = oracle.insert(R8, sd, blk, bd, big, off)
+        assert np.array_equal(oracle.slice(R8, sd, np.zeros(sl.size), bd, big2, off), blk)
+        assert int(np.sum(big2 != big)) == sl.size  # nothing outside the block moved
+        acc = oracle.slice(R8, sd, np.full(sl.size, 0.5), bd, big, off, accumulative=1)
+        assert np.array_equal(acc, want + 0.5)
 
 
 def test_contract_closed_form_constant_tensors(oracle):
