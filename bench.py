@@ -265,7 +265,20 @@ def bench_batch(oracle, dk, ntask=4096, reps=5, check_sample=128):
     for (pat, dd, ld, rd, L, R, lt, rt, dt) in tasks:
         for t in (lt, rt, dt):
             t.destruct()
-    return {"tasks": ntask, "flop": flop, "batched_tflops": flop / batched_s / 1e12,
+    # per-task roofline: max(FLOP / MFMA peak, bytes / HBM peak), bytes = L + R read + D written
+    # (non-accumulative); K = sqrt(vol L * vol R / vol D) for a contraction without hyper-indices
+    esz, peak = (8, FP64_PEAK_TFLOPS) if dk == R8 else (4, FP32_PEAK_TFLOPS)
+    bound_s, nbytes = 0.0, 0.0
+    for (pat, dd, ld, rd, *_rest) in tasks:
+        vd, vl, vr = (float(np.prod(x)) if x else 1.0 for x in (dd, ld, rd))
+        f = 2.0 * vd * np.sqrt(vl * vr / vd)
+        by = esz * (vl + vr + vd)
+        nbytes += by
+        bound_s += max(f / (peak * 1e12), by / (HBM_PEAK_GBS * 1e9))
+    gpu_s = float(np.median(gpu_ms)) * 1e-3
+    return {"tasks": ntask, "flop": flop, "bytes": nbytes, "batched_tflops": flop / batched_s / 1e12,
+            "batched_gpu_tflops": flop / gpu_s / 1e12, "batched_gpu_gbs": nbytes / gpu_s / 1e9,
+            "roofline_bound_ms": 1e3 * bound_s, "roofline_frac": bound_s / gpu_s,
             "batched_ms": 1e3 * batched_s, "batched_gpu_ms": float(np.median(gpu_ms)),
             "per_task_tflops": flop / single_s / 1e12, "per_task_ms": 1e3 * single_s,
             "speedup_vs_per_task": single_s / batched_s, "first_execution_s": first_s,
