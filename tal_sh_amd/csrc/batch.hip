@@ -50,15 +50,22 @@ constexpr int kMaxTile = 64;
 
 __global__ void batch_reset_kernel(unsigned long long *counter) { *counter = 0ull; }
 
+// One work item = one tm x tn tile of one task's D.  Thread tid always loads
+// row tid % tm of the A tile and column tid % tn of the B tile (tm, tn divide
+// the 256 threads), so the row/column offsets come from the tables once per
+// item and only the contracted-index table is read per element.  The next
+// K step is gathered into registers while the current one is multiplied
+// from LDS (double-buffered, one barrier per step).
 template <typename S>
-__global__ __launch_bounds__(kThreads) void batch_contract_kernel(const BatchTask *__restrict__ tasks,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void batch_contract_kernel(const BatchTask *__restrict__ tasks,
                                                                   const long long *__restrict__ prefix, int ntasks,
                                                                   const int *__restrict__ tabs,
                                                                   unsigned long long *counter, long long total) {
   using MF = BMfma<S>;
   using acc_t = typename MF::acc_t;
-  __shared__ S As[kMaxTile][kBK + 1];
-  __shared__ S Bs[kMaxTile][kBK + 1];
+  constexpr int NR = kMaxTile * kBK / kThreads;  // registers per operand (tile 64 x 16)
+  __shared__ S As[2][kMaxTile][kBK + 1];
+  __shared__ S Bs[2][kMaxTile][kBK + 1];
   __shared__ long long s_item;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   while (true) {
@@ -76,49 +83,64 @@ __global__ __launch_bounds__(kThreads) void batch_contract_kernel(const BatchTas
     }
     const BatchTask &T = tasks[lo];
     const long long local = item - prefix[lo];
-    const int tm = T.tm, tn = T.tn;
+    const int tm = T.tm, tn = T.tn, K = T.K;
     const int m0 = (int)(local % T.tiles_m) * tm, n0 = (int)(local / T.tiles_m) * tn;
     const int *lM = tabs + T.tab, *lK = lM + T.M, *rK = lK + T.K, *rN = rK + T.K, *dM = rN + T.N, *dN = dM + T.M;
     const S *L = (const S *)T.L, *R = (const S *)T.R;
+    // this thread's A row / B column and its k positions within a K step
+    const int am = tid % tm, ak = tid / tm, astep = kThreads / tm, na = tm * kBK / kThreads;
+    const int bn = tid % tn, bk = tid / tn, bstep = kThreads / tn, nb = tn * kBK / kThreads;
+    const bool arow_ok = m0 + am < T.M, bcol_ok = n0 + bn < T.N;
+    const int64_t arow = arow_ok ? lM[m0 + am] : 0, bcol = bcol_ok ? rN[n0 + bn] : 0;
+    S ra[NR], rb[NR];
+    auto gather = [&](int k0) {
+#pragma unroll
+      for (int i = 0; i < NR; ++i) {
+        const int ka = k0 + ak + i * astep, kb = k0 + bk + i * bstep;
+        ra[i] = (i < na && arow_ok && ka < K) ? L[arow + lK[ka]] : (S)0;
+        rb[i] = (i < nb && bcol_ok && kb < K) ? R[rK[kb] + bcol] : (S)0;
+      }
+    };
+    auto stage = [&](int buf) {
+#pragma unroll
+      for (int i = 0; i < NR; ++i) {
+        if (i < na) As[buf][am][ak + i * astep] = ra[i];
+        if (i < nb) Bs[buf][bn][bk + i * bstep] = rb[i];
+      }
+    };
     const int subm = tm >> 4, nsub = subm * (tn >> 4);
     acc_t acc[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) acc[q] = acc_t{0, 0, 0, 0};
-    for (int k0 = 0; k0 < T.K; k0 += kBK) {
-      for (int e = tid; e < tm * kBK; e += kThreads) {
-        const int m = e % tm, k = e / tm;
-        S v = (S)0;
-        if (m0 + m < T.M && k0 + k < T.K) v = L[lM[m0 + m] + lK[k0 + k]];
-        As[m][k] = v;
-      }
-      for (int e = tid; e < tn * kBK; e += kThreads) {
-        const int n = e % tn, k = e / tn;
-        S v = (S)0;
-        if (n0 + n < T.N && k0 + k < T.K) v = R[rK[k0 + k] + rN[n0 + n]];
-        Bs[n][k] = v;
-      }
-      __syncthreads();
+    gather(0);
+    stage(0);
+    __syncthreads();
+    const int nk = (K + kBK - 1) / kBK;
+    for (int kt = 0; kt < nk; ++kt) {
+      const int buf = kt & 1;
+      if (kt + 1 < nk) gather((kt + 1) * kBK);
 #pragma unroll
       for (int ks = 0; ks < kBK / 4; ++ks) {
         const int kk = ks * 4 + (lane >> 4);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const int s = wave + 4 * q;
-          if (s < nsub) {
-            const int si = s % subm, sj = s / subm;
-            acc[q] = MF::op(As[si * 16 + (lane & 15)][kk], Bs[sj * 16 + (lane & 15)][kk], acc[q]);
+          const int sidx = wave + 4 * q;
+          if (sidx < nsub) {
+            const int si = sidx % subm, sj = sidx / subm;
+            acc[q] = MF::op(As[buf][si * 16 + (lane & 15)][kk], Bs[buf][sj * 16 + (lane & 15)][kk], acc[q]);
           }
         }
       }
+      if (kt + 1 < nk) stage(buf ^ 1);
       __syncthreads();
     }
     S *D = (S *)T.D;
     const S alpha = (S)T.alpha;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const int s = wave + 4 * q;
-      if (s >= nsub) continue;
-      const int si = s % subm, sj = s / subm;
+      const int sidx = wave + 4 * q;
+      if (sidx >= nsub) continue;
+      const int si = sidx % subm, sj = sidx / subm;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int m = m0 + si * 16 + MF::row(lane, r), n = n0 + sj * 16 + (lane & 15);

@@ -45,6 +45,12 @@ struct talshAmdBatch {
   hipGraphExec_t exec = nullptr;
   int tuned_classes = 0, replays = 0;
   double tune_ms = 0.0;
+  // Ordering state shared by every operand: the hazard entries of all its
+  // tensors point at `hz`, re-recorded after each execution.  `epoch` is the
+  // hazard-table epoch right after the last execution; while it is unchanged
+  // nothing else has touched any operand (or moved a body) since.
+  tal::EvRef hz;
+  uint64_t epoch = 0;
 };
 
 namespace tal {
@@ -413,17 +419,25 @@ int talshAmdBatchExecute(talshAmdBatch *b, int dev_id, int dev_kind, talsh_task_
   if (g < 0) return TALSH_NOT_AVAILABLE;
   Device &dev = rt().gpus[g];
   (void)hipSetDevice(dev.hip_id);
-  if (!b->built || b->g != g || signature(b, g) != b->sig) {
+  // Replay fast path: same GPU, graph built, hazard table unchanged since the
+  // last execution, so every operand's ordering state is exactly `hz` and one
+  // wait on it covers all of them (no per-tensor lookups).
+  const bool fast = b->built && b->g == g && b->hz && rt().hazard_epoch == b->epoch;
+  if (!fast && (!b->built || b->g != g || signature(b, g) != b->sig)) {
     const int e = build(b, g);
     if (e) return e;
   }
-  Task *task = task_new(g);
+  Task *task = task_new(g, 3.0 * (double)b->entries.size() * 4096.0);
   hipStream_t s = task->stream;
   (void)hipEventRecord(task->ev_start->ev, s);
-  for (size_t i = 0; i < b->entries.size(); ++i) {
-    hazard_wait_write(b->sig[3 * i], s);
-    hazard_wait_read(b->sig[3 * i + 1], s);
-    hazard_wait_read(b->sig[3 * i + 2], s);
+  if (fast) {
+    (void)hipStreamWaitEvent(s, b->hz->ev, 0);
+  } else {
+    for (size_t i = 0; i < b->entries.size(); ++i) {
+      hazard_wait_write(b->sig[3 * i], s);
+      hazard_wait_read(b->sig[3 * i + 1], s);
+      hazard_wait_read(b->sig[3 * i + 2], s);
+    }
   }
   (void)hipEventRecord(task->ev_in->ev, s);
   task->ev_mm0 = new_event(g);
@@ -436,14 +450,19 @@ int talshAmdBatchExecute(talshAmdBatch *b, int dev_id, int dev_kind, talsh_task_
   (void)hipEventRecord(task->ev_mm1->ev, s);
   (void)hipEventRecord(task->ev_comp->ev, s);
   (void)hipEventRecord(task->ev_done->ev, s);
-  for (size_t i = 0; i < b->entries.size(); ++i) {
-    hazard_set_writer(b->sig[3 * i], task->ev_done);
-    hazard_add_reader(b->sig[3 * i + 1], task->ev_done);
-    hazard_add_reader(b->sig[3 * i + 2], task->ev_done);
-    talsh_tens_t *d = b->entries[i].d;  // other images of each output become stale
-    const int gi = image_find(d, flat_gpu(g), NO_TYPE, true);
-    if (gi >= 0) image_discard_other(d, gi);
+  if (!fast) {
+    if (!b->hz) b->hz = new_event(g);
+    for (size_t i = 0; i < b->entries.size(); ++i) {
+      hazard_set_writer(b->sig[3 * i], b->hz);
+      hazard_add_reader(b->sig[3 * i + 1], b->hz);
+      hazard_add_reader(b->sig[3 * i + 2], b->hz);
+      talsh_tens_t *d = b->entries[i].d;  // other images of each output become stale
+      const int gi = image_find(d, flat_gpu(g), NO_TYPE, true);
+      if (gi >= 0) image_discard_other(d, gi);
+    }
   }
+  (void)hipEventRecord(b->hz->ev, s);
+  b->epoch = rt().hazard_epoch;
   task->flops = b->flops;
   dev.tasks_submitted++;
   dev.active++;

@@ -110,7 +110,10 @@ void hazard_wait_write(const void *p, hipStream_t s) {
 }
 
 void hazard_add_reader(const void *p, const EvRef &e) {
+  rt().hazard_epoch++;
   Hazard &h = rt().hazards[p];
+  for (auto &x : h.readers)
+    if (x == e) return;  // the same event (re-recorded by its owner) is listed once
   if (h.readers.size() >= 16) {  // prune completed readers
     std::vector<EvRef> keep;
     for (auto &x : h.readers)
@@ -121,6 +124,7 @@ void hazard_add_reader(const void *p, const EvRef &e) {
 }
 
 void hazard_set_writer(const void *p, const EvRef &e) {
+  rt().hazard_epoch++;
   Hazard &h = rt().hazards[p];
   h.writer = e;
   h.readers.clear();
@@ -129,6 +133,7 @@ void hazard_set_writer(const void *p, const EvRef &e) {
 void hazard_drain(const void *p) {
   auto it = rt().hazards.find(p);
   if (it == rt().hazards.end()) return;
+  rt().hazard_epoch++;
   if (it->second.writer && it->second.writer->ev) (void)hipEventSynchronize(it->second.writer->ev);
   for (auto &e : it->second.readers)
     if (e && e->ev) (void)hipEventSynchronize(e->ev);
@@ -376,6 +381,7 @@ int talshShutdown(void) {
     (void)hipDeviceSynchronize();
   }
   r.hazards.clear();
+  r.hazard_epoch++;
   for (auto &d : r.gpus) {
     if (!d.on) continue;
     d.backlog.clear();

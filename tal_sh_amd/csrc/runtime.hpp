@@ -63,6 +63,9 @@ struct Runtime {
   int nstreams = 4;
   bool log_mem = false, log_ops = false;  // talshMemManagerLogStart / talshTensorOpLogStart
   std::unordered_map<const void *, Hazard> hazards;
+  // bumped by every change of the hazard table: a caller that saw the same
+  // value before knows every entry is still what it left there
+  uint64_t hazard_epoch = 0;
   std::recursive_mutex mu;
 };
 

@@ -111,3 +111,39 @@ def test_batch_rejects_aliased_outputs(talsh_gpu):
         assert b.add("D(a,b)+=L(a,i)*R(i,b)", d, l, r) == 0
     assert b.execute() == ts.TALSH_INVALID_ARGS
     b.destroy()
+
+
+@pytest.mark.parametrize("dk", (R8, R4))
+def test_batch_back_to_back_replays(talsh_gpu, oracle, dk):
+    """Replays with nothing else touching the operands in between take the
+    replay fast path (one wait on the batch's own event); accumulative tasks
+    must then hold D0 + reps * alpha * L * R, and a Host read in between still
+    sees every replay before it."""
+    ts = talsh_gpu
+    rng = random.Random(77 + dk)
+    tasks = []
+    for t in range(96):
+        pat, dd, ld, rd = batch_contraction(rng)
+        L, R, D = tensor(dk, ld, 5 * t), tensor(dk, rd, 5 * t + 1), tensor(dk, dd, 5 * t + 2)
+        tasks.append((pat, dd, ld, rd, L, R, D, upload(ts, dk, ld, L), upload(ts, dk, rd, R), upload(ts, dk, dd, D)))
+    b = ts.Batch()
+    for (pat, dd, ld, rd, L, R, D, lt, rt, dt) in tasks:
+        assert b.add(pat, dt, lt, rt, alpha=0.25, accumulative=1) == 0
+    one = [oracle.contract(dk, pat, dd, np.zeros_like(D), ld, L, rd, R, (0.25, 0.0), 0)
+           for (pat, dd, ld, rd, L, R, D, lt, rt, dt) in tasks]
+    tol = 1e-12 if dk == R8 else 1e-5
+    reps = 0
+    for burst in (3, 2):
+        say("batch back-to-back", dk, burst)
+        for _ in range(burst):
+            tk = ts.Task()
+            assert b.execute(tk) == 0
+            assert tk.wait() == ts.TALSH_TASK_COMPLETED
+            tk.destruct()
+            reps += 1
+        for i, (pat, dd, ld, rd, L, R, D, lt, rt, dt) in enumerate(tasks):
+            want = D + reps * one[i]
+            assert relerr(dt.host_array(), want) <= tol, (reps, i, pat)
+            dt.place(ts.gpu_flat(0))
+    assert b.stats()["executions"] == reps
+    b.destroy()
