@@ -50,123 +50,169 @@ constexpr int kMaxTile = 64;
 
 __global__ void batch_reset_kernel(unsigned long long *counter) { *counter = 0ull; }
 
-// One work item = one tm x tn tile of one task's D.  Thread tid always loads
-// row tid % tm of the A tile and column tid % tn of the B tile (tm, tn divide
-// the 256 threads), so the row/column offsets come from the tables once per
-// item and only the contracted-index table is read per element.  The next
-// K step is gathered into registers while the current one is multiplied
-// from LDS (double-buffered, one barrier per step).
+// One work item = one tm x tn tile of one task's D.  Element i of a thread
+// in the A tile is (row am(i), step column ak(i)): along m (AM = 0: one row
+// tid % tm, k = tid / tm + i * 256 / tm) when an uncontracted index is the
+// contiguous one of L, along k (AM = 1: k = tid % 16, rows tid / 16 + 16 i)
+// when a contracted index is, so consecutive lanes read consecutive
+// addresses either way; likewise for B with n (BM).  Row / column offsets
+// come from the tables once per item; only the contracted-index table is read
+// per element.  The next K step is gathered into registers while the current
+// one is multiplied from LDS (double-buffered, one barrier per step).
 template <typename S>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void batch_contract_kernel(const BatchTask *__restrict__ tasks,
-                                                                  const long long *__restrict__ prefix, int ntasks,
-                                                                  const int *__restrict__ tabs,
-                                                                  unsigned long long *counter, long long total) {
+struct TileSmem {
+  S A[2][kMaxTile][kBK + 1];
+  S B[2][kMaxTile][kBK + 1];
+};
+
+template <typename S, int AM, int BM>
+__device__ __forceinline__ void tile_body(const BatchTask &T, long long local, const int *__restrict__ tabs,
+                                          TileSmem<S> &sm, int tid) {
   using MF = BMfma<S>;
   using acc_t = typename MF::acc_t;
-  constexpr int NR = kMaxTile * kBK / kThreads;  // registers per operand (tile 64 x 16)
-  __shared__ S As[2][kMaxTile][kBK + 1];
-  __shared__ S Bs[2][kMaxTile][kBK + 1];
-  __shared__ long long s_item;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  while (true) {
-    if (tid == 0) s_item = (long long)atomicAdd(counter, 1ull);
-    __syncthreads();
-    const long long item = s_item;
-    __syncthreads();
-    if (item >= total) break;
-    // task owning this item: last t with prefix[t] <= item
-    int lo = 0, hi = ntasks - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (prefix[mid] <= item) lo = mid;
-      else hi = mid - 1;
+  constexpr int NR = kMaxTile * kBK / kThreads;  // elements per thread and operand (tile 64 x 16)
+  const int lane = tid & 63, wave = tid >> 6;
+  const int tm = T.tm, tn = T.tn, K = T.K;
+  const int m0 = (int)(local % T.tiles_m) * tm, n0 = (int)(local / T.tiles_m) * tn;
+  const int *lM = tabs + T.tab, *lK = lM + T.M, *rK = lK + T.K, *rN = rK + T.K, *dM = rN + T.N, *dN = dM + T.M;
+  const S *L = (const S *)T.L, *R = (const S *)T.R;
+  const int na = tm * kBK / kThreads, nb = tn * kBK / kThreads;
+  const int tml = __builtin_ctz(tm), tnl = __builtin_ctz(tn);  // tm, tn are powers of two
+  auto am = [&](int i) { return AM ? (tid >> 4) + kBK * i : tid & (tm - 1); };
+  auto ak = [&](int i) { return AM ? tid & (kBK - 1) : (tid >> tml) + i * (kThreads >> tml); };
+  auto bn = [&](int i) { return BM ? (tid >> 4) + kBK * i : tid & (tn - 1); };
+  auto bk = [&](int i) { return BM ? tid & (kBK - 1) : (tid >> tnl) + i * (kThreads >> tnl); };
+  int aoff[NR], boff[NR];
+#pragma unroll
+  for (int i = 0; i < NR; ++i) {
+    aoff[i] = (i < na && m0 + am(i) < T.M) ? lM[m0 + am(i)] : -1;
+    boff[i] = (i < nb && n0 + bn(i) < T.N) ? rN[n0 + bn(i)] : -1;
+  }
+  // two register sets: step s is gathered into set s & 1 two steps ahead
+  S ra0[NR], rb0[NR], ra1[NR], rb1[NR];
+#define TAL_GATHER(RA, RB, k0)                                           \
+  _Pragma("unroll") for (int i = 0; i < NR; ++i) {                       \
+    const int ka = (k0) + ak(i), kb = (k0) + bk(i);                      \
+    RA[i] = (aoff[i] >= 0 && ka < K) ? L[aoff[i] + lK[ka]] : (S)0;       \
+    RB[i] = (boff[i] >= 0 && kb < K) ? R[rK[kb] + boff[i]] : (S)0;       \
+  }
+#define TAL_STAGE(RA, RB, buf)                                           \
+  _Pragma("unroll") for (int i = 0; i < NR; ++i) {                       \
+    if (i < na) sm.A[buf][am(i)][ak(i)] = RA[i];                         \
+    if (i < nb) sm.B[buf][bn(i)][bk(i)] = RB[i];                         \
+  }
+  const int subm = tm >> 4, nsub = subm * (tn >> 4);
+  acc_t acc[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) acc[q] = acc_t{0, 0, 0, 0};
+  constexpr int UK = sizeof(S) == 8 ? 1 : 2;  // fewer live fragments for FP64 (register budget)
+#define TAL_COMPUTE(buf)                                                                                   \
+  _Pragma("unroll UK") for (int ks = 0; ks < kBK / 4; ++ks) {                                              \
+    const int kk = ks * 4 + (lane >> 4);                                                                   \
+    _Pragma("unroll") for (int q = 0; q < 4; ++q) {                                                        \
+      const int sidx = wave + 4 * q;                                                                       \
+      if (sidx < nsub) {                                                                                   \
+        const int si = sidx % subm, sj = sidx / subm;                                                      \
+        acc[q] = MF::op(sm.A[buf][si * 16 + (lane & 15)][kk], sm.B[buf][sj * 16 + (lane & 15)][kk], acc[q]); \
+      }                                                                                                    \
+    }                                                                                                      \
+  }
+  const int nk = (K + kBK - 1) / kBK;
+  TAL_GATHER(ra0, rb0, 0)
+  TAL_STAGE(ra0, rb0, 0)
+  __syncthreads();
+  if (nk > 1) {
+    TAL_GATHER(ra1, rb1, kBK)
+  }
+  if (nk > 2) {
+    TAL_GATHER(ra0, rb0, 2 * kBK)
+  }
+  for (int kt = 0; kt < nk; kt += 2) {
+    // even step kt from buffer 0; step kt+1 (set 1) goes to buffer 1, set 1 refills with kt+3
+    TAL_COMPUTE(0)
+    if (kt + 1 < nk) {
+      TAL_STAGE(ra1, rb1, 1)
     }
-    const BatchTask &T = tasks[lo];
-    const long long local = item - prefix[lo];
-    const int tm = T.tm, tn = T.tn, K = T.K;
-    const int m0 = (int)(local % T.tiles_m) * tm, n0 = (int)(local / T.tiles_m) * tn;
-    const int *lM = tabs + T.tab, *lK = lM + T.M, *rK = lK + T.K, *rN = rK + T.K, *dM = rN + T.N, *dN = dM + T.M;
-    const S *L = (const S *)T.L, *R = (const S *)T.R;
-    // this thread's A row / B column and its k positions within a K step
-    const int am = tid % tm, ak = tid / tm, astep = kThreads / tm, na = tm * kBK / kThreads;
-    const int bn = tid % tn, bk = tid / tn, bstep = kThreads / tn, nb = tn * kBK / kThreads;
-    const bool arow_ok = m0 + am < T.M, bcol_ok = n0 + bn < T.N;
-    const int64_t arow = arow_ok ? lM[m0 + am] : 0, bcol = bcol_ok ? rN[n0 + bn] : 0;
-    S ra[NR], rb[NR];
-    auto gather = [&](int k0) {
-#pragma unroll
-      for (int i = 0; i < NR; ++i) {
-        const int ka = k0 + ak + i * astep, kb = k0 + bk + i * bstep;
-        ra[i] = (i < na && arow_ok && ka < K) ? L[arow + lK[ka]] : (S)0;
-        rb[i] = (i < nb && bcol_ok && kb < K) ? R[rK[kb] + bcol] : (S)0;
-      }
-    };
-    auto stage = [&](int buf) {
-#pragma unroll
-      for (int i = 0; i < NR; ++i) {
-        if (i < na) As[buf][am][ak + i * astep] = ra[i];
-        if (i < nb) Bs[buf][bn][bk + i * bstep] = rb[i];
-      }
-    };
-    const int subm = tm >> 4, nsub = subm * (tn >> 4);
-    acc_t acc[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) acc[q] = acc_t{0, 0, 0, 0};
-    gather(0);
-    stage(0);
     __syncthreads();
-    const int nk = (K + kBK - 1) / kBK;
-    for (int kt = 0; kt < nk; ++kt) {
-      const int buf = kt & 1;
-      if (kt + 1 < nk) gather((kt + 1) * kBK);
-#pragma unroll
-      for (int ks = 0; ks < kBK / 4; ++ks) {
-        const int kk = ks * 4 + (lane >> 4);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int sidx = wave + 4 * q;
-          if (sidx < nsub) {
-            const int si = sidx % subm, sj = sidx / subm;
-            acc[q] = MF::op(As[buf][si * 16 + (lane & 15)][kk], Bs[buf][sj * 16 + (lane & 15)][kk], acc[q]);
-          }
-        }
-      }
-      if (kt + 1 < nk) stage(buf ^ 1);
-      __syncthreads();
+    if (kt + 3 < nk) {
+      TAL_GATHER(ra1, rb1, (kt + 3) * kBK)
     }
-    S *D = (S *)T.D;
-    const S alpha = (S)T.alpha;
+    if (kt + 1 >= nk) break;
+    // odd step kt+1 from buffer 1; step kt+2 (set 0) goes to buffer 0, set 0 refills with kt+4
+    TAL_COMPUTE(1)
+    if (kt + 2 < nk) {
+      TAL_STAGE(ra0, rb0, 0)
+    }
+    __syncthreads();
+    if (kt + 4 < nk) {
+      TAL_GATHER(ra0, rb0, (kt + 4) * kBK)
+    }
+  }
+#undef TAL_COMPUTE
+#undef TAL_GATHER
+#undef TAL_STAGE
+  S *D = (S *)T.D;
+  const S alpha = (S)T.alpha;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int sidx = wave + 4 * q;
-      if (sidx >= nsub) continue;
-      const int si = sidx % subm, sj = sidx / subm;
+  for (int q = 0; q < 4; ++q) {
+    const int sidx = wave + 4 * q;
+    if (sidx >= nsub) continue;
+    const int si = sidx % subm, sj = sidx / subm;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int m = m0 + si * 16 + MF::row(lane, r), n = n0 + sj * 16 + (lane & 15);
-        if (m < T.M && n < T.N) {
-          S *d = D + dM[m] + dN[n];
-          const S v = alpha * acc[q][r];
-          *d = T.beta_one ? *d + v : v;
-        }
+    for (int r = 0; r < 4; ++r) {
+      const int m = m0 + si * 16 + MF::row(lane, r), n = n0 + sj * 16 + (lane & 15);
+      if (m < T.M && n < T.N) {
+        S *d = D + dM[m] + dN[n];
+        const S v = alpha * acc[q][r];
+        *d = T.beta_one ? *d + v : v;
       }
     }
   }
 }
 
+// The index of the next work item is fetched (one atomic by thread 0) while
+// the current tile is computed, and published through a two-slot LDS buffer,
+// so neither the atomic's round trip nor a task search sits between tiles.
+template <typename S>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void batch_contract_kernel(
+    const BatchTask *__restrict__ tasks, const long long *__restrict__ prefix, const int *__restrict__ item_task,
+    const int *__restrict__ tabs, unsigned long long *counter, long long total) {
+  __shared__ TileSmem<S> sm;
+  __shared__ long long s_item[2];
+  const int tid = threadIdx.x;
+  if (tid == 0) s_item[0] = (long long)atomicAdd(counter, 1ull);
+  __syncthreads();
+  long long item = s_item[0];
+  for (int it = 1; item < total; ++it) {
+    // slot it & 1 was last read before the barriers of the previous tile
+    if (tid == 0) s_item[it & 1] = (long long)atomicAdd(counter, 1ull);
+    const int t = item_task[item];
+    const BatchTask &T = tasks[t];
+    const long long local = item - prefix[t];
+    switch (T.amode * 2 + T.bmode) {  // uniform over the block
+      case 0: tile_body<S, 0, 0>(T, local, tabs, sm, tid); break;
+      case 1: tile_body<S, 0, 1>(T, local, tabs, sm, tid); break;
+      case 2: tile_body<S, 1, 0>(T, local, tabs, sm, tid); break;
+      default: tile_body<S, 1, 1>(T, local, tabs, sm, tid); break;
+    }
+    __syncthreads();  // the next index is visible; every thread leaves the loop together
+    item = s_item[it & 1];
+  }
+}
+
 }  // namespace
 
-int batch_launch(int dk, const BatchTask *tasks, const long long *prefix, int ntasks, const int *tabs,
+int batch_launch(int dk, const BatchTask *tasks, const long long *prefix, const int *item_task, const int *tabs,
                  unsigned long long *counter, long long total, int grid, hipStream_t s) {
-  if (!tasks || !prefix || !tabs || !counter || ntasks <= 0 || total < 0 || grid <= 0) return -1;
+  if (!tasks || !prefix || !item_task || !tabs || !counter || total < 0 || grid <= 0) return -1;
   batch_reset_kernel<<<1, 1, 0, s>>>(counter);
   if (total == 0) return hipGetLastError() == hipSuccess ? 0 : -2;
   switch (dk) {
     case 8:
-      batch_contract_kernel<double><<<grid, kThreads, 0, s>>>(tasks, prefix, ntasks, tabs, counter, total);
+      batch_contract_kernel<double><<<grid, kThreads, 0, s>>>(tasks, prefix, item_task, tabs, counter, total);
       break;
     case 4:
-      batch_contract_kernel<float><<<grid, kThreads, 0, s>>>(tasks, prefix, ntasks, tabs, counter, total);
+      batch_contract_kernel<float><<<grid, kThreads, 0, s>>>(tasks, prefix, item_task, tabs, counter, total);
       break;
     default: return -1;
   }

@@ -39,6 +39,7 @@ struct talshAmdBatch {
   int dev_handle = -1;
   tal::BatchTask *d_tasks = nullptr;
   long long *d_prefix = nullptr;
+  int *d_item_task = nullptr;
   unsigned long long *d_counter = nullptr;
   int *d_tabs = nullptr;
   hipGraph_t graph = nullptr;
@@ -51,6 +52,7 @@ struct talshAmdBatch {
   // nothing else has touched any operand (or moved a body) since.
   tal::EvRef hz;
   uint64_t epoch = 0;
+  int policy = 0;  // tile policy chosen by whole-batch timing
 };
 
 namespace tal {
@@ -159,10 +161,32 @@ int plan_task(const talshAmdBatch::Entry &e, int g, HostTask *h) {
       nd.push_back(ds[c - 1]);
     }
   }
+  // Contracted indices run in L's storage order unless only R has one as its
+  // contiguous index: then in R's order, so the B gathers can run along k.
+  const bool l_kfast = p.lrank > 0 && p.dig[0] < 0, r_kfast = p.rrank > 0 && p.dig[p.lrank] < 0;
+  if (r_kfast && !l_kfast) {
+    std::vector<size_t> ord(kext.size());
+    for (size_t i = 0; i < ord.size(); ++i) ord[i] = i;
+    std::sort(ord.begin(), ord.end(), [&](size_t x, size_t y) { return kr[x] < kr[y]; });
+    std::vector<int> e2;
+    std::vector<long long> l2, r2;
+    for (size_t i : ord) {
+      e2.push_back(kext[i]);
+      l2.push_back(kl[i]);
+      r2.push_back(kr[i]);
+    }
+    kext.swap(e2);
+    kl.swap(l2);
+    kr.swap(r2);
+  }
   std::vector<int> lM, dM, lK, rK, rN, dN;
   odometer(mext, ml, md, &lM, &dM);
   odometer(kext, kl, kr, &lK, &rK);
   odometer(next, nr, nd, &rN, &dN);
+  // gather layouts: along k when k (not m / n) is unit-stride in the operand
+  auto unit = [](const std::vector<int> &v) { return v.size() >= 2 && v[1] - v[0] == 1; };
+  h->t.amode = (unit(lK) && !unit(lM)) ? 1 : 0;
+  h->t.bmode = (unit(rK) && !unit(rN)) ? 1 : 0;
   int dk[3];
   h->t.D = gpu_body(e.d, g, &dk[0]);
   h->t.L = gpu_body(e.l, g, &dk[1]);
@@ -193,13 +217,14 @@ void set_tile(HostTask &h, TileShape s) {
   h.t.tiles_n = (h.t.N + s.tn - 1) / s.tn;
 }
 
-// Device image of a task list: [tasks][prefix][counter][tables].
+// Device image of a task list: [tasks][prefix][counter][item -> task][tables].
 struct DeviceBatch {
   void *base = nullptr;
   int handle = -1;
   BatchTask *tasks = nullptr;
   long long *prefix = nullptr;
   unsigned long long *counter = nullptr;
+  int *item_task = nullptr;
   int *tabs = nullptr;
   long long total = 0;
 };
@@ -216,18 +241,25 @@ int upload(int g, std::vector<HostTask> &hts, DeviceBatch *db, hipStream_t s) {
     prefix[i + 1] = prefix[i] + (long long)tasks[i].tiles_m * tasks[i].tiles_n;
   }
   if (tabs.empty()) tabs.push_back(0);
+  std::vector<int> item_task((size_t)std::max<long long>(1, prefix[n]));
+  for (int i = 0; i < n; ++i)
+    for (long long it = prefix[i]; it < prefix[i + 1]; ++it) item_task[(size_t)it] = i;
   const size_t bt = ((sizeof(BatchTask) * n + 255) / 256) * 256, bp = ((8 * (n + 1) + 255) / 256) * 256;
-  const size_t bytes = bt + bp + 256 + 4 * tabs.size();
+  const size_t bi = ((4 * item_task.size() + 255) / 256) * 256;
+  const size_t bytes = bt + bp + 256 + bi + 4 * tabs.size();
   int e = dev_alloc(g, bytes, &db->base, &db->handle);
   if (e) return e;
   char *b = (char *)db->base;
   db->tasks = (BatchTask *)b;
   db->prefix = (long long *)(b + bt);
   db->counter = (unsigned long long *)(b + bt + bp);
-  db->tabs = (int *)(b + bt + bp + 256);
+  db->item_task = (int *)(b + bt + bp + 256);
+  db->tabs = (int *)(b + bt + bp + 256 + bi);
   db->total = prefix[n];
   if (hipMemcpyAsync(db->tasks, tasks.data(), sizeof(BatchTask) * n, hipMemcpyHostToDevice, s) != hipSuccess ||
       hipMemcpyAsync(db->prefix, prefix.data(), 8 * (n + 1), hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(db->item_task, item_task.data(), 4 * item_task.size(), hipMemcpyHostToDevice, s) !=
+          hipSuccess ||
       hipMemcpyAsync(db->tabs, tabs.data(), 4 * tabs.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess) {
     dev_free(g, db->base, db->handle);
@@ -236,17 +268,21 @@ int upload(int g, std::vector<HostTask> &hts, DeviceBatch *db, hipStream_t s) {
   return TALSH_SUCCESS;
 }
 
-int persistent_grid(int g) {
+// Workgroups of the persistent launch: as many as are resident on the CUs at
+// once (4 per CU: 4 waves per SIMD by registers, 35 KB of LDS each for R8).
+int persistent_grid(int g, int dk) {
   hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, rt().gpus[g].hip_id) != hipSuccess) return 1024;
-  return std::max(1, prop.multiProcessorCount) * 4;
+  const int per_cu = 4;
+  (void)dk;
+  if (hipGetDeviceProperties(&prop, rt().gpus[g].hip_id) != hipSuccess) return 256 * per_cu;
+  return std::max(1, prop.multiProcessorCount) * per_cu;
 }
 
 // Times every candidate tile shape on up to 32 tasks of each untuned class.
 int autotune(int g, int dk, std::vector<HostTask> &hts, hipStream_t s, int *nclasses, double *ms_spent) {
   std::map<int, std::vector<int>> classes;
   for (int i = 0; i < (int)hts.size(); ++i) classes[hts[i].cls].push_back(i);
-  const int grid = persistent_grid(g);
+  const int grid = persistent_grid(g, dk);
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return TALSH_FAILURE;
   const double t0 = wall_time();
@@ -278,7 +314,7 @@ int autotune(int g, int dk, std::vector<HostTask> &hts, hipStream_t s, int *ncla
       float ms = 0;
       for (int rep = 0; rep < 2; ++rep) {  // first launch warms caches
         (void)hipEventRecord(e0, s);
-        batch_launch(dk, db.tasks, db.prefix, (int)probe.size(), db.tabs, db.counter, db.total, grid, s);
+        batch_launch(dk, db.tasks, db.prefix, db.item_task, db.tabs, db.counter, db.total, grid, s);
         (void)hipEventRecord(e1, s);
         (void)hipEventSynchronize(e1);
         (void)hipEventElapsedTime(&ms, e0, e1);
@@ -296,6 +332,75 @@ int autotune(int g, int dk, std::vector<HostTask> &hts, hipStream_t s, int *ncla
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  *ms_spent += 1e3 * (wall_time() - t0);
+  return err;
+}
+
+// Second tuning stage, on the whole batch: a class timed alone favours small
+// tiles (they expose parallelism a 32-task probe lacks), while the full batch
+// has work items to spare and pays a fixed cost per tile.  Each tile policy
+// below is timed on one launch of the entire batch writing scratch outputs;
+// the fastest is applied.  Skipped (policy 0) when the scratch outputs would
+// not fit in a quarter of the free pool.
+enum TilePolicy { POLICY_CLASS_TUNED = 0, POLICY_FIT = 1, POLICY_FIT_MIN32 = 2, POLICY_MAX = 3, NPOLICY = 4 };
+
+TileShape policy_tile(int policy, int dk, const HostTask &h) {
+  const TileShape fit = default_tile(h.t.M, h.t.N);
+  switch (policy) {
+    case POLICY_FIT: return fit;
+    case POLICY_FIT_MIN32: return TileShape{std::max(32, fit.tm), std::max(32, fit.tn)};
+    case POLICY_MAX: return TileShape{64, 64};
+    default: {
+      auto it = tuned_tiles().find(std::make_pair(dk, h.cls));
+      return it != tuned_tiles().end() ? it->second : fit;
+    }
+  }
+}
+
+int choose_policy(int g, int dk, std::vector<HostTask> hts, hipStream_t s, int *policy, double *ms_spent) {
+  *policy = POLICY_CLASS_TUNED;
+  size_t scratch = 0;
+  for (auto &h : hts) scratch += (h.dbytes + 255) / 256 * 256;
+  if (scratch == 0 || scratch > rt().gpus[g].arena.free_bytes() / 4) return TALSH_SUCCESS;
+  void *sp = nullptr;
+  int sh = -1;
+  if (dev_alloc(g, scratch, &sp, &sh) != 0) return TALSH_SUCCESS;
+  size_t off = 0;
+  for (auto &h : hts) {  // scratch outputs: user tensors are never written while tuning
+    h.t.D = (char *)sp + off;
+    h.t.beta_one = 0;
+    off += (h.dbytes + 255) / 256 * 256;
+  }
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    dev_free(g, sp, sh);
+    return TALSH_FAILURE;
+  }
+  const double t0 = wall_time();
+  const int grid = persistent_grid(g, dk);
+  float best = 1e30f;
+  int err = TALSH_SUCCESS;
+  for (int pol = 0; pol < NPOLICY && !err; ++pol) {
+    for (auto &h : hts) set_tile(h, policy_tile(pol, dk, h));
+    DeviceBatch db;
+    if ((err = upload(g, hts, &db, s)) != 0) break;
+    float ms = 0;
+    for (int rep = 0; rep < 2; ++rep) {  // first launch warms caches
+      (void)hipEventRecord(e0, s);
+      batch_launch(dk, db.tasks, db.prefix, db.item_task, db.tabs, db.counter, db.total, grid, s);
+      (void)hipEventRecord(e1, s);
+      (void)hipEventSynchronize(e1);
+      (void)hipEventElapsedTime(&ms, e0, e1);
+    }
+    dev_free(g, db.base, db.handle);
+    if (ms < best) {
+      best = ms;
+      *policy = pol;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  dev_free(g, sp, sh);
   *ms_spent += 1e3 * (wall_time() - t0);
   return err;
 }
@@ -343,10 +448,12 @@ int build(talshAmdBatch *b, int g) {
   Device &dev = rt().gpus[g];
   hipStream_t s = dev.streams[0];
   if (int e = autotune(g, dk, hts, s, &b->tuned_classes, &b->tune_ms)) return e;
+  int policy = POLICY_CLASS_TUNED;
+  if (int e = choose_policy(g, dk, hts, s, &policy, &b->tune_ms)) return e;
+  b->policy = policy;
   b->flops = 0;
   for (auto &h : hts) {
-    auto it = tuned_tiles().find(std::make_pair(dk, h.cls));
-    set_tile(h, it != tuned_tiles().end() ? it->second : default_tile(h.t.M, h.t.N));
+    set_tile(h, policy_tile(policy, dk, h));
     b->flops += 2.0 * h.t.M * (double)h.t.N * h.t.K;
   }
   DeviceBatch db;
@@ -355,11 +462,12 @@ int build(talshAmdBatch *b, int g) {
   b->dev_handle = db.handle;
   b->d_tasks = db.tasks;
   b->d_prefix = db.prefix;
+  b->d_item_task = db.item_task;
   b->d_counter = db.counter;
   b->d_tabs = db.tabs;
   b->total = db.total;
   b->ntasks = (int)hts.size();
-  b->grid = persistent_grid(g);
+  b->grid = persistent_grid(g, dk);
   b->g = g;
   b->dk = dk;
   // capture [clear counter, persistent kernel] once; executions replay it
@@ -367,7 +475,7 @@ int build(talshAmdBatch *b, int g) {
   if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return TALSH_FAILURE;
   int err = TALSH_SUCCESS;
   if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) err = TALSH_FAILURE;
-  if (!err && batch_launch(dk, b->d_tasks, b->d_prefix, b->ntasks, b->d_tabs, b->d_counter, b->total, b->grid, cs))
+  if (!err && batch_launch(dk, b->d_tasks, b->d_prefix, b->d_item_task, b->d_tabs, b->d_counter, b->total, b->grid, cs))
     err = TALSH_FAILURE;
   if (hipStreamEndCapture(cs, &b->graph) != hipSuccess) err = TALSH_FAILURE;
   if (!err && hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess) err = TALSH_FAILURE;
