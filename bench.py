@@ -119,6 +119,70 @@ def bench_contraction(oracle, dk, pat, n, steps, warmup, check=True, seed=100):
     return res
 
 
+def logged_call(fn):
+    """Runs fn with the library's op log on; returns (result, log text)."""
+    import tempfile
+    lib = ts.load()
+    libc = ctypes.CDLL(None)
+    sys.stdout.flush()
+    saved = os.dup(1)
+    with tempfile.TemporaryFile() as tmp:
+        os.dup2(tmp.fileno(), 1)
+        lib.talshTensorOpLogStart()
+        try:
+            res = fn()
+        finally:
+            lib.talshTensorOpLogFinish()
+            libc.fflush(None)
+            os.dup2(saved, 1)
+            os.close(saved)
+        tmp.seek(0)
+        return res, tmp.read().decode(errors="replace")
+
+
+def bench_xl(oracle, n=96, grant=2 << 30, reps=3, seed=300):
+    """talshTensorContractXL of the config-3 contraction with Host-resident operands (pinned host buffer):
+    with the default device-memory grant (one piece) and with `grant` bytes (the operation is cut into
+    pieces).  Each call moves L, R and D to the GPU and D back, so the bound is the host link."""
+    import re
+    pat = "D(a,b,c,d)+=L(a,i,c,j)*R(j,b,i,d)"
+    dims = [n] * 4
+    size = int(np.prod(dims))
+    L, R, D0 = uniform(seed, size), uniform(seed + 1, size), uniform(seed + 2, size)
+    l, r, d = (ts.Tensor(R8, dims, 0, data=x) for x in (L, R, D0))
+    flop = 2.0 * float(n) ** 6
+    out = {"flop_per_call": flop, "operand_bytes": 8 * size}
+    for label, g in (("one_piece", 0), ("pieces", grant)):
+        ts.set_xl_memory(g)
+        log("xl", label, "grant", g)
+        (err, text) = logged_call(lambda: ts.contract_xl(pat, d, l, r, alpha=1.0))
+        ts.check(err, "contract_xl")
+        m = re.findall(r"(\d+) pieces \((\d+) D boxes x (\d+) contracted tiles\)", text)
+        times = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            ts.check(ts.contract_xl(pat, d, l, r, alpha=1.0), "contract_xl")
+            times.append(time.perf_counter() - t0)
+        t = float(np.median(times))
+        res = {"ms": 1e3 * t, "tflops": flop / t / 1e12}
+        if m:
+            pc, nb, nk = (int(x) for x in m[-1])
+            res.update({"pieces": pc, "d_boxes": nb, "k_tiles": nk})
+        mv = re.findall(r"talshTensorContractXL done: \S+ flop, (\S+) bytes in, (\S+) bytes out", text)
+        if mv:
+            bi, bo = (float(x) for x in mv[-1])
+            res.update({"link_bytes": bi + bo, "link_gbs": (bi + bo) / t / 1e9})
+        out[label] = res
+        log("xl", label, res)
+    ts.set_xl_memory(0)
+    # D = D0 + (2 + 2 * reps) * L*R after the calls above
+    calls = 2 * (1 + reps)
+    out["rel_err_vs_fp64"] = sampled_error(oracle, R8, pat, dims, D0, L, R, d.host_array(), float(calls))
+    for x in (l, r, d):
+        x.destruct()
+    return out
+
+
 def bench_transposes(oracle, steps):
     """Config 2: rank-6 R8 16^6, identity + reversal + 32 seeded permutations; add; norm2."""
     import random
@@ -581,7 +645,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--only", default="",
-                    help="comma list of parts to run (head,r4,bf16,c8,config1,transpose,batch,serve)")
+                    help="comma list of parts to run (head,r4,bf16,c8,config1,transpose,batch,xl,serve)")
     ap.add_argument("--serve-requests", type=int, default=4000, help="serving stream length (full: 20000)")
     ap.add_argument("--serve-rate", type=float, default=None, help="arrivals per second (default 0.8 x capacity)")
     ap.add_argument("--serve-streams", type=int, default=16, help="streams per GPU for the serving run")
@@ -616,6 +680,8 @@ def main():
     if want("batch"):
         secondary["batched_config4_r8"] = bench_batch(oracle, R8)
         secondary["batched_config4_r4"] = bench_batch(oracle, R4)
+    if want("xl"):
+        secondary["xl_config3_host_operands"] = bench_xl(oracle)
     ts.shutdown()
     if want("serve"):
         # 200 GB admission budget: the device pool holds it plus the resident templates

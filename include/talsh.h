@@ -259,6 +259,15 @@ int talshTensorContract(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *lt
 int talshTensorContract_(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, talsh_tens_t *rtens,
                          double scale_real, double scale_imag, int dev_id, int dev_kind, int copy_ctrl,
                          int accumulative, talsh_task_t *talsh_task);
+/* Extra-large contraction (talsh.h:531-542): blocking; the operation is cut into pieces that fit the GPU's
+ * device buffer (or the limit set by talshAmdSetContractXLMemory), operands may live on the host; on return
+ * dtens holds one image, on the host.  DEV_HOST execution returns TALSH_NOT_AVAILABLE. */
+int talshTensorContractXL(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, talsh_tens_t *rtens,
+                          double scale_real TALSH_DEFAULT(1.0), double scale_imag TALSH_DEFAULT(0.0),
+                          int dev_id TALSH_DEFAULT(DEV_DEFAULT), int dev_kind TALSH_DEFAULT(DEV_DEFAULT),
+                          int accumulative TALSH_DEFAULT(YEP));
+int talshTensorContractXL_(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, talsh_tens_t *rtens,
+                           double scale_real, double scale_imag, int dev_id, int dev_kind, int accumulative);
 /* Squared 2-norm into a rank-0 tensor dtens (TALSH_manual.txt:893-913; see the header comment). */
 int talshTensorNorm2(talsh_tens_t *dtens, talsh_tens_t *ltens, int dev_id TALSH_DEFAULT(DEV_DEFAULT),
                      int dev_kind TALSH_DEFAULT(DEV_DEFAULT), int copy_ctrl TALSH_DEFAULT(COPY_TT),

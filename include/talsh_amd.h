@@ -29,6 +29,11 @@ int talshAmdSetDeviceBufferSize(size_t bytes);
  * Must be called before talshInit. */
 int talshAmdSetNumStreams(int n);
 
+/* Device memory (bytes) one talshTensorContractXL call may use for its pieces
+ * (two are in flight); 0 (default) = 90 % of the largest free range of the
+ * device buffer.  May be called at any time. */
+int talshAmdSetContractXLMemory(size_t bytes);
+
 /* Blocks until all work queued on GPU `gpu` (kind-specific number) is done. */
 int talshAmdDeviceSynchronize(int gpu);
 

@@ -123,6 +123,8 @@ SIGNATURES = {
     "talshTensorCopy": (_I, [ctypes.c_char_p, _TP, _TP, _I, _I, _I, _KP]),
     "talshTensorAdd": (_I, [ctypes.c_char_p, _TP, _TP, _D, _D, _I, _I, _I, _KP]),
     "talshTensorContract": (_I, [ctypes.c_char_p, _TP, _TP, _TP, _D, _D, _I, _I, _I, _I, _KP]),
+    "talshTensorContractXL": (_I, [ctypes.c_char_p, _TP, _TP, _TP, _D, _D, _I, _I, _I]),
+    "talshTensorContractXL_": (_I, [ctypes.c_char_p, _TP, _TP, _TP, _D, _D, _I, _I, _I]),
     "talshTensorNorm2": (_I, [_TP, _TP, _I, _I, _I, _KP]),
     "talshTensorSlice": (_I, [_TP, _TP, _IP, _I, _I, _I, _I, _KP]),
     "talshTensorInsert": (_I, [_TP, _TP, _IP, _I, _I, _I, _I, _KP]),
@@ -135,6 +137,7 @@ SIGNATURES = {
     "talshQueryFastMath": (_I, [_I, _I]),
     "talshAmdSetDeviceBufferSize": (_I, [_S]),
     "talshAmdSetNumStreams": (_I, [_I]),
+    "talshAmdSetContractXLMemory": (_I, [_S]),
     "talshAmdDeviceSynchronize": (_I, [_I]),
     "talshAmdBuildInfo": (_I, [ctypes.c_char_p, _I]),
     "talshAmdBatchCreate": (_I, [ctypes.POINTER(_P)]),
@@ -405,6 +408,18 @@ def contract(pattern, d, l, r, alpha=1.0, dev_id=DEV_DEFAULT, dev_kind=DEV_DEFAU
     a = complex(alpha)
     return load().talshTensorContract(pattern.encode(), d.ptr, l.ptr, r.ptr, a.real, a.imag, dev_id, dev_kind,
                                       copy_ctrl, accumulative, task.ptr if task else None)
+
+
+def contract_xl(pattern, d, l, r, alpha=1.0, dev_id=DEV_DEFAULT, dev_kind=DEV_DEFAULT, accumulative=YEP):
+    """talshTensorContractXL: blocking; d ends with one image, on the Host."""
+    a = complex(alpha)
+    return load().talshTensorContractXL(pattern.encode(), d.ptr, l.ptr, r.ptr, a.real, a.imag, dev_id, dev_kind,
+                                        accumulative)
+
+
+def set_xl_memory(nbytes):
+    """talshAmdSetContractXLMemory: device bytes an XL contraction may use (0 = from the free pool)."""
+    return load().talshAmdSetContractXLMemory(nbytes)
 
 
 def slice(d, l, offsets, dev_id=DEV_DEFAULT, dev_kind=DEV_DEFAULT, copy_ctrl=None, accumulative=NOPE, task=None):

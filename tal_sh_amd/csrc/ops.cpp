@@ -16,6 +16,7 @@
 #include <cstring>
 #include <string>
 
+#include "contract.hpp"
 #include "plan.hpp"
 #include "runtime.hpp"
 #include "talsh_amd.h"
@@ -139,10 +140,10 @@ class Op {
   hipStream_t stream() const { return s_; }
   Task *task() { return task_; }
 
-  void mark_gemm(bool start) {
-    EvRef &e = start ? task_->ev_mm0 : task_->ev_mm1;
-    e = new_event(g_);
-    (void)hipEventRecord(e->ev, s_);
+  // Events that bracket the GEMM (talshTaskTime's mmul); the enqueuer records them.
+  void gemm_events() {
+    task_->ev_mm0 = new_event(g_);
+    task_->ev_mm1 = new_event(g_);
   }
 
   // Ends the task: output transfers, completion event, coherence bookkeeping.
@@ -286,6 +287,8 @@ class Op {
   int data_kind_ = NO_TYPE, coh_ctrl_ = 0;
 };
 
+}  // namespace
+
 // Execution GPU for (dev_id, dev_kind); operands are consulted when the
 // caller leaves the choice to the runtime.  Host execution is not provided.
 int exec_gpu(int dev_id, int dev_kind, std::initializer_list<const talsh_tens_t *> tens, int *g) {
@@ -327,6 +330,8 @@ int exec_gpu(int dev_id, int dev_kind, std::initializer_list<const talsh_tens_t 
   return TALSH_SUCCESS;
 }
 
+namespace {
+
 int check_task(talsh_task_t *tt) {
   if (tt && talshTaskIsEmpty(tt) != YEP) return TALSH_OBJECT_NOT_EMPTY;
   return TALSH_SUCCESS;
@@ -342,6 +347,63 @@ std::vector<int> dims_of(const talsh_tens_t *t) {
 }
 
 }  // namespace
+
+TtgtTemps ttgt_temp_bytes(const ContractPlan &plan, int dk, size_t lbytes, size_t rbytes, size_t dbytes) {
+  TtgtTemps t;
+  if (plan.a.transpose) t.a = plan.swap ? rbytes : lbytes;
+  if (plan.b.transpose) t.b = plan.swap ? lbytes : rbytes;
+  if (plan.d_transpose) t.d = dbytes;
+  if (dk == R4 && rt().r4mode == R4_SPLIT_BF16) t.w = gemm_split_workspace(plan.M, plan.N, plan.K);
+  return t;
+}
+
+int enqueue_ttgt(const DigitalPattern &p, const ContractPlan &plan, int dk, bool acc, double ar, double ai,
+                 void *D, const void *L, const void *R, const int *ddims, const int *ldims, const int *rdims,
+                 const TtgtBuffers &tmp, hipStream_t s, hipEvent_t gemm_start, hipEvent_t gemm_end) {
+  const bool cplx = (dk == C4 || dk == C8);
+  const void *X = plan.swap ? R : L, *Y = plan.swap ? L : R;
+  const int *xd = plan.swap ? rdims : ldims, *yd = plan.swap ? ldims : rdims;
+  const int xr = plan.swap ? p.rrank : p.lrank, yr = plan.swap ? p.lrank : p.rrank;
+  const void *A = X, *B = Y;
+  void *C = D;
+  if (plan.a.transpose) {
+    if (!tmp.a || launch_permute(dk, X, tmp.a, xr, xd, plan.a.o2n.data(), PERM_COPY, 1, 0, 0, s))
+      return TALSH_FAILURE;
+    A = tmp.a;
+  }
+  if (plan.b.transpose) {
+    if (!tmp.b || launch_permute(dk, Y, tmp.b, yr, yd, plan.b.o2n.data(), PERM_COPY, 1, 0, 0, s))
+      return TALSH_FAILURE;
+    B = tmp.b;
+  }
+  if (plan.d_transpose) {
+    if (!tmp.d) return TALSH_FAILURE;
+    C = tmp.d;
+    if (acc && launch_permute(dk, D, C, p.drank, ddims, plan.d_o2n.data(), PERM_COPY, 1, 0, 0, s))
+      return TALSH_FAILURE;
+  }
+  int lconj = (p.conj >> 1) & 1, rconj = (p.conj >> 2) & 1;
+  if (p.conj & 1) { lconj ^= 1; rconj ^= 1; }
+  if (!cplx) lconj = rconj = 0;
+  if (gemm_start) (void)hipEventRecord(gemm_start, s);
+  const bool split = (dk == R4 && rt().r4mode == R4_SPLIT_BF16);
+  if (split) {
+    if (!tmp.w || launch_gemm_split(plan.a.op, plan.b.op, plan.M, plan.N, plan.K, ar, A, plan.a.ld, B, plan.b.ld,
+                                    acc ? 1 : 0, C, std::max<int64_t>(1, plan.M), tmp.w, s))
+      return TALSH_FAILURE;
+  } else if (launch_gemm(dk, R4_FP32, plan.a.op, plan.b.op, plan.swap ? rconj : lconj, plan.swap ? lconj : rconj,
+                         plan.M, plan.N, plan.K, ar, cplx ? ai : 0.0, A, plan.a.ld, B, plan.b.ld, acc ? 1 : 0, C,
+                         std::max<int64_t>(1, plan.M), s)) {
+    return TALSH_FAILURE;
+  }
+  if (gemm_end) (void)hipEventRecord(gemm_end, s);
+  if (plan.d_transpose &&
+      launch_permute(dk, C, D, (int)plan.dprime_dims.size(), plan.dprime_dims.data(), plan.d_back_o2n.data(),
+                     PERM_COPY, 1, 0, 0, s))
+    return TALSH_FAILURE;
+  return TALSH_SUCCESS;
+}
+
 }  // namespace tal
 
 using namespace tal;
@@ -594,62 +656,26 @@ int talshTensorContract(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *lt
   const int dk = op.operand(0).dk;
   if (op.operand(1).dk != dk || op.operand(2).dk != dk) return TALSH_INVALID_ARGS;
   const bool cplx = (dk == C4 || dk == C8);
-  // temporaries: [A'], [B'], [D']
+  // temporaries: [A'], [B'], [D'], [split-BF16 planes]
+  const TtgtTemps tb = ttgt_temp_bytes(plan, dk, op.operand(1).bytes, op.operand(2).bytes, op.operand(0).bytes);
   std::vector<size_t> temps;
-  const int xi = plan.swap ? 2 : 1, yi = plan.swap ? 1 : 2;
-  int ta = -1, tb = -1, td = -1;
-  if (plan.a.transpose) { ta = (int)temps.size(); temps.push_back(op.operand(xi).bytes); }
-  if (plan.b.transpose) { tb = (int)temps.size(); temps.push_back(op.operand(yi).bytes); }
-  if (plan.d_transpose) { td = (int)temps.size(); temps.push_back(op.operand(0).bytes); }
-  // split-BF16 R4: workspace for the BF16 planes of both GEMM operands
-  const bool split = (dk == R4 && rt().r4mode == R4_SPLIT_BF16);
-  int tw = -1;
-  if (split) { tw = (int)temps.size(); temps.push_back(gemm_split_workspace(plan.M, plan.N, plan.K)); }
+  int ia = -1, ib = -1, id = -1, iw = -1;
+  if (tb.a) { ia = (int)temps.size(); temps.push_back(tb.a); }
+  if (tb.b) { ib = (int)temps.size(); temps.push_back(tb.b); }
+  if (tb.d) { id = (int)temps.size(); temps.push_back(tb.d); }
+  if (tb.w) { iw = (int)temps.size(); temps.push_back(tb.w); }
   if (int e = op.acquire(temps)) return deferred(g, e);
   if (int e = op.begin(dk, copy_ctrl)) return e;
-  hipStream_t s = op.stream();
-  Operand &D = op.operand(0), &X = op.operand(xi), &Y = op.operand(yi);
-  const void *A = X.dptr, *B = Y.dptr;
-  void *C = D.dptr;
-  if (ta >= 0) {
-    std::vector<int> xd = dims_of(X.t);
-    if (launch_permute(dk, X.dptr, op.temp(ta), (int)xd.size(), xd.data(), plan.a.o2n.data(), PERM_COPY, 1, 0, 0, s))
-      return TALSH_FAILURE;
-    A = op.temp(ta);
-  }
-  if (tb >= 0) {
-    std::vector<int> yd = dims_of(Y.t);
-    if (launch_permute(dk, Y.dptr, op.temp(tb), (int)yd.size(), yd.data(), plan.b.o2n.data(), PERM_COPY, 1, 0, 0, s))
-      return TALSH_FAILURE;
-    B = op.temp(tb);
-  }
-  if (td >= 0) {
-    C = op.temp(td);
-    if (acc) {
-      std::vector<int> dd = dims_of(dtens);
-      if (launch_permute(dk, D.dptr, C, (int)dd.size(), dd.data(), plan.d_o2n.data(), PERM_COPY, 1, 0, 0, s))
-        return TALSH_FAILURE;
-    }
-  }
-  int lconj = (p.conj >> 1) & 1, rconj = (p.conj >> 2) & 1;
-  if (p.conj & 1) { lconj ^= 1; rconj ^= 1; }
-  if (!cplx) lconj = rconj = 0;
-  op.mark_gemm(true);
-  if (split) {
-    if (launch_gemm_split(plan.a.op, plan.b.op, plan.M, plan.N, plan.K, scale_real, A, plan.a.ld, B, plan.b.ld,
-                          acc ? 1 : 0, C, std::max<int64_t>(1, plan.M), op.temp(tw), s))
-      return TALSH_FAILURE;
-  } else if (launch_gemm(dk, R4_FP32, plan.a.op, plan.b.op, plan.swap ? rconj : lconj, plan.swap ? lconj : rconj,
-                         plan.M, plan.N, plan.K, scale_real, cplx ? scale_imag : 0.0, A, plan.a.ld, B, plan.b.ld,
-                         acc ? 1 : 0, C, std::max<int64_t>(1, plan.M), s)) {
-    return TALSH_FAILURE;
-  }
-  op.mark_gemm(false);
-  if (td >= 0) {
-    if (launch_permute(dk, C, D.dptr, (int)plan.dprime_dims.size(), plan.dprime_dims.data(),
-                       plan.d_back_o2n.data(), PERM_COPY, 1, 0, 0, s))
-      return TALSH_FAILURE;
-  }
+  TtgtBuffers bufs;
+  bufs.a = ia >= 0 ? op.temp(ia) : nullptr;
+  bufs.b = ib >= 0 ? op.temp(ib) : nullptr;
+  bufs.d = id >= 0 ? op.temp(id) : nullptr;
+  bufs.w = iw >= 0 ? op.temp(iw) : nullptr;
+  op.gemm_events();
+  if (int e = enqueue_ttgt(p, plan, dk, acc, scale_real, scale_imag, op.operand(0).dptr, op.operand(1).dptr,
+                           op.operand(2).dptr, dtens->shape_p->dims, ltens->shape_p->dims, rtens->shape_p->dims,
+                           bufs, op.stream(), op.task()->ev_mm0->ev, op.task()->ev_mm1->ev))
+    return e;
   const double flops = (cplx ? 8.0 : 2.0) * (double)plan.M * (double)plan.N * (double)plan.K;
   if (int e = op.end(flops)) return e;
   return op.publish(3);

@@ -706,6 +706,8 @@ int launch_slice(int dk, int mode, const void *src, void *dst, int rank, const i
   if (rank < 0 || rank > kMaxRank || (rank > 0 && (!sdims || !bdims || !offsets)) || !src || !dst ||
       (mode != 0 && mode != 1))
     return -1;
+  for (int k = 0; k < rank; ++k)
+    if (sdims[k] < 1 || offsets[k] < 0 || bdims[k] < 1 || (int64_t)offsets[k] + sdims[k] > bdims[k]) return -1;
   switch (dk) {
     case 4: return launch_slice_typed(mode, (const float *)src, (float *)dst, rank, sdims, bdims, offsets, accumulative, s);
     case 8: return launch_slice_typed(mode, (const double *)src, (double *)dst, rank, sdims, bdims, offsets, accumulative, s);

@@ -61,6 +61,7 @@ struct Runtime {
   int r4mode = R4_FP32;
   size_t dev_buffer_request = 0;  // 0: take a fraction of the free memory
   int nstreams = 4;
+  size_t xl_mem = 0;  // device bytes a talshTensorContractXL call may use (0: from the free pool)
   bool log_mem = false, log_ops = false;  // talshMemManagerLogStart / talshTensorOpLogStart
   std::unordered_map<const void *, Hazard> hazards;
   // bumped by every change of the hazard table: a caller that saw the same

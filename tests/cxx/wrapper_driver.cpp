@@ -4,13 +4,19 @@
 //   1. contractAccumulate with constant tensors against the closed form
 //      d0 + alpha * l * r * K (the reference's own check, test.cpp:193-226);
 //   2. extractSlice of a tensor whose values are their own dlf positions;
-//   3. insertSlice of a constant block, everything outside it unchanged.
+//   3. insertSlice of a constant block, everything outside it unchanged;
+//   4. contractAccumulateXL with complex constant tensors, cut into pieces by a
+//      small device-memory grant, against the closed form of test.cpp:326-371,
+//      and the full reduction into a scalar.
 // Prints "cxx wrapper driver: OK" and exits 0 when every check holds.
 #include "talshxx.hpp"
 
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <vector>
+
+#include "talsh_amd.h"
 
 static int fails = 0;
 static void expect(bool ok, const char *what) {
@@ -72,6 +78,33 @@ int main() {
           if (b[lin] != (in ? -1.0 : (double)lin)) ok = false;
         }
     expect(ok, "insert wrote exactly the block");
+  }
+  {
+    const int o = 8, v = 16;
+    const std::vector<int> dims{o, v, o, v};
+    const float d_init = 1e-4f;
+    talsh::Tensor r(dims, std::complex<float>{0.001f, 0.0f}), l(dims, std::complex<float>{0.01f, 0.0f});
+    talsh::Tensor d(dims, std::complex<float>{d_init, 0.0f}), s(std::vector<int>{}, std::complex<float>{0.0f, 0.0f});
+    expect(talshAmdSetContractXLMemory(64 << 10) == TALSH_SUCCESS, "XL memory grant");
+    const int e = d.contractAccumulateXL(nullptr, "D(i,a,j,b)+=L(j,a,k,c)*R(k,b,i,c)", l, r, DEV_NVIDIA_GPU, 0,
+                                         std::complex<float>{0.5f, 0.0f});
+    expect(e == TALSH_SUCCESS, "contractAccumulateXL returned an error");
+    expect(d.sync(DEV_HOST, 0), "sync of the XL result");
+    const std::complex<float> *p = nullptr;
+    expect(d.getDataAccessHostConst(&p), "host access to the XL result");
+    const double want = (double)(o * v) * 1e-3 * 1e-2 * 0.5 + d_init;
+    double worst = 0.0;
+    for (std::size_t i = 0; p && i < (std::size_t)o * v * o * v; ++i) worst = std::fmax(worst, std::abs(p[i] - std::complex<float>((float)want, 0.0f)));
+    expect(p && worst <= 1e-6 * want, "XL contraction closed form");
+    const int e2 = s.contractAccumulateXL(nullptr, "D()+=L(i,a,j,b)*R(j,b,i,a)", l, r, DEV_NVIDIA_GPU, 0,
+                                          std::complex<float>{1.0f, 0.0f});
+    expect(e2 == TALSH_SUCCESS, "contractAccumulateXL into a scalar returned an error");
+    expect(s.sync(DEV_HOST, 0), "sync of the XL scalar");
+    const std::complex<float> *q = nullptr;
+    expect(s.getDataAccessHostConst(&q), "host access to the XL scalar");
+    const double sw = (double)(o * v) * (o * v) * 1e-3 * 1e-2;
+    expect(q && std::abs(q[0] - std::complex<float>((float)sw, 0.0f)) <= 1e-5 * sw, "XL scalar closed form");
+    talshAmdSetContractXLMemory(0);
   }
   talsh::shutdown();
   std::printf("cxx wrapper driver: %s\n", fails ? "FAIL" : "OK");

@@ -3,8 +3,10 @@
 oracle/_ref/cxx_wrapper_driver is tests/cxx/wrapper_driver.cpp linked with the
 reference's talshxx.cpp / talsh_task.cpp compiled unchanged against include/
 (oracle/Makefile target cxx, built in the development container).  It runs a
-contraction through talsh::Tensor::contractAccumulate against its closed form
-and a slice / insert round trip through extractSlice / insertSlice."""
+contraction through talsh::Tensor::contractAccumulate against its closed form,
+a slice / insert round trip through extractSlice / insertSlice, and
+contractAccumulateXL (cut into pieces) against the closed forms of the
+reference's own XL test."""
 import os
 import subprocess
 
