@@ -43,7 +43,8 @@ int talsh_get_contr_ptrn_str2dig(const char *c_str, int *dig_ptrn, int *drank, i
                                  int *conj_bits);
 
 /* The TTGT plan talshTensorContract would use for a pattern and shapes:
- * mnk = {M, N, K}; flags = {A is R, A transposed, B transposed, D transposed};
+ * mnk = {M, N, K} (of one GEMM; a hyper-contraction runs one per combination
+ * of its hyper-index values); flags = {A is R, A transposed, B transposed, D transposed};
  * ops = GEMM forms of A and B ('N'/'T'); perms = O2N maps of the transposed
  * operands (A, then B, then D->D'; -1 where not transposed); moved = elements
  * copied by the transposes.  Host-only query, no device work. */

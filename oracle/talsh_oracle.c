@@ -470,11 +470,15 @@ int orc_contract(int dk, const int *cptrn, int conj_bits, int drank, const int *
   return 0;
 }
 
-/* One element of D' = sum over contracted multi-index, as a direct loop nest
- * over the digital pattern (independent of the matricization above). */
-int orc_contract_element(int dk, const int *cptrn, int conj_bits, int drank, const int *ddims, size_t doff,
-                         int lrank, const int *ldims, const void *l, int rrank, const int *rdims, const void *r,
-                         double *out_re, double *out_im) {
+/* One element of D = sum over the contracted multi-index, as a direct loop
+ * nest over the digital pattern (independent of the matricization above).
+ * An index that D shares with both L and R (hyper index) simply fixes that
+ * position in both inputs: this is the definition the reference's CPU path
+ * evaluates with a batched GEMM (tensor_algebra_cpu.F90:5757-5956,
+ * :3983-3985). */
+static void element_sum(int dk, const int *cptrn, int conj_bits, int drank, const int *ddims, size_t doff, int lrank,
+                        const int *ldims, const void *l, int rrank, const int *rdims, const void *r,
+                        long double *out_re, long double *out_im) {
   size_t dix[ORC_MAX_RANK];
   int cplx = (dk == ORC_C4 || dk == ORC_C8);
   int lc, rc;
@@ -528,8 +532,49 @@ int orc_contract_element(int dk, const int *cptrn, int conj_bits, int drank, con
       cnt[q] = 0;
     }
   }
+  *out_re = sre;
+  *out_im = sim;
+}
+
+int orc_contract_element(int dk, const int *cptrn, int conj_bits, int drank, const int *ddims, size_t doff,
+                         int lrank, const int *ldims, const void *l, int rrank, const int *rdims, const void *r,
+                         double *out_re, double *out_im) {
+  long double sre, sim;
+  element_sum(dk, cptrn, conj_bits, drank, ddims, doff, lrank, ldims, l, rrank, rdims, r, &sre, &sim);
   *out_re = (double)sre;
   *out_im = (double)sim;
+  return 0;
+}
+
+/* Whole contraction by the direct loop nest, hyper indices included:
+ * D = [D] + alpha * sum, each element accumulated in long double and rounded
+ * once, as orc_contract does. */
+int orc_contract_direct(int dk, const int *cptrn, int conj_bits, int drank, const int *ddims, void *d, int lrank,
+                        const int *ldims, const void *l, int rrank, const int *rdims, const void *r, double ar,
+                        double ai, int accumulative) {
+  if (dk_size(dk) == 0) return 1;
+  size_t vd = volume(drank, ddims);
+#pragma omp parallel for schedule(static)
+  for (size_t o = 0; o < vd; ++o) {
+    long double sre, sim;
+    element_sum(dk, cptrn, conj_bits, drank, ddims, o, lrank, ldims, l, rrank, rdims, r, &sre, &sim);
+    long double vre = (long double)ar * sre - (long double)ai * sim;
+    long double vim = (long double)ar * sim + (long double)ai * sre;
+    switch (dk) {
+      case ORC_R4: { float *D = d; D[o] = (float)((accumulative ? (long double)D[o] : 0.0L) + vre); } break;
+      case ORC_R8: { double *D = d; D[o] = (double)((accumulative ? (long double)D[o] : 0.0L) + vre); } break;
+      case ORC_C4: {
+        float *D = d;
+        D[2 * o] = (float)((accumulative ? (long double)D[2 * o] : 0.0L) + vre);
+        D[2 * o + 1] = (float)((accumulative ? (long double)D[2 * o + 1] : 0.0L) + vim);
+      } break;
+      case ORC_C8: {
+        double *D = d;
+        D[2 * o] = (double)((accumulative ? (long double)D[2 * o] : 0.0L) + vre);
+        D[2 * o + 1] = (double)((accumulative ? (long double)D[2 * o + 1] : 0.0L) + vim);
+      } break;
+    }
+  }
   return 0;
 }
 

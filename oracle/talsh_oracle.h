@@ -18,6 +18,10 @@
  *   orc_add                reference adds, bit-exact
  *   orc_contract           closed forms on constant tensors, brute-force
  *                          loop nests, and the reference contraction
+ *   orc_contract_element / numpy einsum (hyper indices, conjugation, all
+ *   orc_contract_direct    kinds), orc_contract on ordinary patterns, and the
+ *                          hyper-sum invariant (summing a hyper result over
+ *                          its hyper index = contracting that index)
  *   orc_norm2              exact sums of small integers + reference norm2
  *   orc_init / orc_scale   reference init/scale, bit-exact
  *   orc_slice / orc_insert closed-form index map (values that encode their
@@ -56,6 +60,11 @@ int orc_contract(int dk, const int *cptrn, int conj_bits, int drank, const int *
 int orc_contract_element(int dk, const int *cptrn, int conj_bits, int drank, const int *ddims, size_t doff,
                          int lrank, const int *ldims, const void *l, int rrank, const int *rdims, const void *r,
                          double *out_re, double *out_im);
+/* Whole contraction by the same direct loop nest (hyper indices allowed);
+ * D = [D] + alpha * sum, long-double accumulation, one rounding. */
+int orc_contract_direct(int dk, const int *cptrn, int conj_bits, int drank, const int *ddims, void *d, int lrank,
+                        const int *ldims, const void *l, int rrank, const int *rdims, const void *r, double ar,
+                        double ai, int accumulative);
 
 /* Squared Frobenius norm, long double accumulation. */
 double orc_norm2(int dk, size_t vol, const void *p);

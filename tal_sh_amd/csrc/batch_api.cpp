@@ -136,8 +136,8 @@ int plan_task(const talshAmdBatch::Entry &e, int g, HostTask *h) {
     return DEVICE_UNABLE;  // 32-bit offset tables: large contractions belong to the TTGT path
   ContractPlan cp;  // validates the pattern against the shapes
   const int pe = plan_contraction(p, d->shape_p->dims, l->shape_p->dims, r->shape_p->dims, e.acc, &cp);
-  if (pe == 2) return DEVICE_UNABLE;
   if (pe) return TALSH_INVALID_ARGS;
+  if (cp.batch > 1) return DEVICE_UNABLE;  // hyper indices: the TTGT path batches them
   const std::vector<long long> ls = strides(l), rs = strides(r), ds = strides(d);
   std::vector<int> mext, next, kext;
   std::vector<long long> ml, md, nr, nd, kl, kr;

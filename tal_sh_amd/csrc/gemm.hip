@@ -95,8 +95,13 @@ template <typename S, int NC, int OPA, int OPB, int CJA, int CJB, int BM, int BN
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void gemm_kernel(int64_t M, int64_t N, int64_t K, const S *__restrict__ A,
                                                         int64_t lda, const S *__restrict__ B, int64_t ldb,
                                                         S *__restrict__ C, int64_t ldc, S alpha_r, S alpha_i,
-                                                        int beta_one, int64_t gx, int64_t gy) {
+                                                        int beta_one, int64_t gx, int64_t gy, int64_t bsa,
+                                                        int64_t bsb, int64_t bsc) {
   using MF = Mfma<S>;
+  // batch entry blockIdx.y: its operands start bsa, bsb, bsc scalars further on
+  A += (int64_t)blockIdx.y * bsa;
+  B += (int64_t)blockIdx.y * bsb;
+  C += (int64_t)blockIdx.y * bsc;
   using acc_t = typename MF::acc_t;
   constexpr int SM = BM / 32, SN = BN / 32;  // 16x16 tiles per wave along M, N
   constexpr int PAD = W;                     // keeps rows 16-byte aligned
@@ -651,31 +656,40 @@ int launch_bf16x6(int64_t M, int64_t N, int64_t K, const void *A, int64_t lda, c
 
 template <typename S, int NC, int OPA, int OPB, int CJA, int CJB, int BM, int BN>
 int launch_typed(int64_t M, int64_t N, int64_t K, const void *A, int64_t lda, const void *B, int64_t ldb,
-                 void *C, int64_t ldc, double ar, double ai, int beta_one, hipStream_t s) {
+                 void *C, int64_t ldc, double ar, double ai, int beta_one, int64_t batch, int64_t sa, int64_t sb,
+                 int64_t sc, hipStream_t s) {
   const int64_t gx = (M + BM - 1) / BM, gy = (N + BN - 1) / BN;
   const int64_t nblk = gx * gy;
   if (nblk > 0x7fffffffLL) return -4;
-  // 16-byte accesses need every contiguous run (and every base) to split into whole vectors
+  // 16-byte accesses need every contiguous run (and every base, every batch entry) to split into whole vectors
   constexpr int WV = (int)(16 / sizeof(S));
   const int64_t acont = (OPA == 0 ? M : K) * NC, bcont = (OPB == 0 ? K : N) * NC;
   const bool vec = ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0) && acont % WV == 0 &&
-                   bcont % WV == 0 && (lda * NC) % WV == 0 && (ldb * NC) % WV == 0;
-  if (vec)
-    gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, WV><<<dim3((unsigned)nblk), dim3(kThreads), 0, s>>>(
-        M, N, K, (const S *)A, lda, (const S *)B, ldb, (S *)C, ldc, (S)ar, (S)ai, beta_one, gx, gy);
-  else
-    gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, 1><<<dim3((unsigned)nblk), dim3(kThreads), 0, s>>>(
-        M, N, K, (const S *)A, lda, (const S *)B, ldb, (S *)C, ldc, (S)ar, (S)ai, beta_one, gx, gy);
+                   bcont % WV == 0 && (lda * NC) % WV == 0 && (ldb * NC) % WV == 0 &&
+                   (batch == 1 || ((sa * NC) % WV == 0 && (sb * NC) % WV == 0));
+  // batch entries go to grid.y, at most 65535 per launch
+  for (int64_t b0 = 0; b0 < batch; b0 += 65535) {
+    const unsigned nb = (unsigned)std::min<int64_t>(65535, batch - b0);
+    const S *a = (const S *)A + b0 * sa * NC, *b = (const S *)B + b0 * sb * NC;
+    S *c = (S *)C + b0 * sc * NC;
+    if (vec)
+      gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, WV><<<dim3((unsigned)nblk, nb), dim3(kThreads), 0, s>>>(
+          M, N, K, a, lda, b, ldb, c, ldc, (S)ar, (S)ai, beta_one, gx, gy, sa * NC, sb * NC, sc * NC);
+    else
+      gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, 1><<<dim3((unsigned)nblk, nb), dim3(kThreads), 0, s>>>(
+          M, N, K, a, lda, b, ldb, c, ldc, (S)ar, (S)ai, beta_one, gx, gy, sa * NC, sb * NC, sc * NC);
+  }
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 template <typename S, int NC, int BM, int BN>
 int dispatch_ops(int opa, int opb, int cja, int cjb, int64_t M, int64_t N, int64_t K, const void *A, int64_t lda,
                  const void *B, int64_t ldb, void *C, int64_t ldc, double ar, double ai, int beta_one,
-                 hipStream_t s) {
+                 int64_t batch, int64_t sa, int64_t sb, int64_t sc, hipStream_t s) {
 #define TAL_GEMM_CASE(a, b, ca, cb)                                                                      \
   if (opa == a && opb == b && cja == ca && cjb == cb)                                                    \
-    return launch_typed<S, NC, a, b, ca, cb, BM, BN>(M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, s);
+    return launch_typed<S, NC, a, b, ca, cb, BM, BN>(M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, \
+                                                     sa, sb, sc, s);
 #define TAL_GEMM_LAYOUTS(ca, cb) \
   TAL_GEMM_CASE(0, 0, ca, cb)    \
   TAL_GEMM_CASE(0, 1, ca, cb)    \
@@ -736,26 +750,34 @@ int launch_gemm_split(char ta, char tb, int64_t M, int64_t N, int64_t K, double 
 
 int launch_gemm(int dk, int r4mode, char ta, char tb, int conja, int conjb, int64_t M, int64_t N, int64_t K,
                 double ar, double ai, const void *A, int64_t lda, const void *B, int64_t ldb, int beta_one, void *C,
-                int64_t ldc, hipStream_t s) {
+                int64_t ldc, hipStream_t s, int64_t batch, int64_t sa, int64_t sb, int64_t sc) {
   const bool cplx = (dk == 14 || dk == 18);
   const int opa = op_code(ta), opb = op_code(tb);
   const int cja = (cplx && conja) ? 1 : 0, cjb = (cplx && conjb) ? 1 : 0;
-  if (opa < 0 || opb < 0 || M < 0 || N < 0 || K < 0 || !A || !B || !C) return -1;
+  if (opa < 0 || opb < 0 || M < 0 || N < 0 || K < 0 || !A || !B || !C || batch < 1) return -1;
   if (M == 0 || N == 0) return 0;
   if (ldc < std::max<int64_t>(1, M)) return -1;
   if (lda < std::max<int64_t>(1, opa == 0 ? M : K)) return -1;
   if (ldb < std::max<int64_t>(1, opb == 0 ? K : N)) return -1;
+  if (batch > 1 && (sa < 0 || sb < 0 || sc < ldc * N)) return -1;  // batch entries of C may not overlap
   if (dk == 4 && r4mode == R4_SPLIT_BF16) {
-    if (opa == 0 && opb == 0) return launch_bf16x6<0, 0>(M, N, K, A, lda, B, ldb, C, ldc, ar, beta_one, s);
-    if (opa == 0 && opb == 1) return launch_bf16x6<0, 1>(M, N, K, A, lda, B, ldb, C, ldc, ar, beta_one, s);
-    if (opa == 1 && opb == 0) return launch_bf16x6<1, 0>(M, N, K, A, lda, B, ldb, C, ldc, ar, beta_one, s);
-    return launch_bf16x6<1, 1>(M, N, K, A, lda, B, ldb, C, ldc, ar, beta_one, s);
+    for (int64_t q = 0; q < batch; ++q) {
+      const float *a = (const float *)A + q * sa, *b = (const float *)B + q * sb;
+      float *c = (float *)C + q * sc;
+      int e;
+      if (opa == 0 && opb == 0) e = launch_bf16x6<0, 0>(M, N, K, a, lda, b, ldb, c, ldc, ar, beta_one, s);
+      else if (opa == 0 && opb == 1) e = launch_bf16x6<0, 1>(M, N, K, a, lda, b, ldb, c, ldc, ar, beta_one, s);
+      else if (opa == 1 && opb == 0) e = launch_bf16x6<1, 0>(M, N, K, a, lda, b, ldb, c, ldc, ar, beta_one, s);
+      else e = launch_bf16x6<1, 1>(M, N, K, a, lda, b, ldb, c, ldc, ar, beta_one, s);
+      if (e) return e;
+    }
+    return 0;
   }
   switch (dk) {
-    case 8: return dispatch_ops<double, 1, 128, 128>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, s);
-    case 4: return dispatch_ops<float, 1, 128, 128>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, s);
-    case 18: return dispatch_ops<double, 2, 64, 64>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, s);
-    case 14: return dispatch_ops<float, 2, 64, 64>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, s);
+    case 8: return dispatch_ops<double, 1, 128, 128>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, sa, sb, sc, s);
+    case 4: return dispatch_ops<float, 1, 128, 128>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, sa, sb, sc, s);
+    case 18: return dispatch_ops<double, 2, 64, 64>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, sa, sb, sc, s);
+    case 14: return dispatch_ops<float, 2, 64, 64>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, sa, sb, sc, s);
     default: return -1;
   }
 }

@@ -50,8 +50,10 @@ int launch_gemm_split(char ta, char tb, int64_t M, int64_t N, int64_t K, double 
 // ta: 'N' A is MxK (lda>=M); 'T' A is KxM (lda>=K).
 // tb: 'N' B is KxN (ldb>=K); 'T' B is NxK (ldb>=N).
 // conja/conjb conjugate complex operands.  Complex kinds use interleaved (re,im) storage.
+// batch > 1: that many independent products, entry q at A + q*sa, B + q*sb,
+// C + q*sc (strides in elements; C entries may not overlap).
 int launch_gemm(int dk, int r4mode, char ta, char tb, int conja, int conjb, int64_t M, int64_t N, int64_t K,
                 double ar, double ai, const void *A, int64_t lda, const void *B, int64_t ldb, int beta_one, void *C,
-                int64_t ldc, hipStream_t s);
+                int64_t ldc, hipStream_t s, int64_t batch = 1, int64_t sa = 0, int64_t sb = 0, int64_t sc = 0);
 
 }  // namespace tal

@@ -387,13 +387,18 @@ int enqueue_ttgt(const DigitalPattern &p, const ContractPlan &plan, int dk, bool
   if (!cplx) lconj = rconj = 0;
   if (gemm_start) (void)hipEventRecord(gemm_start, s);
   const bool split = (dk == R4 && rt().r4mode == R4_SPLIT_BF16);
+  // hyper indices: plan.batch products of M x N x K, each operand strided by one matrix
+  const int64_t sa = plan.M * plan.K, sb = plan.K * plan.N, sc = plan.M * plan.N;
   if (split) {
-    if (!tmp.w || launch_gemm_split(plan.a.op, plan.b.op, plan.M, plan.N, plan.K, ar, A, plan.a.ld, B, plan.b.ld,
-                                    acc ? 1 : 0, C, std::max<int64_t>(1, plan.M), tmp.w, s))
-      return TALSH_FAILURE;
+    if (!tmp.w) return TALSH_FAILURE;
+    for (int64_t q = 0; q < plan.batch; ++q)  // the planes workspace is reused in stream order
+      if (launch_gemm_split(plan.a.op, plan.b.op, plan.M, plan.N, plan.K, ar, (const float *)A + q * sa, plan.a.ld,
+                            (const float *)B + q * sb, plan.b.ld, acc ? 1 : 0, (float *)C + q * sc,
+                            std::max<int64_t>(1, plan.M), tmp.w, s))
+        return TALSH_FAILURE;
   } else if (launch_gemm(dk, R4_FP32, plan.a.op, plan.b.op, plan.swap ? rconj : lconj, plan.swap ? lconj : rconj,
                          plan.M, plan.N, plan.K, ar, cplx ? ai : 0.0, A, plan.a.ld, B, plan.b.ld, acc ? 1 : 0, C,
-                         std::max<int64_t>(1, plan.M), s)) {
+                         std::max<int64_t>(1, plan.M), s, plan.batch, sa, sb, sc)) {
     return TALSH_FAILURE;
   }
   if (gemm_end) (void)hipEventRecord(gemm_end, s);
@@ -644,7 +649,6 @@ int talshTensorContract(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *lt
   ContractPlan plan;
   const int pe = plan_contraction(p, dtens->shape_p->dims, ltens->shape_p->dims, rtens->shape_p->dims,
                                   accumulative != NOPE, &plan);
-  if (pe == 2) return DEVICE_UNABLE;
   if (pe != 0) return TALSH_INVALID_ARGS;
   int g;
   if (int e = exec_gpu(dev_id, dev_kind, {dtens, ltens, rtens}, &g)) return e;
@@ -676,7 +680,7 @@ int talshTensorContract(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *lt
                            op.operand(2).dptr, dtens->shape_p->dims, ltens->shape_p->dims, rtens->shape_p->dims,
                            bufs, op.stream(), op.task()->ev_mm0->ev, op.task()->ev_mm1->ev))
     return e;
-  const double flops = (cplx ? 8.0 : 2.0) * (double)plan.M * (double)plan.N * (double)plan.K;
+  const double flops = (cplx ? 8.0 : 2.0) * (double)plan.M * (double)plan.N * (double)plan.K * (double)plan.batch;
   if (int e = op.end(flops)) return e;
   return op.publish(3);
 }
@@ -974,7 +978,6 @@ int talshAmdPlanContraction(const char *pattern, const int *ddims, const int *ld
   if (!pattern || parse_pattern(pattern, &p) != 0) return TALSH_INVALID_ARGS;
   ContractPlan c;
   const int e = plan_contraction(p, ddims, ldims, rdims, accumulative, &c);
-  if (e == 2) return DEVICE_UNABLE;
   if (e) return TALSH_INVALID_ARGS;
   if (mnk) { mnk[0] = c.M; mnk[1] = c.N; mnk[2] = c.K; }
   if (flags) { flags[0] = c.swap; flags[1] = c.a.transpose; flags[2] = c.b.transpose; flags[3] = c.d_transpose; }

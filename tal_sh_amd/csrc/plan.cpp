@@ -14,7 +14,11 @@
 // both operand roles (L as A, or R as A), both storage forms of each operand
 // ('N' or 'T') and three orderings for the M, N and K index groups are
 // enumerated, and the plan that copies the fewest elements is kept; operands
-// already in a usable layout go straight into the GEMM.
+// already in a usable layout go straight into the GEMM.  Hyper indices (in D,
+// L and R; the reference GPU path refuses them, its CPU path batches them,
+// tensor_algebra_cpu.F90:5757-5956 and :3983-3985) become the batch
+// dimension of a strided batched GEMM, placed after M, N and K in every
+// operand.
 #include "plan.hpp"
 
 #include <algorithm>
@@ -198,10 +202,22 @@ int plan_contraction(const DigitalPattern &p, const int *ddims, const int *ldims
       return 1;
     }
   }
+  // a D index present in both inputs is a hyper index: it becomes a batch
+  // dimension, the slowest of every operand of a strided batched GEMM
+  std::vector<int> hyper;
   for (int i = 0; i < nd; ++i) {
     if (dcount[i] == 0) return 1;
-    if (dcount[i] > 1) return 2;  // hyper-contraction index
+    if (dcount[i] > 1) hyper.push_back(i);
   }
+  auto is_hyper = [&](int s) { return s < kContr && dcount[s] > 1; };
+  auto drop_hyper = [&](const std::vector<int> &v) {
+    std::vector<int> r;
+    for (int s : v)
+      if (!is_hyper(s)) r.push_back(s);
+    return r;
+  };
+  int64_t batch = 1;
+  for (int s : hyper) batch *= ext_of[s];
   const double vd = vol_of(ddims, nd), vl = vol_of(ldims, nl), vr = vol_of(rdims, nr);
 
   bool have = false;
@@ -212,7 +228,7 @@ int plan_contraction(const DigitalPattern &p, const int *ddims, const int *ldims
     const int *xdims = swap ? rdims : ldims, *ydims = swap ? ldims : rdims;
     const int nx = swap ? nr : nl, ny = swap ? nl : nr;
     const double vx = swap ? vr : vl, vy = swap ? vl : vr;
-    const std::vector<int> mx = filter(sx, false), ny_ = filter(sy, false);
+    const std::vector<int> mx = drop_hyper(filter(sx, false)), ny_ = drop_hyper(filter(sy, false));
     std::vector<int> m_sorted(mx), n_sorted(ny_);
     std::sort(m_sorted.begin(), m_sorted.end());
     std::sort(n_sorted.begin(), n_sorted.end());
@@ -223,10 +239,10 @@ int plan_contraction(const DigitalPattern &p, const int *ddims, const int *ldims
           const std::vector<int> &M = mo ? mx : m_sorted, &N = no ? ny_ : n_sorted, &K = ko ? ky : kx;
           ContractPlan c;
           c.swap = swap != 0;
-          const std::vector<int> dseq = cat(M, N);
+          const std::vector<int> dseq = cat(cat(M, N), hyper);
           c.d_transpose = (dseq != sd);
           // operand X (A): 'T' form = K++M, 'N' form = M++K
-          const std::vector<int> xt = cat(K, M), xn = cat(M, K);
+          const std::vector<int> xt = cat(cat(K, M), hyper), xn = cat(cat(M, K), hyper);
           if (sx == xt) { c.a.op = 'T'; }
           else if (sx == xn) { c.a.op = 'N'; }
           else {
@@ -237,7 +253,7 @@ int plan_contraction(const DigitalPattern &p, const int *ddims, const int *ldims
             c.a.o2n = o2n_of(sx, xt);
           }
           // operand Y (B): 'N' form = K++N, 'T' form = N++K
-          const std::vector<int> yn = cat(K, N), yt = cat(N, K);
+          const std::vector<int> yn = cat(cat(K, N), hyper), yt = cat(cat(N, K), hyper);
           if (sy == yn) { c.b.op = 'N'; }
           else if (sy == yt) { c.b.op = 'T'; }
           else {
@@ -251,6 +267,7 @@ int plan_contraction(const DigitalPattern &p, const int *ddims, const int *ldims
           for (int s : N) c.N *= ext_of[s];
           c.K = 1;
           for (int s : K) c.K *= ext_of[s];
+          c.batch = batch;
           c.a.ld = std::max<int64_t>(1, c.a.op == 'T' ? c.K : c.M);
           c.b.ld = std::max<int64_t>(1, c.b.op == 'N' ? c.K : c.N);
           c.moved_elems = (c.a.transpose ? vx : 0) + (c.b.transpose ? vy : 0) +

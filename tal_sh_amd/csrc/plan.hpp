@@ -32,6 +32,7 @@ struct OperandPlan {
 struct ContractPlan {
   bool swap = false;        // A = R and B = L
   int64_t M = 1, N = 1, K = 1;
+  int64_t batch = 1;        // product of the hyper extents: GEMMs of M x N x K, strided by M*K, K*N, M*N
   OperandPlan a, b;
   bool d_transpose = false;
   std::vector<int> d_o2n;        // D -> D' (0-based O2N)
@@ -41,8 +42,8 @@ struct ContractPlan {
 };
 
 // Validates the pattern against the extents and picks the matricization that
-// moves the fewest elements.  Returns 0, or 1 for an invalid pattern/shape,
-// 2 for unsupported index structure (hyper-contraction).
+// moves the fewest elements.  Returns 0, or 1 for an invalid pattern/shape.
+// Hyper indices (present in D, L and R) give plan->batch > 1.
 int plan_contraction(const DigitalPattern &p, const int *ddims, const int *ldims, const int *rdims,
                      int accumulative, ContractPlan *plan);
 

@@ -71,7 +71,6 @@ int estimate(talshAmdServer *s, talshAmdServer::Req *q) {
   tal::ContractPlan plan;
   const int e = tal::plan_contraction(p, r.dtens->shape_p->dims, r.ltens->shape_p->dims, r.rtens->shape_p->dims,
                                       r.accumulative != NOPE, &plan);
-  if (e == 2) return DEVICE_UNABLE;
   if (e) return TALSH_INVALID_ARGS;
   const size_t bd = body_bytes(r.dtens), bl = body_bytes(r.ltens), br = body_bytes(r.rtens);
   size_t need = 0;
@@ -86,7 +85,8 @@ int estimate(talshAmdServer *s, talshAmdServer::Req *q) {
   int dk = NO_TYPE;
   for (int i = 0; i < r.dtens->ndev; ++i)
     if (r.dtens->avail[i] == YEP) dk = r.dtens->data_kind[i];
-  q->flops = ((dk == C4 || dk == C8) ? 8.0 : 2.0) * (double)plan.M * (double)plan.N * (double)plan.K;
+  q->flops = ((dk == C4 || dk == C8) ? 8.0 : 2.0) * (double)plan.M * (double)plan.N * (double)plan.K *
+             (double)plan.batch;
   return TALSH_SUCCESS;
 }
 

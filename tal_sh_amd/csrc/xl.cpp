@@ -18,7 +18,8 @@
 // image by strided DMA copies; an operand that one tile covers and that already
 // lives on the GPU is used in place.  Consecutive groups alternate between two
 // streams, so one group's copies overlap the other's kernels.  As in the
-// reference, D ends with a single, up-to-date host image.
+// reference, D ends with a single, up-to-date host image.  Hyper indices are
+// D indices that L and R share: their tiles cut all three operands alike.
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
@@ -170,9 +171,7 @@ int talshTensorContractXL(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *
   const bool acc = accumulative != NOPE;
   {
     ContractPlan whole;
-    const int pe = plan_contraction(p, dd, ld, rd, acc, &whole);
-    if (pe == 2) return DEVICE_UNABLE;
-    if (pe != 0) return TALSH_INVALID_ARGS;
+    if (plan_contraction(p, dd, ld, rd, acc, &whole) != 0) return TALSH_INVALID_ARGS;
   }
   int g;
   if (int e = exec_gpu(dev_id, dev_kind, {dtens, ltens, rtens}, &g)) return e;
@@ -387,7 +386,7 @@ int talshTensorContractXL(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *
         err = e;
         break;
       }
-      flops += (cplx ? 8.0 : 2.0) * (double)plan.M * (double)plan.N * (double)plan.K;
+      flops += (cplx ? 8.0 : 2.0) * (double)plan.M * (double)plan.N * (double)plan.K * (double)plan.batch;
     }
     if (err) break;
     if (!d_in_place) {

@@ -71,6 +71,48 @@ def random_contraction(rng, max_rank=4, ext_lo=2, ext_hi=6, ncontr=None, allow_s
         return pat, [ext[x] for x in d], [ext[x] for x in l], [ext[x] for x in r]
 
 
+def random_hyper_contraction(rng, max_rank=5, ext_lo=1, ext_hi=6, nhyper=None):
+    """Random contraction with hyper indices (labels in D, L and R at once).
+
+    Returns (pattern, ddims, ldims, rdims); at least one hyper index, zero or
+    more contracted, left-only and right-only indices."""
+    nh = nhyper if nhyper is not None else rng.randint(1, 2)
+    while True:
+        nc, nl, nr = rng.randint(0, 2), rng.randint(0, 2), rng.randint(0, 2)
+        if nh + nc + max(nl, nr) > max_rank:
+            continue
+        labels = [chr(ord("a") + i) for i in range(nh + nc + nl + nr)]
+        ext = {x: rng.randint(ext_lo, ext_hi) for x in labels}
+        hy, cc = labels[:nh], labels[nh:nh + nc]
+        lu, ru = labels[nh + nc:nh + nc + nl], labels[nh + nc + nl:]
+        d = hy + lu + ru
+        rng.shuffle(d)
+        l = hy + lu + cc
+        rng.shuffle(l)
+        r = hy + ru + cc
+        rng.shuffle(r)
+        pat = "D(%s)+=L(%s)*R(%s)" % (",".join(d), ",".join(l), ",".join(r))
+        return pat, [ext[x] for x in d], [ext[x] for x in l], [ext[x] for x in r]
+
+
+def einsum_contract(pat, dd, D, ld, L, rd, R, alpha=1.0, accumulative=1, conj_l=False, conj_r=False):
+    """numpy einsum of a TAL-SH pattern over Fortran-ordered (dlf) views; returns flat D."""
+    lhs, rhs = pat.split("+=")
+    dl = lhs[lhs.index("(") + 1:-1].replace(",", "")
+    lp, rp = rhs.split("*")
+    ll = lp[lp.index("(") + 1:-1].replace(",", "")
+    rl = rp[rp.index("(") + 1:-1].replace(",", "")
+    a = L.reshape(ld, order="F") if ld else L.reshape(())
+    b = R.reshape(rd, order="F") if rd else R.reshape(())
+    if conj_l:
+        a = np.conj(a)
+    if conj_r:
+        b = np.conj(b)
+    E = np.einsum("%s,%s->%s" % (ll, rl, dl), a.astype(np.complex128), b.astype(np.complex128))
+    out = alpha * np.asarray(E).reshape(-1, order="F")
+    return (D.astype(np.complex128) + out) if accumulative else out
+
+
 def random_permutation_pattern(rng, rank):
     labels = [chr(ord("a") + i) for i in range(rank)]
     perm = labels[:]
@@ -143,6 +185,18 @@ class Oracle:
                                       _intp(ldims), l.ctypes.data_as(ctypes.c_void_p), rr, _intp(rdims),
                                       r.ctypes.data_as(ctypes.c_void_p), ctypes.byref(re), ctypes.byref(im))
         return complex(re.value, im.value)
+
+    def contract_direct(self, dk, pattern, ddims, d, ldims, l, rdims, r, alpha=(1.0, 0.0), accumulative=1):
+        """Direct loop-nest contraction (hyper indices allowed); returns the new D."""
+        err, dr, lr, rr, cj, dig = self.parse(pattern)
+        assert err == 0
+        out = np.array(d, dtype=NP_DTYPE[dk], copy=True).reshape(-1)
+        e = self.lib.orc_contract_direct(dk, _intp(dig), cj, dr, _intp(ddims), out.ctypes.data_as(ctypes.c_void_p),
+                                         lr, _intp(ldims), l.ctypes.data_as(ctypes.c_void_p), rr, _intp(rdims),
+                                         r.ctypes.data_as(ctypes.c_void_p), ctypes.c_double(alpha[0]),
+                                         ctypes.c_double(alpha[1]), accumulative)
+        assert e == 0, e
+        return out
 
     def norm2(self, dk, a):
         return self.lib.orc_norm2(dk, a.size, a.ctypes.data_as(ctypes.c_void_p))

@@ -185,9 +185,10 @@ def test_host_side_errors_without_gpu():
         assert ts.contract("D(a,b)+=L(a,i)*R(i,b)", d, l, r, dev_kind=ts.DEV_HOST, dev_id=0) == \
             ts.TALSH_NOT_AVAILABLE
         assert ts.contract("D(a,b)+=L(a,i)*R(i,b)", d, l, r) == ts.TALSH_NOT_AVAILABLE
-        # hyper-contraction indices are not on this path
+        # hyper-contractions are accepted (batched GEMM); with no GPU they are not available either
         dh, lh, rh = ts.Tensor(R8, [2, 3], 0), ts.Tensor(R8, [2, 4, 3], 0), ts.Tensor(R8, [4, 3], 0)
-        assert ts.contract("D(a,c)+=L(a,i,c)*R(i,c)", dh, lh, rh) == ts.DEVICE_UNABLE
+        assert ts.contract("D(a,c)+=L(a,i,c)*R(i,c)", dh, lh, rh) == ts.TALSH_NOT_AVAILABLE
+        assert plan("D(a,c)+=L(a,i,c)*R(i,c)", [2, 3], [2, 4, 3], [4, 3])[:2] == (0, [2, 1, 4])
         # host images: import/export and scalar access
         a = tensor(R8, [2, 3], 5)
         d.upload(a)

@@ -113,6 +113,19 @@ def test_batch_rejects_aliased_outputs(talsh_gpu):
     b.destroy()
 
 
+def test_batch_refers_hyper_contractions_to_ttgt(talsh_gpu):
+    """A hyper index (in D, L and R) is not a gather-GEMM task: the batch reports DEVICE_UNABLE and
+    talshTensorContract, which batches hyper indices, takes it."""
+    ts = talsh_gpu
+    d = upload(ts, R8, [4, 3], np.zeros(12))
+    l = upload(ts, R8, [4, 5, 3], tensor(R8, [4, 5, 3], 1))
+    r = upload(ts, R8, [5, 3], tensor(R8, [5, 3], 2))
+    b = ts.Batch()
+    assert b.add("D(a,h)+=L(a,k,h)*R(k,h)", d, l, r) == 0
+    assert b.execute() == ts.DEVICE_UNABLE
+    b.destroy()
+
+
 @pytest.mark.parametrize("dk", (R8, R4))
 def test_batch_back_to_back_replays(talsh_gpu, oracle, dk):
     """Replays with nothing else touching the operands in between take the

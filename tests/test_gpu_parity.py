@@ -11,7 +11,8 @@ import random
 import numpy as np
 import pytest
 
-from tal_common import C4, C8, R4, R8, NP_DTYPE, random_contraction, random_permutation_pattern, tensor
+from tal_common import (C4, C8, R4, R8, NP_DTYPE, random_contraction, random_hyper_contraction,
+                        random_permutation_pattern, tensor)
 
 pytestmark = pytest.mark.gpu
 
@@ -196,6 +197,34 @@ def test_contract_random_all_kinds(talsh_gpu, oracle):
             assert relerr(dt.host_array(), want) <= tol, (dk, pat)
 
 
+def test_contract_hyper_all_kinds(talsh_gpu, oracle):
+    """Hyper indices (in D, L and R) run as a strided batched MFMA GEMM; compared with the oracle's
+    direct loop nest.  Includes a batch of 70000 entries (more than one launch's grid.y)."""
+    ts = talsh_gpu
+    rng = random.Random(31)
+    cases = []
+    for t in range(40):
+        dk = KINDS[t % 4]
+        pat, dd, ld, rd = random_hyper_contraction(rng, max_rank=5, ext_lo=1, ext_hi=9)
+        if dk in (C4, C8) and t % 3 == 1:
+            pat = pat.replace("L(", "L+(") if t % 2 else pat.replace("R(", "R+(")
+        cases.append((dk, pat, dd, ld, rd, t % 2))
+    cases.append((R8, "D(a)+=L(a)*R(a)", [70000], [70000], [70000], 1))
+    cases.append((C8, "D(i,a,h)+=L(h,k,i)*R(a,h,k)", [40, 36, 70], [70, 33, 40], [36, 70, 33], 0))
+    for t, (dk, pat, dd, ld, rd, acc) in enumerate(cases):
+        alpha = complex(0.5, 0.25) if dk in (C4, C8) else 0.75
+        say("hyper", dk, pat, dd, ld, rd, "acc", acc)
+        L, R, D = tensor(dk, ld, t), tensor(dk, rd, t + 1), tensor(dk, dd, t + 2)
+        lt, rt, dt = on_gpu(ts, dk, ld, L), on_gpu(ts, dk, rd, R), on_gpu(ts, dk, dd, D)
+        ts.check(ts.contract(pat, dt, lt, rt, alpha=alpha, accumulative=acc), "contract")
+        a = (alpha.real, alpha.imag) if dk in (C4, C8) else (alpha, 0.0)
+        want = oracle.contract_direct(dk, pat, dd, D, ld, L, rd, R, a, acc)
+        tol = 1e-12 if dk in (R8, C8) else 1e-5
+        assert relerr(dt.host_array(), want) <= tol, (dk, pat)
+        for x in (lt, rt, dt):
+            x.destruct()
+
+
 def test_contract_split_bf16_r4(talsh_gpu, oracle):
     """R4 with fast math (3-term split-BF16 MFMA, FP32 accumulation): normwise error <= 1e-6
     against the FP64 oracle evaluated on the same FP32 inputs widened to FP64."""
@@ -214,13 +243,15 @@ def test_contract_split_bf16_r4(talsh_gpu, oracle):
         for _ in range(20):
             pat, dd, ld, rd = random_contraction(rng, max_rank=4, ext_lo=1, ext_hi=13)
             cases.append((pat, dd, ld, rd))
+        # hyper index: one split GEMM per batch entry, reusing the planes workspace
+        cases.append(("D(a,h,b)+=L(a,k,h)*R(k,h,b)", [70, 9, 50], [70, 200, 9], [200, 9, 50]))
         for t, (pat, dd, ld, rd) in enumerate(cases):
             say("split-bf16", pat, dd, ld, rd)
             L, R, D = tensor(R4, ld, t), tensor(R4, rd, t + 1), tensor(R4, dd, t + 2)
             lt, rt, dt = on_gpu(ts, R4, ld, L), on_gpu(ts, R4, rd, R), on_gpu(ts, R4, dd, D)
             ts.check(ts.contract(pat, dt, lt, rt, alpha=0.5), "contract")
-            want = oracle.contract(R8, pat, dd, D.astype(np.float64), ld, L.astype(np.float64), rd,
-                                   R.astype(np.float64), (0.5, 0.0), 1)
+            want = oracle.contract_direct(R8, pat, dd, D.astype(np.float64), ld, L.astype(np.float64), rd,
+                                          R.astype(np.float64), (0.5, 0.0), 1)
             assert relerr(dt.host_array().astype(np.float64), want) <= 1e-6, pat
     finally:
         lib.talshDisableFastMath(ts.DEV_NVIDIA_GPU, 0)

@@ -89,8 +89,10 @@ def test_xl_one_piece_equals_contract(talsh_gpu, oracle, capfd):
 
 
 # (pattern, ddims, ldims, rdims): the largest index group differs from case to case,
-# so the tiling splits contracted indices, D indices of L, D indices of R, or several
+# so the tiling splits contracted indices, D indices of L, D indices of R, a hyper
+# index (in D, L and R), or several
 SPLIT_CASES = [
+    ("D(a,h,b)+=L(a,k,h)*R(k,h,b)", [30, 40, 20], [30, 25, 40], [25, 40, 20]),  # hyper index h
     ("D(a,b)+=L(a,k)*R(k,b)", [40, 36], [40, 3000], [3000, 36]),            # long K: contracted tiles
     ("D(a,b,c)+=L(c,k,a)*R(b,k)", [90, 70, 30], [30, 12, 90], [70, 12]),    # large D: D boxes
     ("D(a,b,c,d)+=L(c,i,b,j)*R(d,j,a,i)", [14, 13, 12, 11], [12, 9, 13, 10], [11, 10, 14, 9]),  # both
@@ -119,13 +121,13 @@ def test_xl_split_matches_oracle(talsh_gpu, oracle, capfd, where):
                     assert pieces is not None and pieces > 1, pieces
                     assert dx.images() == [ts.DEV_HOST]
                     a = (alpha.real, alpha.imag) if dk in (C4, C8) else (alpha, 0.0)
-                    want = oracle.contract(dk, pat, dd, D, ld, L, rd, R, a, acc)
+                    want = oracle.contract_direct(dk, pat, dd, D, ld, L, rd, R, a, acc)
                     got = dx.host_array()
                     if dk in (R8, C8):
                         e = relerr(got, want)
                         assert e <= 1e-12, (dk, pat, acc, pieces, e)
                     else:  # FP32: error against the size of the terms (sums such as D() cancel)
-                        mag = oracle.contract(R8, pat, dd, np.abs(D).astype(np.float64), ld,
+                        mag = oracle.contract_direct(R8, pat, dd, np.abs(D).astype(np.float64), ld,
                                               np.abs(L).astype(np.float64), rd, np.abs(R).astype(np.float64),
                                               (abs(alpha), 0.0), acc)
                         e = np.linalg.norm((got - want).astype(np.complex128)) / np.linalg.norm(mag)
@@ -147,7 +149,6 @@ def test_xl_errors(talsh_gpu):
     assert ts.contract_xl(pat, l, l, r) == ts.TALSH_INVALID_ARGS          # D aliases L
     assert ts.contract_xl("D(a,b)+=L(a,k)*R(k)", d, l, r) == ts.TALSH_INVALID_ARGS
     h = ts.Tensor(R8, [4, 4], 0)
-    assert ts.contract_xl("D(a,b)+=L(a,b)*R(b,a)", h, d, ts.Tensor(R8, [4, 4], 0)) == ts.DEVICE_UNABLE
     empty = ts.TalshTens()
     assert lib.talshTensorClean(ctypes.byref(empty)) == 0
     assert lib.talshTensorContractXL(pat.encode(), ctypes.byref(empty), l.ptr, r.ptr, 1.0, 0.0, ts.DEV_DEFAULT,

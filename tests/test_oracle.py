@@ -13,7 +13,7 @@ import random
 import numpy as np
 import pytest
 
-from tal_common import C4, C8, R4, R8, NP_DTYPE, random_contraction, tensor
+from tal_common import C4, C8, R4, R8, NP_DTYPE, einsum_contract, random_contraction, random_hyper_contraction, tensor
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -189,6 +189,43 @@ def test_contract_matches_einsum_and_loop_nest(oracle):
         for off in (0, len(D) // 2, len(D) - 1):
             el = oracle.contract_element(dk, pat, dd, off, ld, L, rd, R)
             assert abs(D[off] + 0.75 * el - want[off]) <= 1e-13 * max(1.0, abs(want[off]))
+
+
+def test_contract_direct_matches_einsum_incl_hyper(oracle):
+    """orc_contract_direct (the direct loop nest) against numpy einsum for contractions with hyper
+    indices, all data kinds, conjugated operands and both accumulate modes; and against the
+    matricized orc_contract for ordinary patterns."""
+    rng = random.Random(23)
+    for t in range(80):
+        dk = (R8, C8, R4, C4)[t % 4]
+        pat, dd, ld, rd = random_hyper_contraction(rng, max_rank=5, ext_lo=1, ext_hi=5)
+        cl = cr = False
+        if dk in (C4, C8) and t % 3:
+            cl, cr = (t % 3 == 1), (t % 3 == 2)
+            pat = pat.replace("L(", "L+(") if cl else pat.replace("R(", "R+(")
+        acc = t % 2
+        alpha = complex(0.5, -0.75) if dk in (C4, C8) else 0.75
+        L, R, D = tensor(dk, ld, t), tensor(dk, rd, t + 100), tensor(dk, dd, t + 200)
+        a = (alpha.real, alpha.imag) if dk in (C4, C8) else (alpha, 0.0)
+        got = oracle.contract_direct(dk, pat, dd, D, ld, L, rd, R, a, acc)
+        want = einsum_contract(pat, dd, D, ld, L, rd, R, alpha, acc, cl, cr)
+        tol = 1e-14 if dk in (R8, C8) else 2e-6
+        assert np.max(np.abs(got - want)) <= tol * max(1.0, np.max(np.abs(want))), (pat, dk)
+    for t in range(20):
+        dk = (R8, C8)[t % 2]
+        pat, dd, ld, rd = random_contraction(rng, max_rank=4, ext_lo=1, ext_hi=5, allow_scalar=True)
+        L, R, D = tensor(dk, ld, t), tensor(dk, rd, t + 100), tensor(dk, dd, t + 200)
+        got = oracle.contract_direct(dk, pat, dd, D, ld, L, rd, R, (0.75, 0.0), 1)
+        ref = oracle.contract(dk, pat, dd, D, ld, L, rd, R, (0.75, 0.0), 1)
+        assert np.max(np.abs(got - ref)) <= 1e-15 * max(1.0, np.max(np.abs(ref))), pat
+    # invariant: summing a hyper result over its hyper index gives the contraction over that index
+    L, R = tensor(R8, [3, 4, 5], 1), tensor(R8, [5, 4, 2], 2)
+    hyp = oracle.contract_direct(R8, "D(a,h,b)+=L(a,h,k)*R(k,h,b)", [3, 4, 2], np.zeros(24), [3, 4, 5], L,
+                                 [5, 4, 2], R, (1.0, 0.0), 0)
+    full = oracle.contract(R8, "D(a,b)+=L(a,h,k)*R(k,h,b)", [3, 2], np.zeros(6), [3, 4, 5], L, [5, 4, 2], R,
+                           (1.0, 0.0), 0)
+    summed = hyp.reshape(3, 4, 2, order="F").sum(axis=1).reshape(-1, order="F")
+    assert np.max(np.abs(summed - full)) <= 1e-14
 
 
 def test_norm2_exact_small_integers(oracle):
