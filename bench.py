@@ -183,6 +183,37 @@ def bench_xl(oracle, n=96, grant=2 << 30, reps=3, seed=300):
     return out
 
 
+def bench_hyper(oracle, steps=5, warmup=1, seed=400):
+    """Hyper-contraction D(a,b,h) += L(a,k,h) * R(k,b,h) (h in D, L and R), R8, a = b = k = 1024, h = 16:
+    16 independent 1024^3 products as one strided batched FP64 MFMA GEMM, GPU-resident operands."""
+    pat = "D(a,b,h)+=L(a,k,h)*R(k,b,h)"
+    n, nh = 1024, 16
+    dims = [n, n, nh]
+    l, L = gpu_tensor(R8, dims, seed)
+    r, R = gpu_tensor(R8, dims, seed + 2)
+    d, D0 = gpu_tensor(R8, dims, seed + 4)
+    wall, mm, tot = timed_contractions(pat, d, l, r, steps, warmup)
+    flop = 2.0 * n ** 3 * nh
+    res = {"tflops": flop * steps / wall / 1e12, "ms_per_step": 1e3 * wall / steps,
+           "gemm_ms": 1e3 * float(np.mean(mm)), "gemm_tflops": flop / float(np.mean(mm)) / 1e12,
+           "gemm_frac_of_fp64_peak": flop / float(np.mean(mm)) / 1e12 / FP64_PEAK_TFLOPS, "flop_per_step": flop}
+    out = d.host_array()
+    reps = warmup + steps
+    # sampled elements against the oracle's direct loop nest: D = D0 + reps * sum_k L * R
+    rng = np.random.default_rng(9)
+    idx = rng.choice(out.size, size=64, replace=False)
+    num = den = 0.0
+    for off in idx:
+        el = oracle.contract_element(R8, pat, dims, int(off), dims, L, dims, R)
+        want = D0[off] + reps * el.real
+        num += (out[off] - want) ** 2
+        den += want ** 2
+    res["rel_err_vs_fp64"] = float(np.sqrt(num / den))
+    for t in (l, r, d):
+        t.destruct()
+    return res
+
+
 def bench_transposes(oracle, steps):
     """Config 2: rank-6 R8 16^6, identity + reversal + 32 seeded permutations; add; norm2."""
     import random
@@ -645,7 +676,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--only", default="",
-                    help="comma list of parts to run (head,r4,bf16,c8,config1,transpose,batch,xl,serve)")
+                    help="comma list of parts to run (head,r4,bf16,c8,config1,transpose,batch,hyper,xl,serve)")
     ap.add_argument("--serve-requests", type=int, default=4000, help="serving stream length (full: 20000)")
     ap.add_argument("--serve-rate", type=float, default=None, help="arrivals per second (default 0.8 x capacity)")
     ap.add_argument("--serve-streams", type=int, default=16, help="streams per GPU for the serving run")
@@ -680,6 +711,8 @@ def main():
     if want("batch"):
         secondary["batched_config4_r8"] = bench_batch(oracle, R8)
         secondary["batched_config4_r4"] = bench_batch(oracle, R4)
+    if want("hyper"):
+        secondary["hyper_r8_batched"] = bench_hyper(oracle)
     if want("xl"):
         secondary["xl_config3_host_operands"] = bench_xl(oracle)
     ts.shutdown()
