@@ -142,8 +142,9 @@ def logged_call(fn):
 
 def bench_xl(oracle, n=96, grant=2 << 30, reps=3, seed=300):
     """talshTensorContractXL of the config-3 contraction with Host-resident operands (pinned host buffer):
-    with the default device-memory grant (one piece) and with `grant` bytes (the operation is cut into
-    pieces).  Each call moves L, R and D to the GPU and D back, so the bound is the host link."""
+    with the default device-memory grant (everything fits: L crosses the link in 4 boxes that overlap the
+    contraction, R once, whole) and with `grant` bytes (cut into pieces that fit).  Each call moves L, R
+    and D to the GPU and D back, so the bound is the host link."""
     import re
     pat = "D(a,b,c,d)+=L(a,i,c,j)*R(j,b,i,d)"
     dims = [n] * 4
@@ -152,7 +153,7 @@ def bench_xl(oracle, n=96, grant=2 << 30, reps=3, seed=300):
     l, r, d = (ts.Tensor(R8, dims, 0, data=x) for x in (L, R, D0))
     flop = 2.0 * float(n) ** 6
     out = {"flop_per_call": flop, "operand_bytes": 8 * size}
-    for label, g in (("one_piece", 0), ("pieces", grant)):
+    for label, g in (("default_grant", 0), ("grant_%dgb" % (grant >> 30), grant)):
         ts.set_xl_memory(g)
         log("xl", label, "grant", g)
         (err, text) = logged_call(lambda: ts.contract_xl(pat, d, l, r, alpha=1.0))

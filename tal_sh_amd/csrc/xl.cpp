@@ -269,6 +269,26 @@ int talshTensorContractXL(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *
     if (best < 0) return DEVICE_UNABLE;
     tile[best] = (tile[best] + 1) / 2;
   }
+  // With host operands and a single piece nothing would overlap the host link
+  // with the GPU: cut the larger host-resident input along one of its D
+  // indices (the slowest, so DMA rows stay long) into kStreamBoxes boxes, so
+  // that one box crosses the link while another is contracted; the other input
+  // then crosses once, whole.
+  constexpr int kStreamBoxes = 4;
+  if (groups_of(tile) == 1 && covers(lid, tile) && covers(rid, tile) && (!hl.gpu || !hr.gpu)) {
+    const size_t lvol = vol_of(lid, ext), rvol = vol_of(rid, ext);
+    const bool use_l = !hl.gpu && (hr.gpu || lvol >= rvol);
+    int best = -1;
+    for (int id : use_l ? lid : rid)
+      if (id < nd && ext[id] >= kStreamBoxes &&
+          (best < 0 || slow[id] > slow[best] || (slow[id] == slow[best] && ext[id] > ext[best])))
+        best = id;
+    if (best >= 0) {
+      std::vector<int> t = tile;
+      t[best] = (ext[best] + kStreamBoxes - 1) / kStreamBoxes;
+      if (2 * need(t) <= budget) tile = t;
+    }
+  }
   std::vector<int> ntile(nid);
   size_t npieces = 1;
   for (int id = 0; id < nid; ++id) {
@@ -282,14 +302,40 @@ int talshTensorContractXL(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *
                 npieces, ngroups, nk);
 
   const bool d_in_place = covers(did, tile) && hd.gpu;
-  const bool l_in_place = covers(lid, tile) && hl.gpu;
-  const bool r_in_place = covers(rid, tile) && hr.gpu;
+  bool l_in_place = covers(lid, tile) && hl.gpu;
+  bool r_in_place = covers(rid, tile) && hr.gpu;
   hipStream_t streams[2] = {dev.streams[0], dev.streams[dev.streams.size() > 1 ? 1 : 0]};
-  Slot slots[2];
+  Slot slots[2], whole;  // whole: inputs moved to the GPU once for the whole call
   for (Slot &sl : slots)
     if (hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess) return TALSH_FAILURE;
+  if (hipEventCreateWithFlags(&whole.done, hipEventDisableTiming) != hipSuccess) return TALSH_FAILURE;
   int err = TALSH_SUCCESS;
   double flops = 0.0, moved_in = 0.0, moved_out = 0.0;
+  // an input that one tile covers but that lives on the host crosses the link
+  // once, up front, into a buffer that every piece reads in place
+  auto upload_whole = [&](Home &h, bool &in_place, const std::vector<int> &ids, int rank, const int *dims) {
+    if (err || in_place || h.gpu || !covers(ids, tile)) return;
+    const size_t bytes = vol_of(ids, ext) * esz;
+    void *buf = slot_alloc(g, whole, bytes, &err);
+    if (err) return;
+    std::vector<int> zero(rank, 0);
+    if (box_dma(true, h.p, (char *)buf, rank, dims, zero.data(), dims, esz, streams[0])) {
+      err = TALSH_FAILURE;
+      return;
+    }
+    moved_in += (double)bytes;
+    h.p = (char *)buf;
+    h.gpu = true;
+    in_place = true;
+    whole.busy = true;
+  };
+  upload_whole(hl, l_in_place, lid, p.lrank, ld);
+  upload_whole(hr, r_in_place, rid, p.rrank, rd);
+  if (!err && whole.busy) {
+    if (hipEventRecord(whole.done, streams[0]) != hipSuccess ||
+        hipStreamWaitEvent(streams[1], whole.done, 0) != hipSuccess)
+      err = TALSH_FAILURE;
+  }
   std::vector<int> pos(nid, 0);  // tile coordinates of the current piece
   std::vector<int> pd(nd), pl(p.lrank), pr(p.rrank), od(nd), ol(p.lrank), orr(p.rrank);
   auto piece_dims = [&]() {
@@ -403,11 +449,13 @@ int talshTensorContractXL(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *
     if (hipEventRecord(sl.done, s) != hipSuccess) { err = TALSH_FAILURE; break; }
     sl.busy = true;
   }
+  if (err) (void)hipStreamSynchronize(streams[0]), (void)hipStreamSynchronize(streams[1]);
   for (Slot &sl : slots) {
-    if (!sl.busy && err) (void)hipStreamSynchronize(streams[0]), (void)hipStreamSynchronize(streams[1]);
     slot_release(g, sl);
     (void)hipEventDestroy(sl.done);
   }
+  slot_release(g, whole);  // every slot's work, which reads these, has completed
+  (void)hipEventDestroy(whole.done);
   if (err) {
     dev.tasks_failed++;
     return err == TRY_LATER || err == DEVICE_UNABLE ? err : TALSH_FAILURE;

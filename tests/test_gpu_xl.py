@@ -63,7 +63,30 @@ def xl_log(capfd, fn):
     return res, (int(m[-1]) if m else None)
 
 
+def test_xl_streams_host_operands(talsh_gpu, oracle, capfd):
+    """Operands on the host and a contraction that fits: the larger input is cut into 4 boxes along
+    its slowest D index (one box crosses the host link while another is contracted) and the other
+    input crosses once, whole."""
+    ts = talsh_gpu
+    cases = [("D(a,b,c,d)+=L(a,i,c,j)*R(j,b,i,d)", [12, 10, 16, 9], [12, 7, 16, 5], [5, 10, 7, 9]),
+             ("D(a,b)+=L(k,b)*R(a,k)", [33, 64], [50, 64], [33, 50])]
+    for t, (pat, dd, ld, rd) in enumerate(cases):
+        for dk in (R8, C8):
+            for acc in (1, 0):
+                say("xl stream", dk, pat, "acc", acc)
+                L, R, D = tensor(dk, ld, 7 * t + dk), tensor(dk, rd, 7 * t + dk + 1), tensor(dk, dd, 7 * t + dk + 2)
+                lx, rx, dx = place(ts, "arena", dk, ld, L), place(ts, "ext", dk, rd, R), place(ts, "arena", dk, dd, D)
+                err, pieces = xl_log(capfd, lambda: ts.contract_xl(pat, dx, lx, rx, alpha=0.5, accumulative=acc))
+                assert err == 0 and pieces == 4, (err, pieces)
+                assert dx.images() == [ts.DEV_HOST]
+                want = oracle.contract_direct(dk, pat, dd, D, ld, L, rd, R, (0.5, 0.0), acc)
+                assert relerr(dx.host_array(), want) <= 1e-12
+                for x in (lx, rx, dx):
+                    x.destruct()
+
+
 def test_xl_one_piece_equals_contract(talsh_gpu, oracle, capfd):
+    """GPU-resident operands that fit: one piece, bit-identical to talshTensorContract."""
     ts = talsh_gpu
     rng = random.Random(5)
     for t in range(8):
@@ -73,7 +96,7 @@ def test_xl_one_piece_equals_contract(talsh_gpu, oracle, capfd):
         alpha = complex(0.5, -0.25) if dk in (C4, C8) else 0.75
         say("xl one piece", dk, pat, dd, ld, rd, "acc", acc)
         L, R, D = tensor(dk, ld, 3 * t), tensor(dk, rd, 3 * t + 1), tensor(dk, dd, 3 * t + 2)
-        lx, rx, dx = place(ts, "arena", dk, ld, L), place(ts, "arena", dk, rd, R), place(ts, "arena", dk, dd, D)
+        lx, rx, dx = place(ts, "gpu", dk, ld, L), place(ts, "gpu", dk, rd, R), place(ts, "gpu", dk, dd, D)
         err, pieces = xl_log(capfd, lambda: ts.contract_xl(pat, dx, lx, rx, alpha=alpha, accumulative=acc))
         assert err == 0 and pieces == 1, (err, pieces)
         assert dx.images() == [ts.DEV_HOST]
