@@ -669,6 +669,18 @@ def check_init(err):
         raise RuntimeError("talshInit failed: %d" % err)
 
 
+def profiled_traffic(kernel_prefix, path=os.path.join(REPO, "profiles", "contract_r8_96_traffic.md")):
+    """HBM-side bytes per launch of a kernel, from the committed counter summary (FETCH_SIZE and WRITE_SIZE
+    passes scaled by a plain copy of known size); None when the summary has no line for it."""
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch " + kernel_prefix):
+                return int(line.rsplit(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -748,7 +760,10 @@ def main():
                    "parallelism": "single GPU"},
         "rel_err_vs_fp64": head.get("rel_err_vs_fp64"),
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                     "traffic": profiled_traffic("gemm_kernel<double, 1, 1, 0, 0, 0, 128, 128, 2>"),
+                     "traffic_source": "profiles/contract_r8_96_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                                       "copy-calibrated; bytes per GEMM launch)",
                      "kernel": "FP64 MFMA GEMM (2*M*N*K FLOP per launch / HIP-event duration)"},
         "cpu_baseline": cpu,
         "secondary": secondary,
