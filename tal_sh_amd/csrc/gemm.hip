@@ -205,6 +205,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
   for (int64_t kt = 0; kt < nk; ++kt) {
     const int buf = (int)(kt & 1);
     if (kt + 1 < nk) load_tiles((kt + 1) * BK);
+    // interleave the LDS fragment reads with the MFMAs (+2 % on FP64, FP32 and complex)
+    __builtin_amdgcn_iglp_opt(1);
 #pragma unroll
     for (int ks = 0; ks < BK / 4; ++ks) {
       const int kk = ks * 4 + (lane >> 4);
