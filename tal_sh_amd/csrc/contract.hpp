@@ -13,12 +13,14 @@
 namespace tal {
 
 // Bytes of the temporaries a plan needs (0 = not needed): transposed A, B,
-// D' and the split-BF16 planes of both GEMM operands.
+// D', the split-BF16 planes of both GEMM operands, and the partial sums of a
+// GEMM whose last round of tiles is split along K.
 struct TtgtTemps {
-  size_t a = 0, b = 0, d = 0, w = 0;
+  size_t a = 0, b = 0, d = 0, w = 0, s = 0;
 };
 struct TtgtBuffers {
-  void *a = nullptr, *b = nullptr, *d = nullptr, *w = nullptr;
+  void *a = nullptr, *b = nullptr, *d = nullptr, *w = nullptr, *s = nullptr;
+  size_t s_bytes = 0;
 };
 TtgtTemps ttgt_temp_bytes(const ContractPlan &plan, int dk, size_t lbytes, size_t rbytes, size_t dbytes);
 

@@ -72,14 +72,9 @@ __device__ inline Vec<S, W> ld(const S *p) {
   return r;
 }
 
-// Maps the launch index to a C tile so that consecutive tiles of one XCD are
-// neighbours: XCD x (= linear id % 8) gets the x-th contiguous slice of the
-// tile sequence, walked in groups of 8 tile-rows.
-__device__ inline void tile_of(int64_t gx, int64_t gy, int64_t &bx, int64_t &by) {
-  const int64_t total = gx * gy;
-  const int64_t id = (int64_t)blockIdx.x;
-  int64_t t = id;
-  if (total >= kXcds * 4 && total % kXcds == 0) t = (id % kXcds) * (total / kXcds) + id / kXcds;
+// Tile t of C -> (bx, by): tiles are walked in groups of 8 tile-rows, column
+// by column inside a group (a bijection of [0, gx*gy)).
+__device__ inline void tile_xy(int64_t t, int64_t gx, int64_t gy, int64_t &bx, int64_t &by) {
   constexpr int64_t G = 8;  // tile-rows per group
   const int64_t group = t / (G * gy);
   const int64_t first = group * G;
@@ -89,6 +84,16 @@ __device__ inline void tile_of(int64_t gx, int64_t gy, int64_t &bx, int64_t &by)
   by = in / rows;
 }
 
+// Tile of this workgroup among the `count` tiles starting at t0, renumbered
+// so that consecutive tiles of one XCD are neighbours: XCD x (= launch index
+// % 8) gets the x-th contiguous slice of them.
+__device__ inline void tile_of(int64_t gx, int64_t gy, int64_t t0, int64_t count, int64_t &bx, int64_t &by) {
+  const int64_t id = (int64_t)blockIdx.x;
+  int64_t t = id;
+  if (count >= kXcds * 4 && count % kXcds == 0) t = (id % kXcds) * (count / kXcds) + id / kXcds;
+  tile_xy(t0 + t, gx, gy, bx, by);
+}
+
 // layout codes: 0 = N, 1 = T; CJA/CJB conjugate the complex operand; W is the
 // vector width in scalars along each operand's contiguous axis.
 template <typename S, int NC, int OPA, int OPB, int CJA, int CJB, int BM, int BN, int W>
@@ -96,7 +101,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
                                                         int64_t lda, const S *__restrict__ B, int64_t ldb,
                                                         S *__restrict__ C, int64_t ldc, S alpha_r, S alpha_i,
                                                         int beta_one, int64_t gx, int64_t gy, int64_t bsa,
-                                                        int64_t bsb, int64_t bsc) {
+                                                        int64_t bsb, int64_t bsc, int64_t t0, int64_t count,
+                                                        int ksplit, int64_t kchunk, S *__restrict__ part) {
+  // ksplit == 1: tiles [t0, t0 + count) of C, the whole K each.  ksplit > 1:
+  // workgroup q*ksplit + c computes K range [c*kchunk, (c+1)*kchunk) of tile
+  // t0 + q into its own BM x BN block of `part` (no alpha, no beta); a second
+  // kernel sums the ksplit blocks of each tile in order.
   using MF = Mfma<S>;
   // batch entry blockIdx.y: its operands start bsa, bsb, bsc scalars further on
   A += (int64_t)blockIdx.y * bsa;
@@ -118,8 +128,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
-  int64_t bx, by;
-  tile_of(gx, gy, bx, by);
+  int64_t bx, by, kbeg = 0, kend = K;
+  if (ksplit > 1) {
+    tile_xy(t0 + (int64_t)blockIdx.x / ksplit, gx, gy, bx, by);
+    kbeg = (int64_t)(blockIdx.x % ksplit) * kchunk;
+    kend = min(K, kbeg + kchunk);
+  } else {
+    tile_of(gx, gy, t0, count, bx, by);
+  }
   const int64_t m0 = bx * BM, n0 = by * BN;
 
   // per-thread load coordinates: contiguous-axis scalar offset and line index
@@ -136,7 +152,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
       int64_t m, k;
       if (OPA == 0) { m = m0 + a_c / NC; k = k0 + line; }
       else { m = m0 + line; k = k0 + a_c / NC; }
-      const bool ok = (m < M) && (k < K);
+      const bool ok = (m < M) && (k < kend);
       const int64_t off = (OPA == 0) ? (m + k * lda) : (k + m * lda);
       if (ok) {
         ra[e] = ld<S, W>(A + off * NC + (a_c % NC));
@@ -151,7 +167,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
       int64_t n, k;
       if (OPB == 0) { k = k0 + b_c / NC; n = n0 + line; }
       else { k = k0 + line; n = n0 + b_c / NC; }
-      const bool ok = (n < N) && (k < K);
+      const bool ok = (n < N) && (k < kend);
       const int64_t off = (OPB == 0) ? (k + n * ldb) : (n + k * ldb);
       if (ok) {
         rb[e] = ld<S, W>(B + off * NC + (b_c % NC));
@@ -197,14 +213,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
 
   const S sa = CJA ? (S)-1 : (S)1;  // sign of A's imaginary part
   const S sb = CJB ? (S)-1 : (S)1;
-  const int64_t nk = (K + BK - 1) / BK;
+  const int64_t nk = (kend - kbeg + BK - 1) / BK;
 
-  load_tiles(0);
+  load_tiles(kbeg);
   store_tiles(0);
   __syncthreads();
   for (int64_t kt = 0; kt < nk; ++kt) {
     const int buf = (int)(kt & 1);
-    if (kt + 1 < nk) load_tiles((kt + 1) * BK);
+    if (kt + 1 < nk) load_tiles(kbeg + (kt + 1) * BK);
     // interleave the LDS fragment reads with the MFMAs (+2 % on FP64, FP32 and complex)
     __builtin_amdgcn_iglp_opt(1);
 #pragma unroll
@@ -239,6 +255,20 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
   }
 
   // epilogue
+  if (ksplit > 1) {  // raw partial sums of this K range, BM x BN block column-major
+    S *pb = part + (int64_t)blockIdx.x * BM * BN * NC;
+#pragma unroll
+    for (int i = 0; i < SM; ++i)
+#pragma unroll
+      for (int j = 0; j < SN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int ml = wm * (BM / 2) + i * 16 + MF::row(lane, r), nl = wn * (BN / 2) + j * 16 + (lane & 15);
+#pragma unroll
+          for (int c = 0; c < NC; ++c) pb[(ml + nl * BM) * NC + c] = acc[c][i][j][r];
+        }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < SM; ++i)
 #pragma unroll
@@ -260,6 +290,40 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
           }
         }
       }
+}
+
+// Second pass of a K-split tile range: sums the ksplit partial blocks of
+// tiles [t0, t0 + ntiles) in order and applies C = alpha * sum (+ C).
+template <typename S, int NC, int BM, int BN>
+__global__ __launch_bounds__(kThreads) void gemm_tail_reduce(int64_t M, int64_t N, const S *__restrict__ part,
+                                                             int ksplit, S *__restrict__ C, int64_t ldc, S alpha_r,
+                                                             S alpha_i, int beta_one, int64_t gx, int64_t gy,
+                                                             int64_t t0, int64_t ntiles) {
+  const int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (e >= ntiles * BM * BN) return;
+  const int64_t q = e / (BM * BN);
+  const int rem = (int)(e % (BM * BN)), ml = rem % BM, nl = rem / BM;
+  int64_t bx, by;
+  tile_xy(t0 + q, gx, gy, bx, by);
+  const int64_t m = bx * BM + ml, n = by * BN + nl;
+  if (m >= M || n >= N) return;
+  S x[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) x[c] = (S)0;
+  for (int ch = 0; ch < ksplit; ++ch) {
+    const S *pb = part + ((q * ksplit + ch) * BM * BN + ml + (int64_t)nl * BM) * NC;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) x[c] += pb[c];
+  }
+  S *cp = C + (m + n * ldc) * NC;
+  if (NC == 1) {
+    const S v = alpha_r * x[0];
+    cp[0] = beta_one ? cp[0] + v : v;
+  } else {
+    const S vr = alpha_r * x[0] - alpha_i * x[NC - 1], vi = alpha_r * x[NC - 1] + alpha_i * x[0];
+    cp[0] = beta_one ? cp[0] + vr : vr;
+    cp[NC - 1] = beta_one ? cp[NC - 1] + vi : vi;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -311,7 +375,7 @@ __global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(2)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WM, wn = wave / WM;
   int64_t bx, by;
-  tile_of(gx, gy, bx, by);
+  tile_of(gx, gy, 0, gx * gy, bx, by);
   const int64_t m0 = bx * BM, n0 = by * BN;
   const int a_c = (tid % ATPL) * W, a_l = tid / ATPL;
   const int b_c = (tid % BTPL) * W, b_l = tid / BTPL;
@@ -521,7 +585,7 @@ __global__ __launch_bounds__(512) void gemm_bf16x6p_kernel(int64_t M, int64_t N,
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 3, wn = wave >> 2;
   int64_t bx, by;
-  tile_of(gx, gy, bx, by);
+  tile_of(gx, gy, 0, gx * gy, bx, by);
   const int64_t kpc = Kp / 8;  // chunks per plane row
   // thread tid loads chunk (tid & 3) of row (tid >> 2) [+128] of each plane
   const int lr = tid >> 2, lc = tid & 3;
@@ -656,10 +720,56 @@ int launch_bf16x6(int64_t M, int64_t N, int64_t K, const void *A, int64_t lda, c
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
+// Workgroups of gemm_kernel<S, NC, ..., BM, BN> resident at once on the
+// current device (CUs x blocks per CU), cached per device.
+template <typename S, int NC, int BM, int BN>
+int64_t gemm_slots() {
+  static int64_t cache[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+  if (!cache[dev]) {
+    int cus = 0, per = 0;
+    constexpr int WV = (int)(16 / sizeof(S));
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, gemm_kernel<S, NC, 1, 0, 0, 0, BM, BN, WV>, kThreads, 0) !=
+            hipSuccess)
+      return 0;
+    cache[dev] = (int64_t)cus * std::max(per, 1);
+  }
+  return cache[dev];
+}
+
+// The tiles of C are dealt to the resident workgroups in rounds.  When the
+// last round is at most a quarter full (or the whole GEMM is), its `rem`
+// tiles are computed as ksplit K ranges each (filling the GPU) plus a
+// summing pass; ksplit = 1 means no split.
+struct TailSplit {
+  int64_t rem = 0;
+  int ksplit = 1;
+};
+template <typename S, int NC, int BM, int BN>
+TailSplit tail_split(int64_t M, int64_t N, int64_t K, int64_t batch) {
+  TailSplit t;
+  if (batch != 1 || M <= 0 || N <= 0) return t;
+  const int64_t T = ((M + BM - 1) / BM) * ((N + BN - 1) / BN), slots = gemm_slots<S, NC, BM, BN>();
+  if (slots <= 0) return t;
+  const int64_t rem = T % slots;
+  if (rem == 0 || rem * 4 > slots) return t;
+  const int64_t ks = std::min<int64_t>(std::min<int64_t>(16, slots / rem), ((K + BK - 1) / BK) / 8);
+  if (ks < 2) return t;
+  t.rem = rem;
+  t.ksplit = (int)ks;
+  return t;
+}
+template <typename S, int NC, int BM, int BN>
+size_t tail_bytes(const TailSplit &t) {
+  return t.ksplit > 1 ? (size_t)t.rem * t.ksplit * BM * BN * NC * sizeof(S) : 0;
+}
+
 template <typename S, int NC, int OPA, int OPB, int CJA, int CJB, int BM, int BN>
 int launch_typed(int64_t M, int64_t N, int64_t K, const void *A, int64_t lda, const void *B, int64_t ldb,
                  void *C, int64_t ldc, double ar, double ai, int beta_one, int64_t batch, int64_t sa, int64_t sb,
-                 int64_t sc, hipStream_t s) {
+                 int64_t sc, void *ws, size_t ws_bytes, hipStream_t s) {
   const int64_t gx = (M + BM - 1) / BM, gy = (N + BN - 1) / BN;
   const int64_t nblk = gx * gy;
   if (nblk > 0x7fffffffLL) return -4;
@@ -669,17 +779,39 @@ int launch_typed(int64_t M, int64_t N, int64_t K, const void *A, int64_t lda, co
   const bool vec = ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0) && acont % WV == 0 &&
                    bcont % WV == 0 && (lda * NC) % WV == 0 && (ldb * NC) % WV == 0 &&
                    (batch == 1 || ((sa * NC) % WV == 0 && (sb * NC) % WV == 0));
+  TailSplit ts = tail_split<S, NC, BM, BN>(M, N, K, batch);
+  if (!ws || tail_bytes<S, NC, BM, BN>(ts) > ws_bytes) ts = TailSplit{};
+  const int64_t full = nblk - ts.rem;  // tiles computed whole, in the first launch
+  const int64_t kchunk = ts.ksplit > 1 ? (((K + BK - 1) / BK + ts.ksplit - 1) / ts.ksplit) * BK : 0;
   // batch entries go to grid.y, at most 65535 per launch
   for (int64_t b0 = 0; b0 < batch; b0 += 65535) {
     const unsigned nb = (unsigned)std::min<int64_t>(65535, batch - b0);
     const S *a = (const S *)A + b0 * sa * NC, *b = (const S *)B + b0 * sb * NC;
     S *c = (S *)C + b0 * sc * NC;
+    if (full > 0) {
+      if (vec)
+        gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, WV><<<dim3((unsigned)full, nb), dim3(kThreads), 0, s>>>(
+            M, N, K, a, lda, b, ldb, c, ldc, (S)ar, (S)ai, beta_one, gx, gy, sa * NC, sb * NC, sc * NC, 0, full, 1,
+            0, nullptr);
+      else
+        gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, 1><<<dim3((unsigned)full, nb), dim3(kThreads), 0, s>>>(
+            M, N, K, a, lda, b, ldb, c, ldc, (S)ar, (S)ai, beta_one, gx, gy, sa * NC, sb * NC, sc * NC, 0, full, 1,
+            0, nullptr);
+    }
+  }
+  if (ts.ksplit > 1) {
+    const unsigned nw = (unsigned)(ts.rem * ts.ksplit);
     if (vec)
-      gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, WV><<<dim3((unsigned)nblk, nb), dim3(kThreads), 0, s>>>(
-          M, N, K, a, lda, b, ldb, c, ldc, (S)ar, (S)ai, beta_one, gx, gy, sa * NC, sb * NC, sc * NC);
+      gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, WV><<<dim3(nw), dim3(kThreads), 0, s>>>(
+          M, N, K, (const S *)A, lda, (const S *)B, ldb, (S *)C, ldc, (S)ar, (S)ai, beta_one, gx, gy, 0, 0, 0, full,
+          ts.rem, ts.ksplit, kchunk, (S *)ws);
     else
-      gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, 1><<<dim3((unsigned)nblk, nb), dim3(kThreads), 0, s>>>(
-          M, N, K, a, lda, b, ldb, c, ldc, (S)ar, (S)ai, beta_one, gx, gy, sa * NC, sb * NC, sc * NC);
+      gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, 1><<<dim3(nw), dim3(kThreads), 0, s>>>(
+          M, N, K, (const S *)A, lda, (const S *)B, ldb, (S *)C, ldc, (S)ar, (S)ai, beta_one, gx, gy, 0, 0, 0, full,
+          ts.rem, ts.ksplit, kchunk, (S *)ws);
+    const int64_t nel = ts.rem * BM * BN;
+    gemm_tail_reduce<S, NC, BM, BN><<<dim3((unsigned)((nel + kThreads - 1) / kThreads)), dim3(kThreads), 0, s>>>(
+        M, N, (const S *)ws, ts.ksplit, (S *)C, ldc, (S)ar, (S)ai, beta_one, gx, gy, full, ts.rem);
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
@@ -687,11 +819,11 @@ int launch_typed(int64_t M, int64_t N, int64_t K, const void *A, int64_t lda, co
 template <typename S, int NC, int BM, int BN>
 int dispatch_ops(int opa, int opb, int cja, int cjb, int64_t M, int64_t N, int64_t K, const void *A, int64_t lda,
                  const void *B, int64_t ldb, void *C, int64_t ldc, double ar, double ai, int beta_one,
-                 int64_t batch, int64_t sa, int64_t sb, int64_t sc, hipStream_t s) {
+                 int64_t batch, int64_t sa, int64_t sb, int64_t sc, void *ws, size_t ws_bytes, hipStream_t s) {
 #define TAL_GEMM_CASE(a, b, ca, cb)                                                                      \
   if (opa == a && opb == b && cja == ca && cjb == cb)                                                    \
     return launch_typed<S, NC, a, b, ca, cb, BM, BN>(M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, \
-                                                     sa, sb, sc, s);
+                                                     sa, sb, sc, ws, ws_bytes, s);
 #define TAL_GEMM_LAYOUTS(ca, cb) \
   TAL_GEMM_CASE(0, 0, ca, cb)    \
   TAL_GEMM_CASE(0, 1, ca, cb)    \
@@ -750,9 +882,20 @@ int launch_gemm_split(char ta, char tb, int64_t M, int64_t N, int64_t K, double 
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
+size_t gemm_tail_workspace(int dk, int64_t M, int64_t N, int64_t K, int64_t batch) {
+  switch (dk) {
+    case 8: return tail_bytes<double, 1, 128, 128>(tail_split<double, 1, 128, 128>(M, N, K, batch));
+    case 4: return tail_bytes<float, 1, 128, 128>(tail_split<float, 1, 128, 128>(M, N, K, batch));
+    case 18: return tail_bytes<double, 2, 64, 64>(tail_split<double, 2, 64, 64>(M, N, K, batch));
+    case 14: return tail_bytes<float, 2, 64, 64>(tail_split<float, 2, 64, 64>(M, N, K, batch));
+    default: return 0;
+  }
+}
+
 int launch_gemm(int dk, int r4mode, char ta, char tb, int conja, int conjb, int64_t M, int64_t N, int64_t K,
                 double ar, double ai, const void *A, int64_t lda, const void *B, int64_t ldb, int beta_one, void *C,
-                int64_t ldc, hipStream_t s, int64_t batch, int64_t sa, int64_t sb, int64_t sc) {
+                int64_t ldc, hipStream_t s, int64_t batch, int64_t sa, int64_t sb, int64_t sc, void *ws,
+                size_t ws_bytes) {
   const bool cplx = (dk == 14 || dk == 18);
   const int opa = op_code(ta), opb = op_code(tb);
   const int cja = (cplx && conja) ? 1 : 0, cjb = (cplx && conjb) ? 1 : 0;
@@ -776,10 +919,10 @@ int launch_gemm(int dk, int r4mode, char ta, char tb, int conja, int conjb, int6
     return 0;
   }
   switch (dk) {
-    case 8: return dispatch_ops<double, 1, 128, 128>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, sa, sb, sc, s);
-    case 4: return dispatch_ops<float, 1, 128, 128>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, sa, sb, sc, s);
-    case 18: return dispatch_ops<double, 2, 64, 64>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, sa, sb, sc, s);
-    case 14: return dispatch_ops<float, 2, 64, 64>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, sa, sb, sc, s);
+    case 8: return dispatch_ops<double, 1, 128, 128>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, sa, sb, sc, ws, ws_bytes, s);
+    case 4: return dispatch_ops<float, 1, 128, 128>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, sa, sb, sc, ws, ws_bytes, s);
+    case 18: return dispatch_ops<double, 2, 64, 64>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, sa, sb, sc, ws, ws_bytes, s);
+    case 14: return dispatch_ops<float, 2, 64, 64>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, sa, sb, sc, ws, ws_bytes, s);
     default: return -1;
   }
 }

@@ -52,8 +52,13 @@ int launch_gemm_split(char ta, char tb, int64_t M, int64_t N, int64_t K, double 
 // conja/conjb conjugate complex operands.  Complex kinds use interleaved (re,im) storage.
 // batch > 1: that many independent products, entry q at A + q*sa, B + q*sb,
 // C + q*sc (strides in elements; C entries may not overlap).
+// ws (ws_bytes >= gemm_tail_workspace(...)) lets a GEMM whose last round of
+// tiles would leave most of the GPU idle compute those tiles in K ranges;
+// without it every tile runs whole.  The result depends only on the shapes.
+size_t gemm_tail_workspace(int dk, int64_t M, int64_t N, int64_t K, int64_t batch);
 int launch_gemm(int dk, int r4mode, char ta, char tb, int conja, int conjb, int64_t M, int64_t N, int64_t K,
                 double ar, double ai, const void *A, int64_t lda, const void *B, int64_t ldb, int beta_one, void *C,
-                int64_t ldc, hipStream_t s, int64_t batch = 1, int64_t sa = 0, int64_t sb = 0, int64_t sc = 0);
+                int64_t ldc, hipStream_t s, int64_t batch = 1, int64_t sa = 0, int64_t sb = 0, int64_t sc = 0,
+                void *ws = nullptr, size_t ws_bytes = 0);
 
 }  // namespace tal

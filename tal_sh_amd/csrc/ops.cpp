@@ -354,6 +354,7 @@ TtgtTemps ttgt_temp_bytes(const ContractPlan &plan, int dk, size_t lbytes, size_
   if (plan.b.transpose) t.b = plan.swap ? lbytes : rbytes;
   if (plan.d_transpose) t.d = dbytes;
   if (dk == R4 && rt().r4mode == R4_SPLIT_BF16) t.w = gemm_split_workspace(plan.M, plan.N, plan.K);
+  else t.s = gemm_tail_workspace(dk, plan.M, plan.N, plan.K, plan.batch);
   return t;
 }
 
@@ -398,7 +399,7 @@ int enqueue_ttgt(const DigitalPattern &p, const ContractPlan &plan, int dk, bool
         return TALSH_FAILURE;
   } else if (launch_gemm(dk, R4_FP32, plan.a.op, plan.b.op, plan.swap ? rconj : lconj, plan.swap ? lconj : rconj,
                          plan.M, plan.N, plan.K, ar, cplx ? ai : 0.0, A, plan.a.ld, B, plan.b.ld, acc ? 1 : 0, C,
-                         std::max<int64_t>(1, plan.M), s, plan.batch, sa, sb, sc)) {
+                         std::max<int64_t>(1, plan.M), s, plan.batch, sa, sb, sc, tmp.s, tmp.s_bytes)) {
     return TALSH_FAILURE;
   }
   if (gemm_end) (void)hipEventRecord(gemm_end, s);
@@ -663,11 +664,12 @@ int talshTensorContract(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *lt
   // temporaries: [A'], [B'], [D'], [split-BF16 planes]
   const TtgtTemps tb = ttgt_temp_bytes(plan, dk, op.operand(1).bytes, op.operand(2).bytes, op.operand(0).bytes);
   std::vector<size_t> temps;
-  int ia = -1, ib = -1, id = -1, iw = -1;
+  int ia = -1, ib = -1, id = -1, iw = -1, is = -1;
   if (tb.a) { ia = (int)temps.size(); temps.push_back(tb.a); }
   if (tb.b) { ib = (int)temps.size(); temps.push_back(tb.b); }
   if (tb.d) { id = (int)temps.size(); temps.push_back(tb.d); }
   if (tb.w) { iw = (int)temps.size(); temps.push_back(tb.w); }
+  if (tb.s) { is = (int)temps.size(); temps.push_back(tb.s); }
   if (int e = op.acquire(temps)) return deferred(g, e);
   if (int e = op.begin(dk, copy_ctrl)) return e;
   TtgtBuffers bufs;
@@ -675,6 +677,8 @@ int talshTensorContract(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *lt
   bufs.b = ib >= 0 ? op.temp(ib) : nullptr;
   bufs.d = id >= 0 ? op.temp(id) : nullptr;
   bufs.w = iw >= 0 ? op.temp(iw) : nullptr;
+  bufs.s = is >= 0 ? op.temp(is) : nullptr;
+  bufs.s_bytes = tb.s;
   op.gemm_events();
   if (int e = enqueue_ttgt(p, plan, dk, acc, scale_real, scale_imag, op.operand(0).dptr, op.operand(1).dptr,
                            op.operand(2).dptr, dtens->shape_p->dims, ltens->shape_p->dims, rtens->shape_p->dims,

@@ -235,15 +235,14 @@ int talshTensorContractXL(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *
     if (covers(lid, t) && hl.gpu) b -= lv;
     if (covers(rid, t) && hr.gpu) b -= rv;
     b += dv + lv + rv;
-    if (dk == R4 && r.r4mode == R4_SPLIT_BF16) {
-      int64_t M = 1, N = 1, K = 1;
-      for (int i = 0; i < p.lrank; ++i)
-        if (lid[i] < nd) M *= t[lid[i]];
-        else K *= t[lid[i]];
-      for (int j = 0; j < p.rrank; ++j)
-        if (rid[j] < nd) N *= t[rid[j]];
-      b += gemm_split_workspace(M, N, K);
-    }
+    int64_t M = 1, N = 1, K = 1;  // a bound: hyper indices count in M and N
+    for (int i = 0; i < p.lrank; ++i)
+      if (lid[i] < nd) M *= t[lid[i]];
+      else K *= t[lid[i]];
+    for (int j = 0; j < p.rrank; ++j)
+      if (rid[j] < nd) N *= t[rid[j]];
+    if (dk == R4 && r.r4mode == R4_SPLIT_BF16) b += gemm_split_workspace(M, N, K);
+    else b += gemm_tail_workspace(dk, M, N, K, 1);
     return b;
   };
   size_t budget = dev.arena.largest_free() / 10 * 9;
@@ -423,6 +422,8 @@ int talshTensorContractXL(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *
       grow(want.b, have.b, tmp.b, fb);
       grow(want.d, have.d, tmp.d, dv);
       grow(want.w, have.w, tmp.w, 0);
+      grow(want.s, have.s, tmp.s, 0);
+      tmp.s_bytes = have.s;
       if (err) break;
       const void *lp = fetch(hl, l_in_place, p.lrank, ld, ol.data(), pl.data(), lbuf, s);
       const void *rp = fetch(hr, r_in_place, p.rrank, rd, orr.data(), pr.data(), rbuf, s);

@@ -225,6 +225,35 @@ def test_contract_hyper_all_kinds(talsh_gpu, oracle):
             x.destruct()
 
 
+def test_contract_gemm_k_split_tail(talsh_gpu, oracle):
+    """GEMMs whose tiles fill only a small part of the GPU (or leave a mostly idle last round) run those tiles
+    as K ranges plus an ordered summing pass.  Checked against the oracle, and the same shapes give the same
+    bits on every call."""
+    ts = talsh_gpu
+    cases = [(R8, "D(a,b)+=L(k,a)*R(k,b)", [768, 768], [2048, 768], [2048, 768]),        # 36 tiles, all split
+             (R8, "D(a,b)+=L(a,k)*R(k,b)", [4352, 2304], [4352, 512], [512, 2304]),      # 612 tiles: last round split
+             (R4, "D(a,b)+=L(k,a)*R(b,k)", [1000, 700], [1500, 1000], [700, 1500]),      # ragged edges
+             (C8, "D(a,b)+=L(k,a)*R(k,b)", [384, 320], [1024, 384], [1024, 320])]
+    for t, (dk, pat, dd, ld, rd) in enumerate(cases):
+        alpha = complex(0.5, -0.25) if dk in (C4, C8) else 0.75
+        say("k-split tail", dk, pat, dd, ld, rd)
+        L, R, D = tensor(dk, ld, 50 + t), tensor(dk, rd, 60 + t), tensor(dk, dd, 70 + t)
+        lt, rt = on_gpu(ts, dk, ld, L), on_gpu(ts, dk, rd, R)
+        outs = []
+        for _ in range(2):
+            dt = on_gpu(ts, dk, dd, D)
+            ts.check(ts.contract(pat, dt, lt, rt, alpha=alpha, accumulative=1), "contract")
+            outs.append(dt.host_array())
+            dt.destruct()
+        assert np.array_equal(outs[0], outs[1]), (dk, pat)
+        a = (alpha.real, alpha.imag) if dk in (C4, C8) else (alpha, 0.0)
+        want = oracle.contract(dk, pat, dd, D, ld, L, rd, R, a, 1)
+        tol = 1e-12 if dk in (R8, C8) else 1e-5
+        assert relerr(outs[0], want) <= tol, (dk, pat)
+        for x in (lt, rt):
+            x.destruct()
+
+
 def test_contract_split_bf16_r4(talsh_gpu, oracle):
     """R4 with fast math (3-term split-BF16 MFMA, FP32 accumulation): normwise error <= 1e-6
     against the FP64 oracle evaluated on the same FP32 inputs widened to FP64."""
