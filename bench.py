@@ -670,11 +670,12 @@ def check_init(err):
 
 
 def profiled_traffic(kernel_prefix, path=os.path.join(REPO, "profiles", "contract_r8_96_traffic.md")):
-    """HBM-side bytes per launch of a kernel, from the committed counter summary (FETCH_SIZE and WRITE_SIZE
-    passes scaled by a plain copy of known size); None when the summary has no line for it."""
+    """HBM-side bytes per GEMM stage (the whole-tile launch, the K-split launch and its summing pass), from
+    the committed counter summary (FETCH_SIZE and WRITE_SIZE passes scaled by a plain copy of known size);
+    None when the summary has no line for it."""
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_launch " + kernel_prefix):
+            if line.startswith("traffic_bytes_per_gemm_stage " + kernel_prefix):
                 return int(line.rsplit(":", 1)[1])
     except OSError:
         pass
@@ -763,8 +764,9 @@ def main():
                      "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
                      "traffic": profiled_traffic("gemm_kernel<double, 1, 1, 0, 0, 0, 128, 128, 2>"),
                      "traffic_source": "profiles/contract_r8_96_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
-                                       "copy-calibrated; bytes per GEMM launch)",
-                     "kernel": "FP64 MFMA GEMM (2*M*N*K FLOP per launch / HIP-event duration)"},
+                                       "copy-calibrated; bytes per GEMM stage)",
+                     "kernel": "FP64 MFMA GEMM stage: whole-tile launch + K-split launch of the last round + "
+                               "summing pass (2*M*N*K FLOP / HIP-event duration of the stage)"},
         "cpu_baseline": cpu,
         "secondary": secondary,
     }
