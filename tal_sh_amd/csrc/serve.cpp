@@ -23,6 +23,7 @@
 #include <string>
 #include <vector>
 
+#include "contract.hpp"
 #include "plan.hpp"
 #include "runtime.hpp"
 #include "talsh_amd.h"
@@ -72,19 +73,18 @@ int estimate(talshAmdServer *s, talshAmdServer::Req *q) {
   const int e = tal::plan_contraction(p, r.dtens->shape_p->dims, r.ltens->shape_p->dims, r.rtens->shape_p->dims,
                                       r.accumulative != NOPE, &plan);
   if (e) return TALSH_INVALID_ARGS;
+  int dk = NO_TYPE;
+  for (int i = 0; i < r.dtens->ndev; ++i)
+    if (r.dtens->avail[i] == YEP) dk = r.dtens->data_kind[i];
   const size_t bd = body_bytes(r.dtens), bl = body_bytes(r.ltens), br = body_bytes(r.rtens);
   size_t need = 0;
   if (!resident(r.dtens, s->g)) need += bd;
   if (!resident(r.ltens, s->g)) need += bl;
   if (!resident(r.rtens, s->g)) need += br;
-  const size_t ba = plan.swap ? br : bl, bb = plan.swap ? bl : br;
-  if (plan.a.transpose) need += ba;
-  if (plan.b.transpose) need += bb;
-  if (plan.d_transpose) need += bd;
+  // the TTGT temporaries talshTensorContract will take from the pool
+  const tal::TtgtTemps t = tal::ttgt_temp_bytes(plan, dk, bl, br, bd);
+  need += t.a + t.b + t.d + t.w + t.s;
   q->need = need;
-  int dk = NO_TYPE;
-  for (int i = 0; i < r.dtens->ndev; ++i)
-    if (r.dtens->avail[i] == YEP) dk = r.dtens->data_kind[i];
   q->flops = ((dk == C4 || dk == C8) ? 8.0 : 2.0) * (double)plan.M * (double)plan.N * (double)plan.K *
              (double)plan.batch;
   return TALSH_SUCCESS;
