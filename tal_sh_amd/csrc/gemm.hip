@@ -578,14 +578,24 @@ __global__ __launch_bounds__(512) void gemm_bf16x6p_kernel(int64_t M, int64_t N,
                                                            const uint4 *__restrict__ Ap, int64_t aplane,
                                                            const uint4 *__restrict__ Bp, int64_t bplane,
                                                            float *__restrict__ C, int64_t ldc, float alpha,
-                                                           int beta_one, int64_t gx, int64_t gy) {
-  // aplane / bplane / row strides in 16-byte chunks
+                                                           int beta_one, int64_t gx, int64_t gy, int64_t t0,
+                                                           int64_t count, int ksplit, int64_t ksteps,
+                                                           float *__restrict__ part) {
+  // aplane / bplane / row strides in 16-byte chunks.  ksplit > 1: workgroup
+  // q*ksplit + c computes K steps [c*ksteps, (c+1)*ksteps) of tile t0 + q
+  // into its own PBM x PBN block of `part` (big + small, no alpha, no beta).
   __shared__ uint4 As[2][3][PBM * 4];
   __shared__ uint4 Bs[2][3][PBN * 4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 3, wn = wave >> 2;
-  int64_t bx, by;
-  tile_of(gx, gy, 0, gx * gy, bx, by);
+  int64_t bx, by, kt0 = 0, kt1 = Kp / PBK;
+  if (ksplit > 1) {
+    tile_xy(t0 + (int64_t)blockIdx.x / ksplit, gx, gy, bx, by);
+    kt0 = (int64_t)(blockIdx.x % ksplit) * ksteps;
+    kt1 = min(kt1, kt0 + ksteps);
+  } else {
+    tile_of(gx, gy, t0, count, bx, by);
+  }
   const int64_t kpc = Kp / 8;  // chunks per plane row
   // thread tid loads chunk (tid & 3) of row (tid >> 2) [+128] of each plane
   const int lr = tid >> 2, lc = tid & 3;
@@ -628,14 +638,16 @@ __global__ __launch_bounds__(512) void gemm_bf16x6p_kernel(int64_t M, int64_t N,
   // fragment slot of row 16*x + (lane & 15), chunk lane >> 4: the swizzle of
   // that row is pswz(lane & 15), with bit 1 flipped when x is odd
   const int fs[2] = {pslot(lane & 15, lane >> 4), (lane & 15) * 4 + ((lane >> 4) ^ pswz(lane & 15) ^ 2)};
-  const int64_t nk = Kp / PBK;
-  TAL_PLOAD(0)
-  TAL_PSTORE(0)
+  const int64_t nk = kt1 - kt0;  // may be <= 0 for a trailing K range: its partial sums are zeros
+  if (nk > 0) {
+    TAL_PLOAD(kt0 * (PBK / 8))
+    TAL_PSTORE(0)
+  }
   __syncthreads();
   for (int64_t kt = 0; kt < nk; ++kt) {
     const int cur = (int)(kt & 1);
     if (kt + 1 < nk) {
-      TAL_PLOAD((kt + 1) * (PBK / 8))
+      TAL_PLOAD((kt0 + kt + 1) * (PBK / 8))
     }
     bf16x8 fb[3][4];
 #pragma unroll
@@ -669,6 +681,19 @@ __global__ __launch_bounds__(512) void gemm_bf16x6p_kernel(int64_t M, int64_t N,
 #undef TAL_PG
 #undef TAL_PLOAD
 #undef TAL_PSTORE
+  if (ksplit > 1) {
+    float *pb = part + (int64_t)blockIdx.x * PBM * PBN;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int ml = wm * 64 + i * 16 + 4 * (lane >> 4) + r, nl = wn * 64 + j * 16 + (lane & 15);
+          pb[ml + nl * PBM] = big[i][j][r] + small[i][j][r];
+        }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -854,10 +879,42 @@ int op_code(char t) {
 
 bool gemm_split_bf16_available() { return true; }
 
+// Resident gemm_bf16x6p_kernel workgroups on the current device, cached per device.
+static int64_t bf16p_slots() {
+  static int64_t cache[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+  if (!cache[dev]) {
+    int cus = 0, per = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, gemm_bf16x6p_kernel, 512, 0) != hipSuccess)
+      return 0;
+    cache[dev] = (int64_t)cus * std::max(per, 1);
+  }
+  return cache[dev];
+}
+
+// K split of a mostly idle last round of tiles (as for the FP32/FP64 GEMM).
+static TailSplit bf16p_tail(int64_t M, int64_t N, int64_t K) {
+  TailSplit t;
+  if (M <= 0 || N <= 0) return t;
+  const int64_t T = (round_up(M, PBM) / PBM) * (round_up(N, PBN) / PBN), slots = bf16p_slots();
+  if (slots <= 0) return t;
+  const int64_t rem = T % slots;
+  if (rem == 0 || rem * 4 > slots) return t;
+  const int64_t ks = std::min<int64_t>(std::min<int64_t>(16, slots / rem), (round_up(std::max<int64_t>(K, 1), PBK) / PBK) / 8);
+  if (ks < 2) return t;
+  t.rem = rem;
+  t.ksplit = (int)ks;
+  return t;
+}
+
 size_t gemm_split_workspace(int64_t M, int64_t N, int64_t K) {
   const int64_t Mp = round_up(std::max<int64_t>(M, 1), PBM), Np = round_up(std::max<int64_t>(N, 1), PBN);
   const int64_t Kp = round_up(std::max<int64_t>(K, 1), PBK);
-  return (size_t)(round_up(6 * Mp * Kp, 256) + 6 * Np * Kp);
+  const TailSplit t = bf16p_tail(M, N, K);
+  const size_t tail = t.ksplit > 1 ? (size_t)t.rem * t.ksplit * PBM * PBN * sizeof(float) : 0;
+  return (size_t)(round_up(6 * Mp * Kp, 256) + round_up(6 * Np * Kp, 256)) + tail;
 }
 
 int launch_gemm_split(char ta, char tb, int64_t M, int64_t N, int64_t K, double ar, const void *A, int64_t lda,
@@ -876,9 +933,23 @@ int launch_gemm_split(char ta, char tb, int64_t M, int64_t N, int64_t K, double 
   if (int e = split_operand(opb == 0, (const float *)B, ldb, N, K, Np, Kp, pb, s)) return e;
   const int64_t gx = Mp / PBM, gy = Np / PBN;
   if (gx * gy > 0x7fffffffLL) return -4;
-  gemm_bf16x6p_kernel<<<dim3((unsigned)(gx * gy)), dim3(512), 0, s>>>(
-      M, N, Kp, (const uint4 *)pa, Mp * Kp / 8, (const uint4 *)pb, Np * Kp / 8, (float *)C, ldc, (float)ar,
-      beta_one, gx, gy);
+  // the planes take the front of ws; partial sums of a K-split last round follow them
+  const TailSplit ts = bf16p_tail(M, N, K);
+  float *part = (float *)((char *)ws + round_up(6 * Mp * Kp, 256) + round_up(6 * Np * Kp, 256));
+  const int64_t full = gx * gy - ts.rem, nsteps = Kp / PBK;
+  const int64_t ksteps = ts.ksplit > 1 ? (nsteps + ts.ksplit - 1) / ts.ksplit : 0;
+  if (full > 0)
+    gemm_bf16x6p_kernel<<<dim3((unsigned)full), dim3(512), 0, s>>>(
+        M, N, Kp, (const uint4 *)pa, Mp * Kp / 8, (const uint4 *)pb, Np * Kp / 8, (float *)C, ldc, (float)ar,
+        beta_one, gx, gy, 0, full, 1, 0, nullptr);
+  if (ts.ksplit > 1) {
+    gemm_bf16x6p_kernel<<<dim3((unsigned)(ts.rem * ts.ksplit)), dim3(512), 0, s>>>(
+        M, N, Kp, (const uint4 *)pa, Mp * Kp / 8, (const uint4 *)pb, Np * Kp / 8, (float *)C, ldc, (float)ar,
+        beta_one, gx, gy, full, ts.rem, ts.ksplit, ksteps, part);
+    const int64_t nel = ts.rem * PBM * PBN;
+    gemm_tail_reduce<float, 1, PBM, PBN><<<dim3((unsigned)((nel + kThreads - 1) / kThreads)), dim3(kThreads), 0, s>>>(
+        M, N, part, ts.ksplit, (float *)C, ldc, (float)ar, 0.0f, beta_one, gx, gy, full, ts.rem);
+  }
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
