@@ -129,6 +129,11 @@ size_t talshDeviceBufferFreeSize_(int dev_num, int dev_kind);
 double talshDeviceGetFlops(int dev_kind TALSH_DEFAULT(DEV_DEFAULT), int dev_id TALSH_DEFAULT(DEV_DEFAULT));
 int talshStats(int dev_id TALSH_DEFAULT(-1), int dev_kind TALSH_DEFAULT(DEV_NULL));
 int talshStats_(int dev_id, int dev_kind);
+/* Host memory allocation policy (talsh.h:163-165, talshf.F90:882-893): where new Host tensor bodies come from.
+ * MEM_ALLOC_REGULAR: ordinary (pageable) memory; MEM_ALLOC_TMP_BUF / MEM_ALLOC_ALL_BUF: the pinned Host buffer
+ * given to talshInit, with fallback != 0 letting a full buffer fall back to ordinary memory (otherwise the
+ * construction returns TRY_LATER).  ierr = 0 on success.  Defaults: TMP_BUF with fallback, as in the reference. */
+void talshSetMemAllocPolicyHost(int mem_policy, int fallback, int *ierr);
 /* Fast math (talsh.h:167-175): selects the split-BF16 MFMA GEMM for R4/C4 contractions. */
 int talshEnableFastMath(int dev_kind TALSH_DEFAULT(DEV_DEFAULT), int dev_id TALSH_DEFAULT(DEV_DEFAULT));
 int talshDisableFastMath(int dev_kind TALSH_DEFAULT(DEV_DEFAULT), int dev_id TALSH_DEFAULT(DEV_DEFAULT));

@@ -18,6 +18,7 @@ NO_TYPE, R4, R8, C4, C8 = 0, 4, 8, 14, 18
 DEV_NULL = DEV_DEFAULT = -1
 DEV_HOST, DEV_NVIDIA_GPU, DEV_INTEL_MIC, DEV_AMD_GPU = 0, 1, 2, 3
 NOPE, YEP = 0, 1
+MEM_ALLOC_REGULAR, MEM_ALLOC_TMP_BUF, MEM_ALLOC_ALL_BUF = 0, 1, 2
 TRY_LATER, DEVICE_UNABLE, NOT_CLEAN = -918273645, -546372819, -192837465
 COPY_D, COPY_M, COPY_T, COPY_K = 0, 1, 2, 3
 TALSH_SUCCESS = 0
@@ -124,6 +125,7 @@ SIGNATURES = {
     "talshTensorAdd": (_I, [ctypes.c_char_p, _TP, _TP, _D, _D, _I, _I, _I, _KP]),
     "talshTensorContract": (_I, [ctypes.c_char_p, _TP, _TP, _TP, _D, _D, _I, _I, _I, _I, _KP]),
     "talshTensorContractXL": (_I, [ctypes.c_char_p, _TP, _TP, _TP, _D, _D, _I, _I, _I]),
+    "talshSetMemAllocPolicyHost": (None, [_I, _I, ctypes.POINTER(_I)]),
     "talshTensorContractXL_": (_I, [ctypes.c_char_p, _TP, _TP, _TP, _D, _D, _I, _I, _I]),
     "talshTensorNorm2": (_I, [_TP, _TP, _I, _I, _I, _KP]),
     "talshTensorSlice": (_I, [_TP, _TP, _IP, _I, _I, _I, _I, _KP]),

@@ -157,8 +157,11 @@ void dev_free(int g, void *p, int handle) {
 
 int host_alloc(size_t bytes, void **p, int *handle) {
   Runtime &r = rt();
-  if (r.host_size > 0 && r.host_arena.alloc(bytes, p, handle) == 0) return 0;
-  // fallback to regular memory (TALSH_MEM_ALLOC_FALLBACK_HOST, talsh.h:21)
+  if (r.host_policy != MEM_ALLOC_REGULAR) {
+    if (r.host_size > 0 && r.host_arena.alloc(bytes, p, handle) == 0) return 0;
+    if (!r.host_fallback) return TRY_LATER;
+  }
+  // regular memory (policy MEM_ALLOC_REGULAR, or the fallback of talsh.h:20)
   *p = std::malloc(bytes ? bytes : 1);
   *handle = -1;
   return *p ? 0 : TRY_LATER;
@@ -321,6 +324,9 @@ int talshInit(size_t *host_buf_size, int *host_arg_max, int ngpus, int gpu_list[
   if (r.on) return TALSH_ALREADY_INITIALIZED;
   if (nmics > 0) return TALSH_NOT_AVAILABLE;
   if (ngpus < 0 || namds < 0 || ngpus > MAX_GPUS_PER_NODE || namds > MAX_AMDS_PER_NODE) return TALSH_INVALID_ARGS;
+  // default Host allocation policy (talshc.cpp:602, talsh.h:19-20)
+  r.host_policy = MEM_ALLOC_TMP_BUF;
+  r.host_fallback = 1;
   // GPUs may be listed under either device kind; both lists name HIP device ordinals.
   std::vector<int> list;
   for (int i = 0; i < ngpus; ++i) list.push_back(gpu_list ? gpu_list[i] : i);
@@ -527,6 +533,16 @@ int talshDetermineOptimalDevice(const talsh_tens_t *tens0, const talsh_tens_t *t
 // operations are printed to stdout while on.
 void talshMemManagerLogStart(void) { rt().log_mem = true; }
 void talshMemManagerLogFinish(void) { rt().log_mem = false; }
+void talshSetMemAllocPolicyHost(int mem_policy, int fallback, int *ierr) {
+  std::lock_guard<std::recursive_mutex> lk(rt().mu);
+  const bool ok = mem_policy == MEM_ALLOC_REGULAR || mem_policy == MEM_ALLOC_TMP_BUF || mem_policy == MEM_ALLOC_ALL_BUF;
+  if (ok) {
+    rt().host_policy = mem_policy;
+    rt().host_fallback = fallback != 0;
+  }
+  if (ierr) *ierr = ok ? 0 : 1;
+}
+
 void talshTensorOpLogStart(void) { rt().log_ops = true; }
 void talshTensorOpLogFinish(void) { rt().log_ops = false; }
 

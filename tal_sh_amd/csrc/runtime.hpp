@@ -62,6 +62,7 @@ struct Runtime {
   size_t dev_buffer_request = 0;  // 0: take a fraction of the free memory
   int nstreams = 4;
   size_t xl_mem = 0;  // device bytes a talshTensorContractXL call may use (0: from the free pool)
+  int host_policy = 1, host_fallback = 1;  // talshSetMemAllocPolicyHost (MEM_ALLOC_TMP_BUF, fallback on)
   bool log_mem = false, log_ops = false;  // talshMemManagerLogStart / talshTensorOpLogStart
   std::unordered_map<const void *, Hazard> hazards;
   // bumped by every change of the hazard table: a caller that saw the same

@@ -205,3 +205,35 @@ def test_host_side_errors_without_gpu():
             t.destruct()
     finally:
         assert lib.talshShutdown() == 0
+
+
+def test_host_memory_policy():
+    """talshSetMemAllocPolicyHost (talshf.F90:882-893): Host bodies from the pinned Host buffer or ordinary memory,
+    with or without fallback when the buffer cannot hold them.  No GPU needed (without a GPU driver the pinned
+    buffer cannot be allocated and has size 0, so every body needs the fallback)."""
+    lib = ts.load()
+    hb, ham = ctypes.c_size_t(1 << 20), ctypes.c_int(0)
+    assert lib.talshInit(ctypes.byref(hb), ctypes.byref(ham), 0, None, 0, None, 0, None) == 0
+    err = ctypes.c_int(-1)
+    try:
+        lo = lib.talshDeviceBufferBasePtr(0, ts.DEV_HOST)
+        inside = (lambda t: lo <= t.host_address() < lo + hb.value) if lo else (lambda t: False)
+        small = ts.Tensor(R8, [16, 16], 0)                    # default: Host buffer when there is one
+        assert inside(small) == bool(lo)
+        lib.talshSetMemAllocPolicyHost(ts.MEM_ALLOC_ALL_BUF, 0, ctypes.byref(err))
+        assert err.value == 0
+        with pytest.raises(RuntimeError):                      # larger than the buffer, no fallback
+            ts.Tensor(R8, [1024, 1024], 0)
+        lib.talshSetMemAllocPolicyHost(ts.MEM_ALLOC_TMP_BUF, 1, ctypes.byref(err))
+        big = ts.Tensor(R8, [1024, 1024], 0)                   # falls back to ordinary memory
+        assert not inside(big)
+        lib.talshSetMemAllocPolicyHost(ts.MEM_ALLOC_REGULAR, 0, ctypes.byref(err))
+        reg = ts.Tensor(R8, [16, 16], 0)
+        assert not inside(reg)
+        lib.talshSetMemAllocPolicyHost(7, 0, ctypes.byref(err))
+        assert err.value != 0
+        for t in (small, big, reg):
+            t.destruct()
+    finally:
+        lib.talshSetMemAllocPolicyHost(ts.MEM_ALLOC_TMP_BUF, 1, ctypes.byref(err))
+        assert lib.talshShutdown() == 0
