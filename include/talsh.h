@@ -35,6 +35,36 @@
 #define TALSH_GFLOP_THRESH_GPU 4
 
 /* error codes (talsh.h:25-41) */
+/* Tensor operation kinds and stages (talsh.h:53-82). */
+#define TALSH_TENSOR_NOOP -1
+#define TALSH_TENSOR_INIT 68
+#define TALSH_TENSOR_NORM1 69
+#define TALSH_TENSOR_NORM2 70
+#define TALSH_TENSOR_MIN 71
+#define TALSH_TENSOR_MAX 72
+#define TALSH_TENSOR_FOLD 73
+#define TALSH_TENSOR_UNFOLD 74
+#define TALSH_TENSOR_SLICE 75
+#define TALSH_TENSOR_INSERT 76
+#define TALSH_TENSOR_COPY 77
+#define TALSH_TENSOR_PERMUTE 78
+#define TALSH_TENSOR_SCALE 79
+#define TALSH_TENSOR_ADD 80
+#define TALSH_TENSOR_TRACE 81
+#define TALSH_TENSOR_CONTRACT 82
+#define TALSH_TENSOR_HADAMARD 83
+#define TALSH_TENSOR_KHATRIRAO 84
+#define TALSH_OP_UNDEFINED -1
+#define TALSH_OP_EMPTY 0
+#define TALSH_OP_PARTIAL 1
+#define TALSH_OP_DEFINED 2
+#define TALSH_OP_RESOURCED 3
+#define TALSH_OP_LOADED 4
+#define TALSH_OP_SCHEDULED 5
+#define TALSH_OP_COMPLETED 6
+#define TALSH_OP_STORED 7
+#define TALSH_OP_RETIRED 8
+
 #define TALSH_SUCCESS 0
 #define TALSH_FAILURE -666
 #define TALSH_NOT_AVAILABLE -888
@@ -101,6 +131,33 @@ typedef struct {
   double flops;
   double exec_time;
 } talsh_task_t;
+
+/* Tensor slice view (talsh.h:96-101): a box of an existing tensor, not owning it. */
+typedef struct {
+  talsh_tens_t *tensor;         /* non-owning */
+  talsh_tens_signature_t bases; /* base offsets of the box inside the tensor */
+  talsh_tens_shape_t shape;     /* extents of the box */
+} talsh_tens_slice_t;
+
+/* Basic tensor operation (talsh.h:124-140): slices of existing tensors as formal operands, the actual
+ * operands built for execution, and the stage it has reached (TALSH_OP_*). */
+typedef struct {
+  int opkind;
+  int data_kind;
+  unsigned int num_args;
+  talsh_tens_slice_t tens_slice[MAX_TENSOR_OPERANDS];
+  const char *symb_pattern; /* non-owning */
+  double alpha_real;
+  double alpha_imag;
+  talsh_tens_t tens_arg[MAX_TENSOR_OPERANDS];
+  talsh_task_t task_handle;
+  int exec_dev_id;
+  int stage;
+  double time_started;
+  double time_scheduled;
+  double time_completed;
+  double time_finished;
+} talsh_tens_op_t;
 
 #ifdef __cplusplus
 extern "C" {
@@ -208,6 +265,47 @@ int talshTaskTime(talsh_task_t *talsh_task, double *total, double *comput TALSH_
 int talshTaskTime_(talsh_task_t *talsh_task, double *total, double *comput, double *input, double *output,
                    double *mmul);
 void talshTaskPrint(const talsh_task_t *talsh_task);
+/* Tensor slice views (talsh.h:312-329). */
+int talshTensorSliceCreate(talsh_tens_slice_t **slice);
+int talshTensorSliceClean(talsh_tens_slice_t *slice);
+int talshTensorSliceConstruct(talsh_tens_slice_t *slice, const talsh_tens_t *tensor, const size_t *offsets,
+                              const int *dims, const int *divs TALSH_DEFAULT(NULL), const int *grps TALSH_DEFAULT(NULL));
+size_t talshTensorSliceVolume(const talsh_tens_slice_t *slice);
+int talshTensorSliceDestruct(talsh_tens_slice_t *slice);
+int talshTensorSliceDestroy(talsh_tens_slice_t *slice);
+/* Tensor operations (talsh.h:373-431): a contraction of tensor slices staged as
+ * DEFINED -> RESOURCED (operand tensors on the execution GPU) -> LOADED (input boxes copied in, from a GPU
+ * image by the slice kernel or from the Host image by strided DMA) -> SCHEDULED (talshTensorContract, async)
+ * -> COMPLETED -> STORED (result box accumulated into the destination tensor's image) -> RETIRED.
+ * Only TALSH_TENSOR_CONTRACT executes, as in the reference. */
+int talshTensorOpCreate(talsh_tens_op_t **tens_op);
+int talshTensorOpClean(talsh_tens_op_t *tens_op);
+int talshTensorOpSetArgument(talsh_tens_op_t *tens_op, const talsh_tens_t *tensor, const size_t *offsets,
+                             const int *dims);
+int talshTensorOpSpecify(talsh_tens_op_t *tens_op, int operation_kind, int data_kind,
+                         const char *symbolic_pattern TALSH_DEFAULT(NULL), double prefactor_real TALSH_DEFAULT(1.0),
+                         double prefactor_imag TALSH_DEFAULT(0.0));
+int talshTensorOpSetExecDevice(talsh_tens_op_t *tens_op, int dev_id, int dev_kind TALSH_DEFAULT(DEV_DEFAULT));
+int talshTensorOpActivate(talsh_tens_op_t *tens_op);
+int talshTensorOpLoadInput(talsh_tens_op_t *tens_op);
+int talshTensorOpExecute(talsh_tens_op_t *tens_op, int dev_id TALSH_DEFAULT(DEV_DEFAULT),
+                         int dev_kind TALSH_DEFAULT(DEV_DEFAULT));
+int talshTensorOpTest(talsh_tens_op_t *tens_op, int *completed, int wait TALSH_DEFAULT(NOPE));
+int talshTensorOpStoreOutput(talsh_tens_op_t *tens_op);
+int talshTensorOpDeactivate(talsh_tens_op_t *tens_op);
+int talshTensorOpDestruct(talsh_tens_op_t *tens_op);
+int talshTensorOpDestroy(talsh_tens_op_t *tens_op);
+int talshTensorOpProgress(talsh_tens_op_t *tens_op, int *done);
+size_t talshTensorOpGetArgVolume(const talsh_tens_op_t *tens_op, unsigned int arg_num);
+size_t talshTensorOpGetArgSize(const talsh_tens_op_t *tens_op, unsigned int arg_num);
+double talshTensorOpGetByteCount(const talsh_tens_op_t *tens_op, unsigned int element_size TALSH_DEFAULT(1));
+double talshTensorOpGetFlopCount(const talsh_tens_op_t *tens_op);
+double talshTensorOpGetIntensity(const talsh_tens_op_t *tens_op);
+/* Splits a contraction in two (talsh.h:427-430): halves the largest extent of the batch (hyper) indices if
+ * any, else of the largest of the right, left and contracted index groups (by volume, in that order of
+ * preference), as the reference does (talshc.cpp:3150-3317), so callers get the same sub-operations. */
+int talshTensorOpDecompose2(const talsh_tens_op_t *tens_op, talsh_tens_op_t *child_op1, talsh_tens_op_t *child_op2);
+void talshTensorOpPrint(const talsh_tens_op_t *tens_op);
 
 /* ---- placement (talsh.h:437-453) ---- */
 int talshTensorPlace(talsh_tens_t *tens, int dev_id, int dev_kind TALSH_DEFAULT(DEV_NULL),

@@ -191,6 +191,10 @@ extern "C" {
  * characters followed by a NUL; ierr = 1 for negative ranks. */
 void get_contr_pattern_sym(const int *rank_left, const int *rank_right, const int *conj_bits, const int *cptrn_dig,
                            char *cptrn_sym, int *cpl, int *ierr);
+/* Initialiser that leaves a new body as allocated (tensor_algebra.h:256): with it talshTensorConstruct skips
+ * the initialisation, on the Host and on GPUs. */
+int talsh_tens_no_init(const talsh_tens_data_t *tens_data, const talsh_tens_shape_t *tens_shape,
+                       const talsh_tens_signature_t *tens_signature);
 #ifdef __cplusplus
 }
 #endif

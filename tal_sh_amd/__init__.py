@@ -19,6 +19,8 @@ DEV_NULL = DEV_DEFAULT = -1
 DEV_HOST, DEV_NVIDIA_GPU, DEV_INTEL_MIC, DEV_AMD_GPU = 0, 1, 2, 3
 NOPE, YEP = 0, 1
 MEM_ALLOC_REGULAR, MEM_ALLOC_TMP_BUF, MEM_ALLOC_ALL_BUF = 0, 1, 2
+TALSH_TENSOR_CONTRACT = 82
+TALSH_OP_EMPTY, TALSH_OP_PARTIAL, TALSH_OP_DEFINED, TALSH_OP_RETIRED = 0, 1, 2, 8
 TRY_LATER, DEVICE_UNABLE, NOT_CLEAN = -918273645, -546372819, -192837465
 COPY_D, COPY_M, COPY_T, COPY_K = 0, 1, 2, 3
 TALSH_SUCCESS = 0
@@ -61,6 +63,28 @@ class TalshTask(ctypes.Structure):
                 ("data_kind", ctypes.c_int), ("coherence", ctypes.c_int), ("num_args", ctypes.c_int),
                 ("tens_args", TalshTensArg * 4), ("data_vol", ctypes.c_double), ("flops", ctypes.c_double),
                 ("exec_time", ctypes.c_double)]
+
+
+class TensSignature(ctypes.Structure):
+    _fields_ = [("num_dim", ctypes.c_int), ("offsets", ctypes.POINTER(ctypes.c_size_t))]
+
+
+class TensShape(ctypes.Structure):
+    _fields_ = [("num_dim", ctypes.c_int), ("dims", ctypes.POINTER(ctypes.c_int)),
+                ("divs", ctypes.POINTER(ctypes.c_int)), ("grps", ctypes.POINTER(ctypes.c_int))]
+
+
+class TalshTensSlice(ctypes.Structure):
+    _fields_ = [("tensor", ctypes.POINTER(TalshTens)), ("bases", TensSignature), ("shape", TensShape)]
+
+
+class TalshTensOp(ctypes.Structure):
+    _fields_ = [("opkind", ctypes.c_int), ("data_kind", ctypes.c_int), ("num_args", ctypes.c_uint),
+                ("tens_slice", TalshTensSlice * 4), ("symb_pattern", ctypes.c_char_p),
+                ("alpha_real", ctypes.c_double), ("alpha_imag", ctypes.c_double), ("tens_arg", TalshTens * 4),
+                ("task_handle", TalshTask), ("exec_dev_id", ctypes.c_int), ("stage", ctypes.c_int),
+                ("time_started", ctypes.c_double), ("time_scheduled", ctypes.c_double),
+                ("time_completed", ctypes.c_double), ("time_finished", ctypes.c_double)]
 
 
 class DevRsc(ctypes.Structure):

@@ -31,6 +31,12 @@ int enqueue_ttgt(const DigitalPattern &p, const ContractPlan &plan, int dk, bool
                  void *D, const void *L, const void *R, const int *ddims, const int *ldims, const int *rdims,
                  const TtgtBuffers &tmp, hipStream_t s, hipEvent_t gemm_start, hipEvent_t gemm_end);
 
+// Copies the box [off, off + sd) of a dlf tensor with extents bd (element
+// size esz) between a host body and a packed device buffer (h2d: host to
+// device) with strided DMA copies on stream s.  Pageable or pinned host memory.
+int box_dma(bool h2d, char *host, char *dev, int rank, const int *bd, const int *off, const int *sd, size_t esz,
+            hipStream_t s);
+
 // Execution GPU index for (dev_id, dev_kind); operands decide when the caller
 // leaves the choice to the runtime.  Host execution is not provided.
 int exec_gpu(int dev_id, int dev_kind, std::initializer_list<const talsh_tens_t *> tens, int *g);

@@ -533,6 +533,12 @@ int talshDetermineOptimalDevice(const talsh_tens_t *tens0, const talsh_tens_t *t
 // operations are printed to stdout while on.
 void talshMemManagerLogStart(void) { rt().log_mem = true; }
 void talshMemManagerLogFinish(void) { rt().log_mem = false; }
+int talsh_tens_no_init(const talsh_tens_data_t *tens_data, const talsh_tens_shape_t *tens_shape,
+                       const talsh_tens_signature_t *tens_signature) {
+  (void)tens_data, (void)tens_shape, (void)tens_signature;
+  return 0;
+}
+
 void talshSetMemAllocPolicyHost(int mem_policy, int fallback, int *ierr) {
   std::lock_guard<std::recursive_mutex> lk(rt().mu);
   const bool ok = mem_policy == MEM_ALLOC_REGULAR || mem_policy == MEM_ALLOC_TMP_BUF || mem_policy == MEM_ALLOC_ALL_BUF;
@@ -695,7 +701,8 @@ int talshTensorConstruct(talsh_tens_t *t, int data_kind, int tens_rank, const in
     }
     return TALSH_SUCCESS;
   }
-  // device body: initialise on the device (blocking, as the reference does)
+  // device body: initialise on the device (blocking, as the reference does), unless asked not to
+  if (init_method == talsh_tens_no_init) return TALSH_SUCCESS;
   if (talshTensorInit(t, init_val_real, init_val_imag, cdev, DEV_NULL, COPY_M, nullptr) != TALSH_SUCCESS)
     return NOT_CLEAN;
   return TALSH_SUCCESS;

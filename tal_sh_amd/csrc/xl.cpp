@@ -31,7 +31,6 @@
 #include "talsh_amd.h"
 
 namespace tal {
-namespace {
 
 // Copies the box [off, off + sd) of a dlf tensor with extents bd between a
 // host body and a packed device buffer.  The box is cut into rows of
@@ -89,6 +88,8 @@ int box_dma(bool h2d, char *host, char *dev, int rank, const int *bd, const int 
   }
   return TALSH_SUCCESS;
 }
+
+namespace {
 
 // Where an operand's body is read from (or, for D, kept) during the call.
 struct Home {

@@ -50,6 +50,32 @@ def test_struct_layouts_match_reference_abi():
     assert ctypes.sizeof(ts.DevRsc) == 24
     assert ts.TalshTask.data_vol.offset == 96 and ts.TalshTask.exec_time.offset == 112
     assert ts.TalshTens.ndev.offset == 36 and ts.DevRsc.mem_attached.offset == 20
+    # slice view and tensor operation (talsh.h:96-101, :124-140)
+    assert ctypes.sizeof(ts.TalshTensSlice) == 56 and ctypes.sizeof(ts.TalshTensOp) == 584
+    assert ts.TalshTensOp.task_handle.offset == 424 and ts.TalshTensOp.stage.offset == 548
+
+
+def test_header_struct_layouts_compiled(tmp_path):
+    """The C headers give the structs the layout of the ctypes mirrors above (compiled with gcc)."""
+    import subprocess
+    src = tmp_path / "layout.c"
+    src.write_text("""#include <stdio.h>
+#include <stddef.h>
+#include "talsh.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu %zu\\n", sizeof(talsh_tens_t), sizeof(talsh_task_t), sizeof(talsh_tens_slice_t),
+         sizeof(talsh_tens_op_t), offsetof(talsh_tens_op_t, task_handle), offsetof(talsh_tens_op_t, stage),
+         offsetof(talsh_tens_op_t, time_finished));
+  return 0;
+}
+""")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(REPO, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    want = [ctypes.sizeof(ts.TalshTens), ctypes.sizeof(ts.TalshTask), ctypes.sizeof(ts.TalshTensSlice),
+            ctypes.sizeof(ts.TalshTensOp), ts.TalshTensOp.task_handle.offset, ts.TalshTensOp.stage.offset,
+            ts.TalshTensOp.time_finished.offset]
+    assert got == want, (got, want)
 
 
 def test_header_constants():
