@@ -216,7 +216,7 @@ def bench_hyper(oracle, steps=5, warmup=1, seed=400):
 
 
 def bench_transposes(oracle, steps):
-    """Config 2: rank-6 R8 16^6, identity + reversal + 32 seeded permutations; add; norm2."""
+    """Config 2: rank-6 R8 16^6, identity + reversal + 32 seeded permutations; add; norm2; slice and insert of a quarter box."""
     import random
 
     dims = [16] * 6
@@ -269,13 +269,34 @@ def bench_transposes(oracle, steps):
         tms.append(tk.times()["comput"])
         tk.destruct()
     norm_gbs = nbytes / float(np.median(tms[1:])) / 1e9
-    for t in (l, d, s):
+    # slice / insert of an 8 x 16^4 x 8 box (1/4 of the tensor, offsets 4 and 8 on the outer dims)
+    sdims, off = [8, 16, 16, 16, 16, 8], [4, 0, 0, 0, 0, 8]
+    sbytes = int(np.prod(sdims)) * 8
+    sm = ts.Tensor(R8, sdims, ts.gpu_flat(0))
+    tms = []
+    for _ in range(steps + 1):
+        tk = ts.Task()
+        ts.check(ts.slice(sm, l, off, task=tk), "slice")
+        tk.wait()
+        tms.append(tk.times()["comput"])
+        tk.destruct()
+    slice_gbs = 2 * sbytes / float(np.median(tms[1:])) / 1e9
+    tms = []
+    for _ in range(steps + 1):
+        tk = ts.Task()
+        ts.check(ts.insert(d, sm, off, accumulative=ts.YEP, task=tk), "insert")
+        tk.wait()
+        tms.append(tk.times()["comput"])
+        tk.destruct()
+    insert_gbs = 3 * sbytes / float(np.median(tms[1:])) / 1e9
+    for t in (l, d, s, sm):
         t.destruct()
     lib.talshAmdDeviceSynchronize(0)
     return {"copy_gbs_mean": float(np.mean(copy_gbs)), "copy_gbs_min": float(np.min(copy_gbs)),
             "copy_gbs_identity": copy_gbs[0], "copy_gbs_reversal": copy_gbs[1],
             "copy_frac_hbm_mean": float(np.mean(copy_gbs)) / HBM_PEAK_GBS, "copy_bit_exact": bool(exact),
-            "add_gbs": add_gbs, "norm2_gbs": norm_gbs, "permutations": len(perms)}
+            "add_gbs": add_gbs, "norm2_gbs": norm_gbs, "permutations": len(perms),
+            "slice_gbs": slice_gbs, "insert_accumulate_gbs": insert_gbs}
 
 
 def batch_contraction(rng, lo=2, hi=24, max_vol=400000):
