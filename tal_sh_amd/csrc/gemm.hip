@@ -335,12 +335,18 @@ __global__ __launch_bounds__(kThreads) void gemm_tail_reduce(int64_t M, int64_t 
 // Heinecke, "Leveraging the bfloat16 Artificial Intelligence Datatype For
 // Higher-Precision Computations", ARITH 2019).  The dropped terms are below
 // 2^-24 relative; with FP32 accumulation the normwise error at K = 9216 is a
-// few 1e-7 (cf. ~1e-6 for the FP32 MFMA path).  Tiles are 256 x 256: the
-// split costs VALU work per element staged, so a large tile keeps both that
-// and the L2 traffic per MFMA low.  FP32 tiles come from HBM as
-// for the FP32 kernel; the split happens once per element when the tile is
-// written to LDS (three BF16 planes, rows padded to 80 bytes so the 16-byte
-// fragment reads of 16 lanes hit all 64 banks once).  MFMA 16x16x32 bf16
+// few 1e-7 (cf. ~1e-6 for the FP32 MFMA path).  Two kernels implement it:
+//  - gemm_bf16x6_kernel (launch_gemm with R4_SPLIT_BF16, no workspace):
+//    128 x 128 tiles; FP32 tiles come from HBM as for the FP32 kernel and the
+//    split happens once per element when the tile is written to LDS (three
+//    BF16 planes, rows padded to 80 bytes so the 16-byte fragment reads of 16
+//    lanes hit all 64 banks once);
+//  - gemm_bf16x6p_kernel (launch_gemm_split, used by talshTensorContract):
+//    both operands are split once into swizzled BF16 planes in a workspace by
+//    split_planes_kernel, then 256 x 128 tiles of 8 waves stream the planes
+//    with no VALU split work in the K loop; the last, mostly idle round of
+//    tiles is split along K into partials summed by gemm_tail_reduce.
+// MFMA 16x16x32 bf16
 // layout: A lane l holds A[l%16][8*(l/16)..+7], B lane l holds
 // B[8*(l/16)..+7][l%16], accumulator register r of lane l holds
 // C[4*(l/16)+r][l%16].
