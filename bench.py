@@ -711,7 +711,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--only", default="",
-                    help="comma list of parts to run (head,r4,bf16,c8,config1,transpose,batch,hyper,xl,serve)")
+                    help="comma list of parts to run (head,r4,bf16,fp16x3,c8,config1,transpose,batch,hyper,xl,serve)")
     ap.add_argument("--serve-requests", type=int, default=4000, help="serving stream length (full: 20000)")
     ap.add_argument("--serve-rate", type=float, default=None, help="arrivals per second (default 0.8 x capacity)")
     ap.add_argument("--serve-streams", type=int, default=16, help="streams per GPU for the serving run")
@@ -732,11 +732,13 @@ def main():
     secondary = {}
     if want("r4"):
         secondary["r4_fp32_config3"] = bench_contraction(oracle, R4, pat3, 96, max(2, args.steps // 2), 1)
-    if want("bf16"):
-        ts.load().talshEnableFastMath(ts.DEV_NVIDIA_GPU, 0)
-        if ts.load().talshQueryFastMath(ts.DEV_NVIDIA_GPU, 0):
-            secondary["r4_split_bf16_config3"] = bench_contraction(oracle, R4, pat3, 96, max(2, args.steps // 2), 1)
-        ts.load().talshDisableFastMath(ts.DEV_NVIDIA_GPU, 0)
+    for name, method in (("bf16", ts.SPLIT_BF16X6), ("fp16x3", ts.SPLIT_FP16X3)):
+        if want(name):
+            if ts.fast_math(True, method) == 0:
+                key = "r4_split_bf16_config3" if name == "bf16" else "r4_split_fp16x3_config3"
+                secondary[key] = bench_contraction(oracle, R4, pat3, 96, max(2, args.steps // 2), 1)
+            ts.fast_math(False)
+            ts.load().talshAmdSetFastMathSplit(ts.SPLIT_BF16X6)
     if want("c8"):
         secondary["c8_config3"] = bench_contraction(oracle, C8, pat3, 96, max(2, args.steps // 4), 1)
     if want("config1"):

@@ -17,9 +17,18 @@ extern "C" {
 #endif
 
 /* Fast math (talshEnableFastMath etc., declared in talsh.h): on this build
- * "fast math" selects the 3-term split-BF16 MFMA GEMM for R4 contractions
- * (FP32 accumulation); disabled, R4/C4 use the FP32 MFMA GEMM.  R8/C8
- * always use the FP64 MFMA GEMM. */
+ * "fast math" selects a split GEMM on lower-precision MFMA for R4
+ * contractions (FP32 accumulation); disabled, R4/C4 use the FP32 MFMA GEMM.
+ * R8/C8 always use the FP64 MFMA GEMM.  The split method is one of: */
+#define TALSH_AMD_SPLIT_BF16X6 0 /* default: 3-term BF16 split, 6 products; no range limit */
+#define TALSH_AMD_SPLIT_FP16X3 1 /* rows/columns scaled by powers of two, 2-term FP16 split, 3 products; \
+                                    small elements keep 2^-25 of their row's largest magnitude */
+
+/* Selects the split method used while fast math is enabled (default
+ * TALSH_AMD_SPLIT_BF16X6).  May be called at any time; applies to
+ * contractions issued afterwards. */
+int talshAmdSetFastMathSplit(int method);
+int talshAmdGetFastMathSplit(void);
 
 /* Size in bytes of the per-GPU buffer pool taken at talshInit (default: 80 %
  * of the free device memory).  Must be called before talshInit. */

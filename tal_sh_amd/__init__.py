@@ -19,6 +19,8 @@ DEV_NULL = DEV_DEFAULT = -1
 DEV_HOST, DEV_NVIDIA_GPU, DEV_INTEL_MIC, DEV_AMD_GPU = 0, 1, 2, 3
 NOPE, YEP = 0, 1
 MEM_ALLOC_REGULAR, MEM_ALLOC_TMP_BUF, MEM_ALLOC_ALL_BUF = 0, 1, 2
+# fast-math split methods of the R4 GEMM (include/talsh_amd.h)
+SPLIT_BF16X6, SPLIT_FP16X3 = 0, 1
 TALSH_TENSOR_CONTRACT = 82
 TALSH_OP_EMPTY, TALSH_OP_PARTIAL, TALSH_OP_DEFINED, TALSH_OP_RETIRED = 0, 1, 2, 8
 TRY_LATER, DEVICE_UNABLE, NOT_CLEAN = -918273645, -546372819, -192837465
@@ -164,6 +166,8 @@ SIGNATURES = {
     "talshAmdSetDeviceBufferSize": (_I, [_S]),
     "talshAmdSetNumStreams": (_I, [_I]),
     "talshAmdSetContractXLMemory": (_I, [_S]),
+    "talshAmdSetFastMathSplit": (_I, [_I]),
+    "talshAmdGetFastMathSplit": (_I, []),
     "talshAmdDeviceSynchronize": (_I, [_I]),
     "talshAmdBuildInfo": (_I, [ctypes.c_char_p, _I]),
     "talshAmdBatchCreate": (_I, [ctypes.POINTER(_P)]),
@@ -441,6 +445,15 @@ def contract_xl(pattern, d, l, r, alpha=1.0, dev_id=DEV_DEFAULT, dev_kind=DEV_DE
     a = complex(alpha)
     return load().talshTensorContractXL(pattern.encode(), d.ptr, l.ptr, r.ptr, a.real, a.imag, dev_id, dev_kind,
                                         accumulative)
+
+
+def fast_math(enable, split=SPLIT_BF16X6):
+    """talshEnableFastMath / talshDisableFastMath on the GPUs, with the split method of the R4 GEMM
+    (talshAmdSetFastMathSplit); returns the status of the call that failed, else 0."""
+    lib = load()
+    if not enable:
+        return lib.talshDisableFastMath(DEV_NVIDIA_GPU, DEV_DEFAULT)
+    return lib.talshAmdSetFastMathSplit(split) or lib.talshEnableFastMath(DEV_NVIDIA_GPU, DEV_DEFAULT)
 
 
 def set_xl_memory(nbytes):

@@ -508,24 +508,43 @@ __global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(2)
 }
 
 // ---------------------------------------------------------------------------
-// Pre-split bf16x6 GEMM.  split_planes_kernel writes the three BF16 planes of
-// an operand once into a workspace (row r, column k of a plane at r*Kp + k;
-// rows padded to the GEMM tile, K padded to 32, zeros in the padding), so the
-// GEMM's inner loop only moves bytes and issues MFMAs: no split arithmetic
-// (which the on-the-fly kernel above repeats for every C tile that reads an
-// element) and no bounds checks.  The dominant product ah*bh goes to one
-// accumulator and the five small ones to a second, so the main accumulator is
-// rounded once per K step of 32 without any VALU adds; the two are summed in
-// the epilogue.
+// Pre-split GEMMs.  split_planes_kernel writes the planes of an operand once
+// into a workspace (row r, column k of a plane at r*Kp + k; rows padded to the
+// GEMM tile, K padded to 32, zeros in the padding), so the GEMM's inner loop
+// only moves bytes and issues MFMAs: no split arithmetic (which the on-the-fly
+// kernel above repeats for every C tile that reads an element) and no bounds
+// checks.  The dominant product of the leading terms goes to one accumulator
+// and the small products to a second, so the main accumulator is rounded once
+// per K step of 32 without any VALU adds; the two are summed in the epilogue.
+//
+// Two splits share the kernel body (NP = planes per operand):
+//  - NP = 3, bf16x6: three BF16 planes h, m, l (x = h + m + l exactly) and six
+//    products per block, as described above;
+//  - NP = 2, fp16x3: every row of op(A) and every column of op(B) is scaled
+//    by a power of two 2^-e, e the exponent of the row's largest magnitude
+//    (so the row lies in (-1, 1)), then split into two FP16 planes
+//    h = fp16(y), m = fp16(y - h) (y - h is exact in FP32).  FP16 keeps 11
+//    significant bits, so |y - h - m| <= 2^-22 |y| for normal m, and the
+//    three products ah*bh + ah*bm + am*bh (v_mfma_f32_16x16x32_f16, the same
+//    rate as BF16) carry about 22 bits of each operand: normwise relative
+//    error a few 1e-7, like bf16x6, for half of its MFMA work.  The dropped
+//    am*bm is below 2^-22 relative.  An element below 2^-24 of its row's
+//    largest keeps only an absolute accuracy of 2^-25 of that largest (FP16
+//    subnormals), so rows that mix magnitudes over more than about 2^10 lose
+//    relative accuracy on their small elements; bf16x6, whose planes keep
+//    FP32's exponent range, has no such limit.  The epilogue multiplies by
+//    2^(ea + eb), which is exact.
 //
 // Block tile 256 x 128, 8 waves of 64 x 64 (two per SIMD), BK = 32.  LDS
-// holds two stages of both operands (2 x (48 + 24) KB); a plane row is 64 bytes
-// (four 16-byte chunks) and chunk c of row r is stored at slot
+// holds two stages of both operands (2 x (16 + 8) KB per plane); a plane row
+// is 64 bytes (four 16-byte chunks) and chunk c of row r is stored at slot
 // c ^ (((r >> 1) ^ (r >> 3)) & 3), so the 16-byte fragment reads are free of
 // bank conflicts whether the LDS serves them 8 lanes over 32 banks or 16 lanes
 // over 64 banks per cycle.
 // The next K step is loaded into registers while the current one is multiplied;
 // one barrier per K step.
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
 __device__ inline void split3_finite(float x, __bf16 &h, __bf16 &m, __bf16 &l) {
   h = (__bf16)x;
   if (!__builtin_isfinite(x)) {  // Inf/NaN carried by the leading term alone
@@ -537,13 +556,68 @@ __device__ inline void split3_finite(float x, __bf16 &h, __bf16 &m, __bf16 &l) {
   l = (__bf16)(r - (float)m);
 }
 
+// Exponent e with max * 2^-e in [0.5, 1) for the bit pattern u of a row's
+// largest magnitude; 0 for an all-zero row or one holding Inf/NaN.
+__device__ inline int row_exp(uint32_t u) {
+  const float v = __uint_as_float(u);
+  if (!(v > 0.0f) || !__builtin_isfinite(v)) return 0;
+  int e;
+  (void)frexpf(v, &e);
+  return e;
+}
+
+__device__ inline void split2_scaled(float x, int e, _Float16 &h, _Float16 &m) {
+  const float y = ldexpf(x, -e);
+  h = (_Float16)y;
+  m = __builtin_isfinite(y) ? (_Float16)(y - (float)h) : (_Float16)0.0f;  // Inf/NaN: leading term alone
+}
+
+__global__ __launch_bounds__(256) void fill_u32_kernel(uint32_t *__restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+// U[r] = max over k < K of the bit pattern of |X(r, k)| (for non-negative
+// floats the pattern orders as the value; NaN patterns sort above Inf), U
+// zeroed beforehand.  Block = 4 waves; grid.x = row groups, grid.y = K
+// segments of kseg.  KC = 1 (X(r, k) at X[r*ld + k]): one wave per row, lanes
+// along k.  KC = 0 (X(r, k) at X[r + k*ld]): lanes along r, the four waves
+// share the segment.
+template <int KC>
+__global__ __launch_bounds__(256) void row_absmax_kernel(const float *__restrict__ X, int64_t ld, int64_t R,
+                                                         int64_t K, int64_t kseg, uint32_t *__restrict__ U) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t k0 = (int64_t)blockIdx.y * kseg, k1 = min(K, k0 + kseg);
+  uint32_t m = 0;
+  if (KC) {
+    const int64_t r = (int64_t)blockIdx.x * 4 + wave;
+    if (r < R) {
+      const float *x = X + r * ld;
+#pragma unroll 8
+      for (int64_t k = k0 + lane; k < k1; k += 64) m = max(m, __float_as_uint(x[k]) & 0x7fffffffu);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o));
+    if (lane == 0 && r < R && m) atomicMax(U + r, m);
+  } else {
+    const int64_t r = (int64_t)blockIdx.x * 64 + lane;
+    if (r < R) {
+#pragma unroll 8
+      for (int64_t k = k0 + wave; k < k1; k += 4) m = max(m, __float_as_uint(X[r + k * ld]) & 0x7fffffffu);
+      if (m) atomicMax(U + r, m);
+    }
+  }
+}
+
 // KC = 1: source element (r, k) at X[r*ld + k]; KC = 0: at X[r + k*ld].
 // One thread per 8 consecutive k of one row (KC = 1: lanes along k, KC = 0:
-// lanes along r, so source reads are coalesced either way).
-template <int KC>
+// lanes along r, so source reads are coalesced either way).  NP = 3: BF16
+// planes; NP = 2: FP16 planes of the row scaled by the exponent from U.
+template <int KC, int NP>
 __global__ __launch_bounds__(256) void split_planes_kernel(const float *__restrict__ X, int64_t ld, int64_t R,
                                                            int64_t K, int64_t Rp, int64_t Kp,
-                                                           __bf16 *__restrict__ P, int64_t plane, int vec) {
+                                                           uint16_t *__restrict__ P, int64_t plane, int vec,
+                                                           const uint32_t *__restrict__ U) {
   const int64_t kcn = Kp / 8;
   const int64_t item = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (item >= Rp * kcn) return;
@@ -562,17 +636,28 @@ __global__ __launch_bounds__(256) void split_planes_kernel(const float *__restri
       v[i] = (r < R && k < K) ? (KC ? X[r * ld + k] : X[r + k * ld]) : 0.0f;
     }
   }
-  __bf16 h[8], m[8], l[8];
-#pragma unroll
-  for (int i = 0; i < 8; ++i) split3_finite(v[i], h[i], m[i], l[i]);
   uint4 u;
   const int64_t off = r * Kp + k0;
-  __builtin_memcpy(&u, h, 16);
-  *reinterpret_cast<uint4 *>(P + off) = u;
-  __builtin_memcpy(&u, m, 16);
-  *reinterpret_cast<uint4 *>(P + plane + off) = u;
-  __builtin_memcpy(&u, l, 16);
-  *reinterpret_cast<uint4 *>(P + 2 * plane + off) = u;
+  if constexpr (NP == 3) {
+    __bf16 h[8], m[8], l[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) split3_finite(v[i], h[i], m[i], l[i]);
+    __builtin_memcpy(&u, h, 16);
+    *reinterpret_cast<uint4 *>(P + off) = u;
+    __builtin_memcpy(&u, m, 16);
+    *reinterpret_cast<uint4 *>(P + plane + off) = u;
+    __builtin_memcpy(&u, l, 16);
+    *reinterpret_cast<uint4 *>(P + 2 * plane + off) = u;
+  } else {
+    const int e = r < R ? row_exp(U[r]) : 0;
+    _Float16 h[8], m[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) split2_scaled(v[i], e, h[i], m[i]);
+    __builtin_memcpy(&u, h, 16);
+    *reinterpret_cast<uint4 *>(P + off) = u;
+    __builtin_memcpy(&u, m, 16);
+    *reinterpret_cast<uint4 *>(P + plane + off) = u;
+  }
 }
 
 constexpr int PBM = 256, PBN = 128, PBK = 32;
@@ -580,18 +665,30 @@ constexpr int PBM = 256, PBN = 128, PBK = 32;
 __device__ inline int pswz(int r) { return ((r >> 1) ^ (r >> 3)) & 3; }
 __device__ inline int pslot(int r, int c) { return r * 4 + (c ^ pswz(r)); }
 
-__global__ __launch_bounds__(512) void gemm_bf16x6p_kernel(int64_t M, int64_t N, int64_t Kp,
-                                                           const uint4 *__restrict__ Ap, int64_t aplane,
-                                                           const uint4 *__restrict__ Bp, int64_t bplane,
-                                                           float *__restrict__ C, int64_t ldc, float alpha,
-                                                           int beta_one, int64_t gx, int64_t gy, int64_t t0,
-                                                           int64_t count, int ksplit, int64_t ksteps,
-                                                           float *__restrict__ part) {
-  // aplane / bplane / row strides in 16-byte chunks.  ksplit > 1: workgroup
-  // q*ksplit + c computes K steps [c*ksteps, (c+1)*ksteps) of tile t0 + q
-  // into its own PBM x PBN block of `part` (big + small, no alpha, no beta).
-  __shared__ uint4 As[2][3][PBM * 4];
-  __shared__ uint4 Bs[2][3][PBN * 4];
+template <int NP>
+__device__ inline f32x4 pmfma(uint4 a, uint4 b, f32x4 c) {
+  if constexpr (NP == 3)
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                   0, 0, 0);
+  else
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c, 0,
+                                                  0, 0);
+}
+
+// aplane / bplane / row strides in 16-byte chunks.  ksplit > 1: workgroup
+// q*ksplit + c computes K steps [c*ksteps, (c+1)*ksteps) of tile t0 + q into
+// its own PBM x PBN block of `part` (big + small, scaled by 2^(ea + eb) for
+// NP = 2, no alpha, no beta).  UA / UB (NP = 2): row maxima of op(A) and
+// column maxima of op(B), Mp and Np entries.
+template <int NP>
+__device__ __forceinline__ void presplit_gemm(int64_t M, int64_t N, int64_t Kp, const uint4 *__restrict__ Ap,
+                                              int64_t aplane, const uint4 *__restrict__ Bp, int64_t bplane,
+                                              float *__restrict__ C, int64_t ldc, float alpha, int beta_one,
+                                              int64_t gx, int64_t gy, int64_t t0, int64_t count, int ksplit,
+                                              int64_t ksteps, float *__restrict__ part,
+                                              const uint32_t *__restrict__ UA, const uint32_t *__restrict__ UB) {
+  __shared__ uint4 As[2][NP][PBM * 4];
+  __shared__ uint4 Bs[2][NP][PBN * 4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 3, wn = wave >> 2;
   int64_t bx, by, kt0 = 0, kt1 = Kp / PBK;
@@ -611,7 +708,7 @@ __global__ __launch_bounds__(512) void gemm_bf16x6p_kernel(int64_t M, int64_t N,
   const uint32_t voff = (uint32_t)((lr * kpc + lc) * 16);
   const int ls = pslot(lr, lc);
   const int64_t a1 = 128 * kpc;
-  // prefetch registers: A planes t = 0,1,2 rows lr and lr+128; B planes t = 0,1,2 row lr
+  // prefetch registers: A planes t rows lr and lr+128; B planes t row lr
   uint4 a0, a1r, a2, a3, a4, a5, b0, b1, b2;
 #define TAL_PG(base, chunk) (*reinterpret_cast<const uint4 *>((base) + (chunk) * 16 + voff))
 #define TAL_PLOAD(kc)                                                                  \
@@ -619,21 +716,25 @@ __global__ __launch_bounds__(512) void gemm_bf16x6p_kernel(int64_t M, int64_t N,
   a1r = TAL_PG(ag, a1 + (kc));                                                         \
   a2 = TAL_PG(ag, aplane + (kc));                                                      \
   a3 = TAL_PG(ag, aplane + a1 + (kc));                                                 \
-  a4 = TAL_PG(ag, 2 * aplane + (kc));                                                  \
-  a5 = TAL_PG(ag, 2 * aplane + a1 + (kc));                                             \
+  if constexpr (NP == 3) {                                                             \
+    a4 = TAL_PG(ag, 2 * aplane + (kc));                                                \
+    a5 = TAL_PG(ag, 2 * aplane + a1 + (kc));                                           \
+  }                                                                                    \
   b0 = TAL_PG(bg, (kc));                                                               \
   b1 = TAL_PG(bg, bplane + (kc));                                                      \
-  b2 = TAL_PG(bg, 2 * bplane + (kc));
+  if constexpr (NP == 3) b2 = TAL_PG(bg, 2 * bplane + (kc));
 #define TAL_PSTORE(st)                                                                 \
   As[st][0][ls] = a0;                                                                  \
   As[st][0][ls + 512] = a1r;                                                           \
   As[st][1][ls] = a2;                                                                  \
   As[st][1][ls + 512] = a3;                                                            \
-  As[st][2][ls] = a4;                                                                  \
-  As[st][2][ls + 512] = a5;                                                            \
+  if constexpr (NP == 3) {                                                             \
+    As[st][NP - 1][ls] = a4;                                                           \
+    As[st][NP - 1][ls + 512] = a5;                                                     \
+  }                                                                                    \
   Bs[st][0][ls] = b0;                                                                  \
   Bs[st][1][ls] = b1;                                                                  \
-  Bs[st][2][ls] = b2;
+  if constexpr (NP == 3) Bs[st][NP - 1][ls] = b2;
 
   f32x4 big[4][4], small[4][4];
 #pragma unroll
@@ -655,29 +756,30 @@ __global__ __launch_bounds__(512) void gemm_bf16x6p_kernel(int64_t M, int64_t N,
     if (kt + 1 < nk) {
       TAL_PLOAD((kt0 + kt + 1) * (PBK / 8))
     }
-    bf16x8 fb[3][4];
+    uint4 fb[NP][4];
 #pragma unroll
-    for (int t = 0; t < 3; ++t)
+    for (int t = 0; t < NP; ++t)
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-        fb[t][j] = __builtin_bit_cast(bf16x8, Bs[cur][t][(wn * 64 + j * 16) * 4 + fs[j & 1]]);
+      for (int j = 0; j < 4; ++j) fb[t][j] = Bs[cur][t][(wn * 64 + j * 16) * 4 + fs[j & 1]];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      bf16x8 fa[3];
+      uint4 fa[NP];
 #pragma unroll
-      for (int t = 0; t < 3; ++t) fa[t] = __builtin_bit_cast(bf16x8, As[cur][t][(wm * 64 + i * 16) * 4 + fs[i & 1]]);
+      for (int t = 0; t < NP; ++t) fa[t] = As[cur][t][(wm * 64 + i * 16) * 4 + fs[i & 1]];
+      if constexpr (NP == 3) {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) small[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[2], fb[0][j], small[i][j], 0, 0, 0);
+        for (int j = 0; j < 4; ++j) small[i][j] = pmfma<NP>(fa[2], fb[0][j], small[i][j]);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) small[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[1], fb[1][j], small[i][j], 0, 0, 0);
+        for (int j = 0; j < 4; ++j) small[i][j] = pmfma<NP>(fa[1], fb[1][j], small[i][j]);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) small[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[0], fb[2][j], small[i][j], 0, 0, 0);
+        for (int j = 0; j < 4; ++j) small[i][j] = pmfma<NP>(fa[0], fb[2][j], small[i][j]);
+      }
 #pragma unroll
-      for (int j = 0; j < 4; ++j) small[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[1], fb[0][j], small[i][j], 0, 0, 0);
+      for (int j = 0; j < 4; ++j) small[i][j] = pmfma<NP>(fa[1], fb[0][j], small[i][j]);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) small[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[0], fb[1][j], small[i][j], 0, 0, 0);
+      for (int j = 0; j < 4; ++j) small[i][j] = pmfma<NP>(fa[0], fb[1][j], small[i][j]);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) big[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[0], fb[0][j], big[i][j], 0, 0, 0);
+      for (int j = 0; j < 4; ++j) big[i][j] = pmfma<NP>(fa[0], fb[0][j], big[i][j]);
     }
     if (kt + 1 < nk) {
       TAL_PSTORE(cur ^ 1)
@@ -687,6 +789,17 @@ __global__ __launch_bounds__(512) void gemm_bf16x6p_kernel(int64_t M, int64_t N,
 #undef TAL_PG
 #undef TAL_PLOAD
 #undef TAL_PSTORE
+  // NP = 2: exponents of this lane's 16 rows (i, r) and 4 columns (j); every
+  // index is below Mp / Np, the sizes of UA / UB
+  int ea[4][4] = {}, eb[4] = {};
+  if constexpr (NP == 2) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) ea[i][r] = row_exp(UA[bx * PBM + wm * 64 + i * 16 + 4 * (lane >> 4) + r]);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) eb[j] = row_exp(UB[by * PBN + wn * 64 + j * 16 + (lane & 15)]);
+  }
   if (ksplit > 1) {
     float *pb = part + (int64_t)blockIdx.x * PBM * PBN;
 #pragma unroll
@@ -696,7 +809,8 @@ __global__ __launch_bounds__(512) void gemm_bf16x6p_kernel(int64_t M, int64_t N,
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int ml = wm * 64 + i * 16 + 4 * (lane >> 4) + r, nl = wn * 64 + j * 16 + (lane & 15);
-          pb[ml + nl * PBM] = big[i][j][r] + small[i][j][r];
+          const float s = big[i][j][r] + small[i][j][r];
+          pb[ml + nl * PBM] = NP == 2 ? ldexpf(s, ea[i][r] + eb[j]) : s;
         }
     return;
   }
@@ -710,24 +824,64 @@ __global__ __launch_bounds__(512) void gemm_bf16x6p_kernel(int64_t M, int64_t N,
         const int64_t n = by * PBN + wn * 64 + j * 16 + (lane & 15);
         if (m < M && n < N) {
           float *c = C + m + n * ldc;
-          const float v = alpha * (big[i][j][r] + small[i][j][r]);
+          const float s = big[i][j][r] + small[i][j][r];
+          const float v = alpha * (NP == 2 ? ldexpf(s, ea[i][r] + eb[j]) : s);
           *c = beta_one ? *c + v : v;
         }
       }
 }
 
+__global__ __launch_bounds__(512) void gemm_bf16x6p_kernel(int64_t M, int64_t N, int64_t Kp,
+                                                           const uint4 *__restrict__ Ap, int64_t aplane,
+                                                           const uint4 *__restrict__ Bp, int64_t bplane,
+                                                           float *__restrict__ C, int64_t ldc, float alpha,
+                                                           int beta_one, int64_t gx, int64_t gy, int64_t t0,
+                                                           int64_t count, int ksplit, int64_t ksteps,
+                                                           float *__restrict__ part) {
+  presplit_gemm<3>(M, N, Kp, Ap, aplane, Bp, bplane, C, ldc, alpha, beta_one, gx, gy, t0, count, ksplit, ksteps,
+                   part, nullptr, nullptr);
+}
+
+__global__ __launch_bounds__(512) void gemm_f16x3p_kernel(int64_t M, int64_t N, int64_t Kp,
+                                                          const uint4 *__restrict__ Ap, int64_t aplane,
+                                                          const uint4 *__restrict__ Bp, int64_t bplane,
+                                                          float *__restrict__ C, int64_t ldc, float alpha,
+                                                          int beta_one, int64_t gx, int64_t gy, int64_t t0,
+                                                          int64_t count, int ksplit, int64_t ksteps,
+                                                          float *__restrict__ part, const uint32_t *__restrict__ UA,
+                                                          const uint32_t *__restrict__ UB) {
+  presplit_gemm<2>(M, N, Kp, Ap, aplane, Bp, bplane, C, ldc, alpha, beta_one, gx, gy, t0, count, ksplit, ksteps,
+                   part, UA, UB);
+}
+
 int64_t round_up(int64_t x, int64_t q) { return (x + q - 1) / q * q; }
 
-// Splits op(X) (rows R, depth K) into planes at P; returns 0 or a launch error.
-int split_operand(bool kcont, const float *X, int64_t ld, int64_t R, int64_t K, int64_t Rp, int64_t Kp, __bf16 *P,
-                  hipStream_t s) {
+// Splits op(X) (rows R, depth K) into NP planes at P (NP = 2: also the row
+// maxima into U, Rp entries, used for the scaling); returns 0 or a launch error.
+template <int NP>
+int split_operand(bool kcont, const float *X, int64_t ld, int64_t R, int64_t K, int64_t Rp, int64_t Kp,
+                  uint16_t *P, uint32_t *U, hipStream_t s) {
   const int64_t items = Rp * (Kp / 8), nblk = (items + 255) / 256;
   if (nblk > 0x7fffffffLL) return -4;
+  if constexpr (NP == 2) {
+    fill_u32_kernel<<<dim3((unsigned)((Rp + 255) / 256)), dim3(256), 0, s>>>(U, Rp);
+    if (R > 0 && K > 0) {
+      const int64_t rows = kcont ? 4 : 64;
+      int64_t kseg = kcont ? 2048 : 512;
+      kseg = std::max(kseg, (K + 65534) / 65535);
+      if ((R + rows - 1) / rows > 0x7fffffffLL) return -4;
+      const dim3 grid((unsigned)((R + rows - 1) / rows), (unsigned)((K + kseg - 1) / kseg));
+      if (kcont)
+        row_absmax_kernel<1><<<grid, dim3(256), 0, s>>>(X, ld, R, K, kseg, U);
+      else
+        row_absmax_kernel<0><<<grid, dim3(256), 0, s>>>(X, ld, R, K, kseg, U);
+    }
+  }
   const int vec = ((uintptr_t)X % 16 == 0) && ld % 4 == 0;
   if (kcont)
-    split_planes_kernel<1><<<dim3((unsigned)nblk), dim3(256), 0, s>>>(X, ld, R, K, Rp, Kp, P, Rp * Kp, vec);
+    split_planes_kernel<1, NP><<<dim3((unsigned)nblk), dim3(256), 0, s>>>(X, ld, R, K, Rp, Kp, P, Rp * Kp, vec, U);
   else
-    split_planes_kernel<0><<<dim3((unsigned)nblk), dim3(256), 0, s>>>(X, ld, R, K, Rp, Kp, P, Rp * Kp, vec);
+    split_planes_kernel<0, NP><<<dim3((unsigned)nblk), dim3(256), 0, s>>>(X, ld, R, K, Rp, Kp, P, Rp * Kp, vec, U);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
@@ -885,26 +1039,31 @@ int op_code(char t) {
 
 bool gemm_split_bf16_available() { return true; }
 
-// Resident gemm_bf16x6p_kernel workgroups on the current device, cached per device.
-static int64_t bf16p_slots() {
+// Resident workgroups of the pre-split GEMM kernel with NP planes on the
+// current device, cached per device.
+template <int NP>
+static int64_t presplit_slots() {
   static int64_t cache[64] = {0};
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
   if (!cache[dev]) {
     int cus = 0, per = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, gemm_bf16x6p_kernel, 512, 0) != hipSuccess)
-      return 0;
+    hipError_t e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess)
+      e = NP == 3 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, gemm_bf16x6p_kernel, 512, 0)
+                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, gemm_f16x3p_kernel, 512, 0);
+    if (e != hipSuccess) return 0;
     cache[dev] = (int64_t)cus * std::max(per, 1);
   }
   return cache[dev];
 }
 
 // K split of a mostly idle last round of tiles (as for the FP32/FP64 GEMM).
-static TailSplit bf16p_tail(int64_t M, int64_t N, int64_t K) {
+template <int NP>
+static TailSplit presplit_tail(int64_t M, int64_t N, int64_t K) {
   TailSplit t;
   if (M <= 0 || N <= 0) return t;
-  const int64_t T = (round_up(M, PBM) / PBM) * (round_up(N, PBN) / PBN), slots = bf16p_slots();
+  const int64_t T = (round_up(M, PBM) / PBM) * (round_up(N, PBN) / PBN), slots = presplit_slots<NP>();
   if (slots <= 0) return t;
   const int64_t rem = T % slots;
   if (rem == 0 || rem * 4 > slots) return t;
@@ -915,48 +1074,84 @@ static TailSplit bf16p_tail(int64_t M, int64_t N, int64_t K) {
   return t;
 }
 
-size_t gemm_split_workspace(int64_t M, int64_t N, int64_t K) {
-  const int64_t Mp = round_up(std::max<int64_t>(M, 1), PBM), Np = round_up(std::max<int64_t>(N, 1), PBN);
-  const int64_t Kp = round_up(std::max<int64_t>(K, 1), PBK);
-  const TailSplit t = bf16p_tail(M, N, K);
-  const size_t tail = t.ksplit > 1 ? (size_t)t.rem * t.ksplit * PBM * PBN * sizeof(float) : 0;
-  return (size_t)(round_up(6 * Mp * Kp, 256) + round_up(6 * Np * Kp, 256)) + tail;
+// Workspace of the pre-split GEMM: the planes of op(A) and op(B) (NP planes
+// of 2-byte values each), for NP = 2 the row / column maxima, then the
+// partial sums of a K-split last round; each part starts on 256 bytes.
+struct SplitLayout {
+  int64_t Mp, Np, Kp;
+  size_t a, b, ua, ub, part, total;
+  TailSplit tail;
+};
+
+template <int NP>
+static SplitLayout split_layout(int64_t M, int64_t N, int64_t K) {
+  SplitLayout l;
+  l.Mp = round_up(std::max<int64_t>(M, 1), PBM);
+  l.Np = round_up(std::max<int64_t>(N, 1), PBN);
+  l.Kp = round_up(std::max<int64_t>(K, 1), PBK);
+  l.tail = presplit_tail<NP>(M, N, K);
+  l.a = 0;
+  l.b = l.a + (size_t)round_up(2 * NP * l.Mp * l.Kp, 256);
+  l.ua = l.b + (size_t)round_up(2 * NP * l.Np * l.Kp, 256);
+  l.ub = l.ua + (NP == 2 ? (size_t)round_up(4 * l.Mp, 256) : 0);
+  l.part = l.ub + (NP == 2 ? (size_t)round_up(4 * l.Np, 256) : 0);
+  l.total = l.part + (l.tail.ksplit > 1 ? (size_t)l.tail.rem * l.tail.ksplit * PBM * PBN * sizeof(float) : 0);
+  return l;
 }
 
-int launch_gemm_split(char ta, char tb, int64_t M, int64_t N, int64_t K, double ar, const void *A, int64_t lda,
-                      const void *B, int64_t ldb, int beta_one, void *C, int64_t ldc, void *ws, hipStream_t s) {
-  const int opa = op_code(ta), opb = op_code(tb);
-  if (opa < 0 || opb < 0 || M < 0 || N < 0 || K < 0 || !A || !B || !C || !ws || (uintptr_t)ws % 16) return -1;
-  if (M == 0 || N == 0) return 0;
-  if (ldc < std::max<int64_t>(1, M)) return -1;
-  if (lda < std::max<int64_t>(1, opa == 0 ? M : K)) return -1;
-  if (ldb < std::max<int64_t>(1, opb == 0 ? K : N)) return -1;
-  const int64_t Mp = round_up(M, PBM), Np = round_up(N, PBN), Kp = round_up(std::max<int64_t>(K, 1), PBK);
-  __bf16 *pa = (__bf16 *)ws;
-  __bf16 *pb = (__bf16 *)((char *)ws + round_up(6 * Mp * Kp, 256));
+template <int NP>
+static int launch_presplit(int opa, int opb, int64_t M, int64_t N, int64_t K, double ar, const void *A, int64_t lda,
+                           const void *B, int64_t ldb, int beta_one, void *C, int64_t ldc, void *ws, hipStream_t s) {
+  const SplitLayout l = split_layout<NP>(M, N, K);
+  char *w = (char *)ws;
+  uint16_t *pa = (uint16_t *)(w + l.a), *pb = (uint16_t *)(w + l.b);
+  uint32_t *ua = NP == 2 ? (uint32_t *)(w + l.ua) : nullptr, *ub = NP == 2 ? (uint32_t *)(w + l.ub) : nullptr;
+  float *part = (float *)(w + l.part);
   // A 'T' (KxM) and B 'N' (KxN) already have K contiguous per row of the plane
-  if (int e = split_operand(opa == 1, (const float *)A, lda, M, K, Mp, Kp, pa, s)) return e;
-  if (int e = split_operand(opb == 0, (const float *)B, ldb, N, K, Np, Kp, pb, s)) return e;
-  const int64_t gx = Mp / PBM, gy = Np / PBN;
+  if (int e = split_operand<NP>(opa == 1, (const float *)A, lda, M, K, l.Mp, l.Kp, pa, ua, s)) return e;
+  if (int e = split_operand<NP>(opb == 0, (const float *)B, ldb, N, K, l.Np, l.Kp, pb, ub, s)) return e;
+  const int64_t gx = l.Mp / PBM, gy = l.Np / PBN;
   if (gx * gy > 0x7fffffffLL) return -4;
-  // the planes take the front of ws; partial sums of a K-split last round follow them
-  const TailSplit ts = bf16p_tail(M, N, K);
-  float *part = (float *)((char *)ws + round_up(6 * Mp * Kp, 256) + round_up(6 * Np * Kp, 256));
-  const int64_t full = gx * gy - ts.rem, nsteps = Kp / PBK;
+  const TailSplit ts = l.tail;
+  const int64_t full = gx * gy - ts.rem, nsteps = l.Kp / PBK;
   const int64_t ksteps = ts.ksplit > 1 ? (nsteps + ts.ksplit - 1) / ts.ksplit : 0;
-  if (full > 0)
-    gemm_bf16x6p_kernel<<<dim3((unsigned)full), dim3(512), 0, s>>>(
-        M, N, Kp, (const uint4 *)pa, Mp * Kp / 8, (const uint4 *)pb, Np * Kp / 8, (float *)C, ldc, (float)ar,
-        beta_one, gx, gy, 0, full, 1, 0, nullptr);
+  const int64_t apl = l.Mp * l.Kp / 8, bpl = l.Np * l.Kp / 8;
+  auto gemm = [&](int64_t blocks, int64_t t0, int64_t count, int ksplit, int64_t kst, float *pt) {
+    if (NP == 3)
+      gemm_bf16x6p_kernel<<<dim3((unsigned)blocks), dim3(512), 0, s>>>(
+          M, N, l.Kp, (const uint4 *)pa, apl, (const uint4 *)pb, bpl, (float *)C, ldc, (float)ar, beta_one, gx, gy,
+          t0, count, ksplit, kst, pt);
+    else
+      gemm_f16x3p_kernel<<<dim3((unsigned)blocks), dim3(512), 0, s>>>(
+          M, N, l.Kp, (const uint4 *)pa, apl, (const uint4 *)pb, bpl, (float *)C, ldc, (float)ar, beta_one, gx, gy,
+          t0, count, ksplit, kst, pt, ua, ub);
+  };
+  if (full > 0) gemm(full, 0, full, 1, 0, nullptr);
   if (ts.ksplit > 1) {
-    gemm_bf16x6p_kernel<<<dim3((unsigned)(ts.rem * ts.ksplit)), dim3(512), 0, s>>>(
-        M, N, Kp, (const uint4 *)pa, Mp * Kp / 8, (const uint4 *)pb, Np * Kp / 8, (float *)C, ldc, (float)ar,
-        beta_one, gx, gy, full, ts.rem, ts.ksplit, ksteps, part);
+    gemm(ts.rem * ts.ksplit, full, ts.rem, ts.ksplit, ksteps, part);
     const int64_t nel = ts.rem * PBM * PBN;
     gemm_tail_reduce<float, 1, PBM, PBN><<<dim3((unsigned)((nel + kThreads - 1) / kThreads)), dim3(kThreads), 0, s>>>(
         M, N, part, ts.ksplit, (float *)C, ldc, (float)ar, 0.0f, beta_one, gx, gy, full, ts.rem);
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+size_t gemm_split_workspace(int method, int64_t M, int64_t N, int64_t K) {
+  return method == SPLIT_FP16X3 ? split_layout<2>(M, N, K).total : split_layout<3>(M, N, K).total;
+}
+
+int launch_gemm_split(int method, char ta, char tb, int64_t M, int64_t N, int64_t K, double ar, const void *A,
+                      int64_t lda, const void *B, int64_t ldb, int beta_one, void *C, int64_t ldc, void *ws,
+                      hipStream_t s) {
+  const int opa = op_code(ta), opb = op_code(tb);
+  if (opa < 0 || opb < 0 || M < 0 || N < 0 || K < 0 || !A || !B || !C || !ws || (uintptr_t)ws % 16) return -1;
+  if (method != SPLIT_BF16X6 && method != SPLIT_FP16X3) return -1;
+  if (M == 0 || N == 0) return 0;
+  if (ldc < std::max<int64_t>(1, M)) return -1;
+  if (lda < std::max<int64_t>(1, opa == 0 ? M : K)) return -1;
+  if (ldb < std::max<int64_t>(1, opb == 0 ? K : N)) return -1;
+  return method == SPLIT_FP16X3 ? launch_presplit<2>(opa, opb, M, N, K, ar, A, lda, B, ldb, beta_one, C, ldc, ws, s)
+                                : launch_presplit<3>(opa, opb, M, N, K, ar, A, lda, B, ldb, beta_one, C, ldc, ws, s);
 }
 
 size_t gemm_tail_workspace(int dk, int64_t M, int64_t N, int64_t K, int64_t batch) {

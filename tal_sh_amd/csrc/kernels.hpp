@@ -39,12 +39,18 @@ enum R4Mode { R4_FP32 = 0, R4_SPLIT_BF16 = 1 };
 // Whether the split-BF16 R4 GEMM is built into this library.
 bool gemm_split_bf16_available();
 
-// Split-BF16 R4 GEMM with pre-split operands (same contract as launch_gemm for
-// dk = R4); ws is device scratch of gemm_split_workspace(M, N, K) bytes,
-// 16-byte aligned, that holds the BF16 planes of both operands.
-size_t gemm_split_workspace(int64_t M, int64_t N, int64_t K);
-int launch_gemm_split(char ta, char tb, int64_t M, int64_t N, int64_t K, double ar, const void *A, int64_t lda,
-                      const void *B, int64_t ldb, int beta_one, void *C, int64_t ldc, void *ws, hipStream_t s);
+// Split methods of the R4 GEMM on lower-precision MFMA: three BF16 planes per
+// operand and six products (bf16x6), or two FP16 planes of the row-scaled
+// operand and three products (fp16x3).
+enum SplitMethod { SPLIT_BF16X6 = 0, SPLIT_FP16X3 = 1 };
+
+// Split R4 GEMM with pre-split operands (same contract as launch_gemm for
+// dk = R4); ws is device scratch of gemm_split_workspace(method, M, N, K)
+// bytes, 16-byte aligned, that holds the planes of both operands.
+size_t gemm_split_workspace(int method, int64_t M, int64_t N, int64_t K);
+int launch_gemm_split(int method, char ta, char tb, int64_t M, int64_t N, int64_t K, double ar, const void *A,
+                      int64_t lda, const void *B, int64_t ldb, int beta_one, void *C, int64_t ldc, void *ws,
+                      hipStream_t s);
 
 // C(MxN, col-major, ldc) = alpha * op(A) * op(B) + beta * C, beta in {0,1}.
 // ta: 'N' A is MxK (lda>=M); 'T' A is KxM (lda>=K).

@@ -353,7 +353,7 @@ TtgtTemps ttgt_temp_bytes(const ContractPlan &plan, int dk, size_t lbytes, size_
   if (plan.a.transpose) t.a = plan.swap ? rbytes : lbytes;
   if (plan.b.transpose) t.b = plan.swap ? lbytes : rbytes;
   if (plan.d_transpose) t.d = dbytes;
-  if (dk == R4 && rt().r4mode == R4_SPLIT_BF16) t.w = gemm_split_workspace(plan.M, plan.N, plan.K);
+  if (dk == R4 && rt().r4mode == R4_SPLIT_BF16) t.w = gemm_split_workspace(rt().split_method, plan.M, plan.N, plan.K);
   else t.s = gemm_tail_workspace(dk, plan.M, plan.N, plan.K, plan.batch);
   return t;
 }
@@ -393,7 +393,7 @@ int enqueue_ttgt(const DigitalPattern &p, const ContractPlan &plan, int dk, bool
   if (split) {
     if (!tmp.w) return TALSH_FAILURE;
     for (int64_t q = 0; q < plan.batch; ++q)  // the planes workspace is reused in stream order
-      if (launch_gemm_split(plan.a.op, plan.b.op, plan.M, plan.N, plan.K, ar, (const float *)A + q * sa, plan.a.ld,
+      if (launch_gemm_split(rt().split_method, plan.a.op, plan.b.op, plan.M, plan.N, plan.K, ar, (const float *)A + q * sa, plan.a.ld,
                             (const float *)B + q * sb, plan.b.ld, acc ? 1 : 0, (float *)C + q * sc,
                             std::max<int64_t>(1, plan.M), tmp.w, s))
         return TALSH_FAILURE;
@@ -848,8 +848,9 @@ int talshTensorDiscardOther_(talsh_tens_t *tens, int dev_id, int dev_kind) {
   return talshTensorDiscardOther(tens, dev_id, dev_kind);
 }
 
-// Fast math (talsh.h:167-174): on this build it selects the split-BF16 MFMA
-// path for R4/C4 GEMMs; R8/C8 are unaffected.
+// Fast math (talsh.h:167-174): on this build it selects the split MFMA GEMM
+// (method: talshAmdSetFastMathSplit) for R4 contractions; C4/R8/C8 are
+// unaffected.
 int talshEnableFastMath(int dev_kind, int dev_id) {
   (void)dev_kind;
   (void)dev_id;
@@ -870,6 +871,14 @@ int talshQueryFastMath(int dev_kind, int dev_id) {
 }
 
 // ---- extensions (talsh_amd.h) ----
+static_assert(SPLIT_BF16X6 == TALSH_AMD_SPLIT_BF16X6 && SPLIT_FP16X3 == TALSH_AMD_SPLIT_FP16X3, "split ids");
+int talshAmdSetFastMathSplit(int method) {
+  if (method != SPLIT_BF16X6 && method != SPLIT_FP16X3) return TALSH_INVALID_ARGS;
+  std::lock_guard<std::recursive_mutex> lk(rt().mu);
+  rt().split_method = method;
+  return TALSH_SUCCESS;
+}
+int talshAmdGetFastMathSplit(void) { return rt().split_method; }
 int talshAmdSetDeviceBufferSize(size_t bytes) {
   if (rt().on) return TALSH_ALREADY_INITIALIZED;
   rt().dev_buffer_request = bytes;

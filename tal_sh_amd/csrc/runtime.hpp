@@ -59,6 +59,7 @@ struct Runtime {
   size_t host_size = 0;
   Arena host_arena;
   int r4mode = R4_FP32;
+  int split_method = SPLIT_BF16X6;  // talshAmdSetFastMathSplit
   size_t dev_buffer_request = 0;  // 0: take a fraction of the free memory
   int nstreams = 4;
   size_t xl_mem = 0;  // device bytes a talshTensorContractXL call may use (0: from the free pool)

@@ -242,7 +242,7 @@ int talshTensorContractXL(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *
       else K *= t[lid[i]];
     for (int j = 0; j < p.rrank; ++j)
       if (rid[j] < nd) N *= t[rid[j]];
-    if (dk == R4 && r.r4mode == R4_SPLIT_BF16) b += gemm_split_workspace(M, N, K);
+    if (dk == R4 && r.r4mode == R4_SPLIT_BF16) b += gemm_split_workspace(r.split_method, M, N, K);
     else b += gemm_tail_workspace(dk, M, N, K, 1);
     return b;
   };

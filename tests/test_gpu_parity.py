@@ -254,13 +254,16 @@ def test_contract_gemm_k_split_tail(talsh_gpu, oracle):
             x.destruct()
 
 
-def test_contract_split_bf16_r4(talsh_gpu, oracle):
-    """R4 with fast math (3-term split-BF16 MFMA, FP32 accumulation): normwise error <= 1e-6
-    against the FP64 oracle evaluated on the same FP32 inputs widened to FP64."""
+@pytest.mark.parametrize("split", ["bf16x6", "fp16x3"])
+def test_contract_split_r4(talsh_gpu, oracle, split):
+    """R4 with fast math, FP32 accumulation: 3-term split-BF16 (6 products) or row-scaled 2-term FP16 (3 products)
+    MFMA.  Normwise error <= 1e-6 against the FP64 oracle evaluated on the same FP32 inputs widened to FP64."""
     ts = talsh_gpu
     lib = ts.load()
-    assert lib.talshEnableFastMath(ts.DEV_NVIDIA_GPU, 0) == 0
+    method = ts.SPLIT_BF16X6 if split == "bf16x6" else ts.SPLIT_FP16X3
+    assert ts.fast_math(True, method) == 0
     assert lib.talshQueryFastMath(ts.DEV_NVIDIA_GPU, 0) == ts.YEP
+    assert lib.talshAmdGetFastMathSplit() == method
     try:
         rng = random.Random(7)
         cases = [("D(a,b,c,d)+=L(a,i,c,j)*R(j,b,i,d)", [24, 20, 24, 20], [24, 40, 24, 36], [36, 20, 40, 20]),
@@ -277,7 +280,7 @@ def test_contract_split_bf16_r4(talsh_gpu, oracle):
         # 32 tiles of 256 x 128: too few for the GPU, run as K ranges plus a summing pass
         cases.append(("D(a,b)+=L(k,a)*R(k,b)", [1024, 1024], [2048, 1024], [2048, 1024]))
         for t, (pat, dd, ld, rd) in enumerate(cases):
-            say("split-bf16", pat, dd, ld, rd)
+            say("split", split, pat, dd, ld, rd)
             L, R, D = tensor(R4, ld, t), tensor(R4, rd, t + 1), tensor(R4, dd, t + 2)
             lt, rt, dt = on_gpu(ts, R4, ld, L), on_gpu(ts, R4, rd, R), on_gpu(ts, R4, dd, D)
             ts.check(ts.contract(pat, dt, lt, rt, alpha=0.5), "contract")
@@ -286,7 +289,45 @@ def test_contract_split_bf16_r4(talsh_gpu, oracle):
                        (0.5, 0.0), 1)
             assert relerr(dt.host_array().astype(np.float64), want) <= 1e-6, pat
     finally:
-        lib.talshDisableFastMath(ts.DEV_NVIDIA_GPU, 0)
+        ts.fast_math(False)
+        lib.talshAmdSetFastMathSplit(ts.SPLIT_BF16X6)
+
+
+@pytest.mark.parametrize("split", ["bf16x6", "fp16x3"])
+def test_contract_split_r4_wide_range(talsh_gpu, oracle, split):
+    """Rows of L and columns of R scaled by powers of two from 2^-50 to 2^50 (L(a,k) has its rows strided
+    along K and R(k,b) its columns contiguous along K, so both row-maximum layouts of the FP16 scaling are
+    used), K = 700 off the 32-grid.  Every element must satisfy |D - D*| <= 1e-6 * sum_k |L||R| (the FP32
+    error bound of the product), which holds only if each row and column is scaled and unscaled by its own
+    exponent; the scales exceed FP16's range, so an unscaled FP16 split overflows or flushes."""
+    ts = talsh_gpu
+    lib = ts.load()
+    method = ts.SPLIT_BF16X6 if split == "bf16x6" else ts.SPLIT_FP16X3
+    assert ts.fast_math(True, method) == 0
+    try:
+        pat = "D(a,b)+=L(a,k)*R(k,b)"
+        m, n, k = 300, 260, 700
+        rng = np.random.default_rng(5)
+        l64 = (rng.uniform(-1, 1, (m, k)) * np.exp2(rng.integers(-50, 51, m))[:, None]).astype(np.float32)
+        r64 = (rng.uniform(-1, 1, (k, n)) * np.exp2(rng.integers(-50, 51, n))[None, :]).astype(np.float32)
+        l64, r64 = l64.astype(np.float64), r64.astype(np.float64)
+        L = l64.astype(np.float32).reshape(-1, order="F")
+        R = r64.astype(np.float32).reshape(-1, order="F")
+        D = np.zeros(m * n, np.float32)
+        say("split wide range", split, pat, m, n, k)
+        lt, rt, dt = on_gpu(ts, R4, [m, k], L), on_gpu(ts, R4, [k, n], R), on_gpu(ts, R4, [m, n], D)
+        ts.check(ts.contract(pat, dt, lt, rt, alpha=1.0), "contract")
+        want = oracle.contract(R8, pat, [m, n], D.astype(np.float64), [m, k], l64.reshape(-1, order="F"), [k, n],
+                               r64.reshape(-1, order="F"), (1.0, 0.0), 1)
+        bound = (np.abs(l64) @ np.abs(r64)).reshape(-1, order="F")
+        got = dt.host_array().astype(np.float64)
+        assert np.all(np.isfinite(got))
+        assert np.all(np.abs(got - want) <= 1e-6 * bound), np.max(np.abs(got - want) / bound)
+        for x in (lt, rt, dt):
+            x.destruct()
+    finally:
+        ts.fast_math(False)
+        lib.talshAmdSetFastMathSplit(ts.SPLIT_BF16X6)
 
 
 def test_contract_reference_shapes(talsh_gpu, oracle):
