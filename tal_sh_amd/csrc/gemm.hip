@@ -583,9 +583,13 @@ __global__ __launch_bounds__(256) void fill_u32_kernel(uint32_t *__restrict__ p,
 // segments of kseg.  KC = 1 (X(r, k) at X[r*ld + k]): one wave per row, lanes
 // along k.  KC = 0 (X(r, k) at X[r + k*ld]): lanes along r, the four waves
 // share the segment.
+// vec (KC = 1): X 16-byte aligned and ld a multiple of 4, so whole rounds of
+// 256 floats of a segment (kseg a multiple of 256) are read as float4.
+__device__ inline uint32_t abs_bits(float x) { return __float_as_uint(x) & 0x7fffffffu; }
+
 template <int KC>
 __global__ __launch_bounds__(256) void row_absmax_kernel(const float *__restrict__ X, int64_t ld, int64_t R,
-                                                         int64_t K, int64_t kseg, uint32_t *__restrict__ U) {
+                                                         int64_t K, int64_t kseg, int vec, uint32_t *__restrict__ U) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t k0 = (int64_t)blockIdx.y * kseg, k1 = min(K, k0 + kseg);
   uint32_t m = 0;
@@ -593,8 +597,18 @@ __global__ __launch_bounds__(256) void row_absmax_kernel(const float *__restrict
     const int64_t r = (int64_t)blockIdx.x * 4 + wave;
     if (r < R) {
       const float *x = X + r * ld;
-#pragma unroll 8
-      for (int64_t k = k0 + lane; k < k1; k += 64) m = max(m, __float_as_uint(x[k]) & 0x7fffffffu);
+      int64_t k = k0;
+      if (vec) {
+        const int64_t kv = k0 + ((k1 - k0) & ~(int64_t)255);
+#pragma unroll 4
+        for (int64_t q = k0 + 4 * lane; q < kv; q += 256) {
+          const float4 v = *reinterpret_cast<const float4 *>(x + q);
+          m = max(m, max(max(abs_bits(v.x), abs_bits(v.y)), max(abs_bits(v.z), abs_bits(v.w))));
+        }
+        k = kv;
+      }
+#pragma unroll 4
+      for (k += lane; k < k1; k += 64) m = max(m, abs_bits(x[k]));
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o));
@@ -603,7 +617,7 @@ __global__ __launch_bounds__(256) void row_absmax_kernel(const float *__restrict
     const int64_t r = (int64_t)blockIdx.x * 64 + lane;
     if (r < R) {
 #pragma unroll 8
-      for (int64_t k = k0 + wave; k < k1; k += 4) m = max(m, __float_as_uint(X[r + k * ld]) & 0x7fffffffu);
+      for (int64_t k = k0 + wave; k < k1; k += 4) m = max(m, abs_bits(X[r + k * ld]));
       if (m) atomicMax(U + r, m);
     }
   }
@@ -863,21 +877,21 @@ int split_operand(bool kcont, const float *X, int64_t ld, int64_t R, int64_t K, 
                   uint16_t *P, uint32_t *U, hipStream_t s) {
   const int64_t items = Rp * (Kp / 8), nblk = (items + 255) / 256;
   if (nblk > 0x7fffffffLL) return -4;
+  const int vec = ((uintptr_t)X % 16 == 0) && ld % 4 == 0;
   if constexpr (NP == 2) {
     fill_u32_kernel<<<dim3((unsigned)((Rp + 255) / 256)), dim3(256), 0, s>>>(U, Rp);
     if (R > 0 && K > 0) {
       const int64_t rows = kcont ? 4 : 64;
       int64_t kseg = kcont ? 2048 : 512;
-      kseg = std::max(kseg, (K + 65534) / 65535);
+      kseg = std::max(kseg, round_up((K + 65534) / 65535, 256));
       if ((R + rows - 1) / rows > 0x7fffffffLL) return -4;
       const dim3 grid((unsigned)((R + rows - 1) / rows), (unsigned)((K + kseg - 1) / kseg));
       if (kcont)
-        row_absmax_kernel<1><<<grid, dim3(256), 0, s>>>(X, ld, R, K, kseg, U);
+        row_absmax_kernel<1><<<grid, dim3(256), 0, s>>>(X, ld, R, K, kseg, vec, U);
       else
-        row_absmax_kernel<0><<<grid, dim3(256), 0, s>>>(X, ld, R, K, kseg, U);
+        row_absmax_kernel<0><<<grid, dim3(256), 0, s>>>(X, ld, R, K, kseg, vec, U);
     }
   }
-  const int vec = ((uintptr_t)X % 16 == 0) && ld % 4 == 0;
   if (kcont)
     split_planes_kernel<1, NP><<<dim3((unsigned)nblk), dim3(256), 0, s>>>(X, ld, R, K, Rp, Kp, P, Rp * Kp, vec, U);
   else
