@@ -739,6 +739,11 @@ def main():
                 secondary[key] = bench_contraction(oracle, R4, pat3, 96, max(2, args.steps // 2), 1)
             ts.fast_math(False)
             ts.load().talshAmdSetFastMathSplit(ts.SPLIT_BF16X6)
+    fp32 = secondary.get("r4_fp32_config3")
+    for key in ("r4_split_bf16_config3", "r4_split_fp16x3_config3"):
+        if fp32 and key in secondary:  # the fast-math target is stated against the plain FP32 kernel
+            secondary[key]["speedup_vs_fp32_gemm"] = secondary[key]["gemm_tflops"] / fp32["gemm_tflops"]
+            secondary[key]["speedup_vs_fp32_step"] = secondary[key]["tflops"] / fp32["tflops"]
     if want("c8"):
         secondary["c8_config3"] = bench_contraction(oracle, C8, pat3, 96, max(2, args.steps // 4), 1)
     if want("config1"):
