@@ -102,11 +102,13 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
                                                         S *__restrict__ C, int64_t ldc, S alpha_r, S alpha_i,
                                                         int beta_one, int64_t gx, int64_t gy, int64_t bsa,
                                                         int64_t bsb, int64_t bsc, int64_t t0, int64_t count,
-                                                        int ksplit, int64_t kchunk, S *__restrict__ part) {
+                                                        int ksplit, int64_t kchunk, S *__restrict__ part,
+                                                        const int64_t *__restrict__ cmap) {
   // ksplit == 1: tiles [t0, t0 + count) of C, the whole K each.  ksplit > 1:
   // workgroup q*ksplit + c computes K range [c*kchunk, (c+1)*kchunk) of tile
   // t0 + q into its own BM x BN block of `part` (no alpha, no beta); a second
-  // kernel sums the ksplit blocks of each tile in order.
+  // kernel sums the ksplit blocks of each tile in order.  cmap (batch 1 only):
+  // element (m, n) of C is at C[cmap[m] + cmap[M + n]] instead of C[m + n*ldc].
   using MF = Mfma<S>;
   // batch entry blockIdx.y: its operands start bsa, bsb, bsc scalars further on
   A += (int64_t)blockIdx.y * bsa;
@@ -278,7 +280,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         const int64_t m = m0 + wm * (BM / 2) + i * 16 + MF::row(lane, r);
         const int64_t n = n0 + wn * (BN / 2) + j * 16 + (lane & 15);
         if (m < M && n < N) {
-          S *c = C + (m + n * ldc) * NC;
+          S *c = C + (cmap ? cmap[m] + cmap[M + n] : m + n * ldc) * NC;
           if (NC == 1) {
             const S v = alpha_r * acc[0][i][j][r];
             c[0] = beta_one ? c[0] + v : v;
@@ -298,7 +300,8 @@ template <typename S, int NC, int BM, int BN>
 __global__ __launch_bounds__(kThreads) void gemm_tail_reduce(int64_t M, int64_t N, const S *__restrict__ part,
                                                              int ksplit, S *__restrict__ C, int64_t ldc, S alpha_r,
                                                              S alpha_i, int beta_one, int64_t gx, int64_t gy,
-                                                             int64_t t0, int64_t ntiles) {
+                                                             int64_t t0, int64_t ntiles,
+                                                             const int64_t *__restrict__ cmap) {
   const int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   if (e >= ntiles * BM * BN) return;
   const int64_t q = e / (BM * BN);
@@ -315,7 +318,7 @@ __global__ __launch_bounds__(kThreads) void gemm_tail_reduce(int64_t M, int64_t 
 #pragma unroll
     for (int c = 0; c < NC; ++c) x[c] += pb[c];
   }
-  S *cp = C + (m + n * ldc) * NC;
+  S *cp = C + (cmap ? cmap[m] + cmap[M + n] : m + n * ldc) * NC;
   if (NC == 1) {
     const S v = alpha_r * x[0];
     cp[0] = beta_one ? cp[0] + v : v;
@@ -870,6 +873,19 @@ __global__ __launch_bounds__(512) void gemm_f16x3p_kernel(int64_t M, int64_t N, 
 
 int64_t round_up(int64_t x, int64_t q) { return (x + q - 1) / q * q; }
 
+__global__ __launch_bounds__(256) void offset_map_kernel(OffsetMap map, int64_t *__restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= map.M + map.N) return;
+  const bool row = e < map.M;
+  int64_t x = row ? e : e - map.M, off = 0;
+  const int t0 = row ? 0 : map.nm, t1 = row ? map.nm : map.nm + map.nn;
+  for (int t = t0; t < t1; ++t) {
+    off += (x % map.ext[t]) * map.str[t];
+    x /= map.ext[t];
+  }
+  out[e] = off;
+}
+
 // Splits op(X) (rows R, depth K) into NP planes at P (NP = 2: also the row
 // maxima into U, Rp entries, used for the scaling); returns 0 or a launch error.
 template <int NP>
@@ -968,7 +984,7 @@ size_t tail_bytes(const TailSplit &t) {
 template <typename S, int NC, int OPA, int OPB, int CJA, int CJB, int BM, int BN>
 int launch_typed(int64_t M, int64_t N, int64_t K, const void *A, int64_t lda, const void *B, int64_t ldb,
                  void *C, int64_t ldc, double ar, double ai, int beta_one, int64_t batch, int64_t sa, int64_t sb,
-                 int64_t sc, void *ws, size_t ws_bytes, hipStream_t s) {
+                 int64_t sc, void *ws, size_t ws_bytes, const int64_t *cmap, hipStream_t s) {
   const int64_t gx = (M + BM - 1) / BM, gy = (N + BN - 1) / BN;
   const int64_t nblk = gx * gy;
   if (nblk > 0x7fffffffLL) return -4;
@@ -991,11 +1007,11 @@ int launch_typed(int64_t M, int64_t N, int64_t K, const void *A, int64_t lda, co
       if (vec)
         gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, WV><<<dim3((unsigned)full, nb), dim3(kThreads), 0, s>>>(
             M, N, K, a, lda, b, ldb, c, ldc, (S)ar, (S)ai, beta_one, gx, gy, sa * NC, sb * NC, sc * NC, 0, full, 1,
-            0, nullptr);
+            0, nullptr, cmap);
       else
         gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, 1><<<dim3((unsigned)full, nb), dim3(kThreads), 0, s>>>(
             M, N, K, a, lda, b, ldb, c, ldc, (S)ar, (S)ai, beta_one, gx, gy, sa * NC, sb * NC, sc * NC, 0, full, 1,
-            0, nullptr);
+            0, nullptr, cmap);
     }
   }
   if (ts.ksplit > 1) {
@@ -1003,14 +1019,14 @@ int launch_typed(int64_t M, int64_t N, int64_t K, const void *A, int64_t lda, co
     if (vec)
       gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, WV><<<dim3(nw), dim3(kThreads), 0, s>>>(
           M, N, K, (const S *)A, lda, (const S *)B, ldb, (S *)C, ldc, (S)ar, (S)ai, beta_one, gx, gy, 0, 0, 0, full,
-          ts.rem, ts.ksplit, kchunk, (S *)ws);
+          ts.rem, ts.ksplit, kchunk, (S *)ws, cmap);
     else
       gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, 1><<<dim3(nw), dim3(kThreads), 0, s>>>(
           M, N, K, (const S *)A, lda, (const S *)B, ldb, (S *)C, ldc, (S)ar, (S)ai, beta_one, gx, gy, 0, 0, 0, full,
-          ts.rem, ts.ksplit, kchunk, (S *)ws);
+          ts.rem, ts.ksplit, kchunk, (S *)ws, cmap);
     const int64_t nel = ts.rem * BM * BN;
     gemm_tail_reduce<S, NC, BM, BN><<<dim3((unsigned)((nel + kThreads - 1) / kThreads)), dim3(kThreads), 0, s>>>(
-        M, N, (const S *)ws, ts.ksplit, (S *)C, ldc, (S)ar, (S)ai, beta_one, gx, gy, full, ts.rem);
+        M, N, (const S *)ws, ts.ksplit, (S *)C, ldc, (S)ar, (S)ai, beta_one, gx, gy, full, ts.rem, cmap);
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
@@ -1018,11 +1034,12 @@ int launch_typed(int64_t M, int64_t N, int64_t K, const void *A, int64_t lda, co
 template <typename S, int NC, int BM, int BN>
 int dispatch_ops(int opa, int opb, int cja, int cjb, int64_t M, int64_t N, int64_t K, const void *A, int64_t lda,
                  const void *B, int64_t ldb, void *C, int64_t ldc, double ar, double ai, int beta_one,
-                 int64_t batch, int64_t sa, int64_t sb, int64_t sc, void *ws, size_t ws_bytes, hipStream_t s) {
+                 int64_t batch, int64_t sa, int64_t sb, int64_t sc, void *ws, size_t ws_bytes,
+                 const int64_t *cmap, hipStream_t s) {
 #define TAL_GEMM_CASE(a, b, ca, cb)                                                                      \
   if (opa == a && opb == b && cja == ca && cjb == cb)                                                    \
     return launch_typed<S, NC, a, b, ca, cb, BM, BN>(M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, \
-                                                     sa, sb, sc, ws, ws_bytes, s);
+                                                     sa, sb, sc, ws, ws_bytes, cmap, s);
 #define TAL_GEMM_LAYOUTS(ca, cb) \
   TAL_GEMM_CASE(0, 0, ca, cb)    \
   TAL_GEMM_CASE(0, 1, ca, cb)    \
@@ -1145,8 +1162,18 @@ static int launch_presplit(int opa, int opb, int64_t M, int64_t N, int64_t K, do
     gemm(ts.rem * ts.ksplit, full, ts.rem, ts.ksplit, ksteps, part);
     const int64_t nel = ts.rem * PBM * PBN;
     gemm_tail_reduce<float, 1, PBM, PBN><<<dim3((unsigned)((nel + kThreads - 1) / kThreads)), dim3(kThreads), 0, s>>>(
-        M, N, part, ts.ksplit, (float *)C, ldc, (float)ar, 0.0f, beta_one, gx, gy, full, ts.rem);
+        M, N, part, ts.ksplit, (float *)C, ldc, (float)ar, 0.0f, beta_one, gx, gy, full, ts.rem, nullptr);
   }
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_offset_map(const OffsetMap &map, int64_t *out, hipStream_t s) {
+  if (map.nm < 0 || map.nn < 0 || map.nm + map.nn > kMaxMapDims || !out) return -1;
+  for (int t = 0; t < map.nm + map.nn; ++t)
+    if (map.ext[t] < 1) return -1;
+  const int64_t n = map.M + map.N, nblk = (n + 255) / 256;
+  if (nblk > 0x7fffffffLL) return -4;
+  offset_map_kernel<<<dim3((unsigned)nblk), dim3(256), 0, s>>>(map, out);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
@@ -1181,7 +1208,7 @@ size_t gemm_tail_workspace(int dk, int64_t M, int64_t N, int64_t K, int64_t batc
 int launch_gemm(int dk, int r4mode, char ta, char tb, int conja, int conjb, int64_t M, int64_t N, int64_t K,
                 double ar, double ai, const void *A, int64_t lda, const void *B, int64_t ldb, int beta_one, void *C,
                 int64_t ldc, hipStream_t s, int64_t batch, int64_t sa, int64_t sb, int64_t sc, void *ws,
-                size_t ws_bytes) {
+                size_t ws_bytes, const int64_t *cmap) {
   const bool cplx = (dk == 14 || dk == 18);
   const int opa = op_code(ta), opb = op_code(tb);
   const int cja = (cplx && conja) ? 1 : 0, cjb = (cplx && conjb) ? 1 : 0;
@@ -1191,6 +1218,7 @@ int launch_gemm(int dk, int r4mode, char ta, char tb, int conja, int conjb, int6
   if (lda < std::max<int64_t>(1, opa == 0 ? M : K)) return -1;
   if (ldb < std::max<int64_t>(1, opb == 0 ? K : N)) return -1;
   if (batch > 1 && (sa < 0 || sb < 0 || sc < ldc * N)) return -1;  // batch entries of C may not overlap
+  if (cmap && (batch > 1 || (dk == 4 && r4mode == R4_SPLIT_BF16))) return -1;
   if (dk == 4 && r4mode == R4_SPLIT_BF16) {
     for (int64_t q = 0; q < batch; ++q) {
       const float *a = (const float *)A + q * sa, *b = (const float *)B + q * sb;
@@ -1205,10 +1233,10 @@ int launch_gemm(int dk, int r4mode, char ta, char tb, int conja, int conjb, int6
     return 0;
   }
   switch (dk) {
-    case 8: return dispatch_ops<double, 1, 128, 128>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, sa, sb, sc, ws, ws_bytes, s);
-    case 4: return dispatch_ops<float, 1, 128, 128>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, sa, sb, sc, ws, ws_bytes, s);
-    case 18: return dispatch_ops<double, 2, 64, 64>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, sa, sb, sc, ws, ws_bytes, s);
-    case 14: return dispatch_ops<float, 2, 64, 64>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, sa, sb, sc, ws, ws_bytes, s);
+    case 8: return dispatch_ops<double, 1, 128, 128>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, sa, sb, sc, ws, ws_bytes, cmap, s);
+    case 4: return dispatch_ops<float, 1, 128, 128>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, sa, sb, sc, ws, ws_bytes, cmap, s);
+    case 18: return dispatch_ops<double, 2, 64, 64>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, sa, sb, sc, ws, ws_bytes, cmap, s);
+    case 14: return dispatch_ops<float, 2, 64, 64>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, sa, sb, sc, ws, ws_bytes, cmap, s);
     default: return -1;
   }
 }

@@ -61,10 +61,24 @@ int launch_gemm_split(int method, char ta, char tb, int64_t M, int64_t N, int64_
 // ws (ws_bytes >= gemm_tail_workspace(...)) lets a GEMM whose last round of
 // tiles would leave most of the GPU idle compute those tiles in K ranges;
 // without it every tile runs whole.  The result depends only on the shapes.
+// cmap (device, M + N entries; batch 1, not the split R4 GEMM): element (m, n)
+// of C is at C[cmap[m] + cmap[M + n]] (elements), so the GEMM can read and
+// write a tensor whose M and N indices interleave in memory (ldc unused).
 size_t gemm_tail_workspace(int dk, int64_t M, int64_t N, int64_t K, int64_t batch);
 int launch_gemm(int dk, int r4mode, char ta, char tb, int conja, int conjb, int64_t M, int64_t N, int64_t K,
                 double ar, double ai, const void *A, int64_t lda, const void *B, int64_t ldb, int beta_one, void *C,
                 int64_t ldc, hipStream_t s, int64_t batch = 1, int64_t sa = 0, int64_t sb = 0, int64_t sc = 0,
-                void *ws = nullptr, size_t ws_bytes = 0);
+                void *ws = nullptr, size_t ws_bytes = 0, const int64_t *cmap = nullptr);
+
+// Offsets of a C map: for the mixed-radix index m over extents ext[0..nm)
+// (first fastest), out[m] = sum of digit_t * str[t]; out[M + n] likewise
+// over ext[nm..nm+nn) with M = prod ext[0..nm), N = prod ext[nm..nm+nn).
+constexpr int kMaxMapDims = 64;
+struct OffsetMap {
+  int nm = 0, nn = 0;
+  int64_t M = 1, N = 1;
+  int64_t ext[kMaxMapDims], str[kMaxMapDims];
+};
+int launch_offset_map(const OffsetMap &map, int64_t *out, hipStream_t s);
 
 }  // namespace tal

@@ -348,11 +348,37 @@ std::vector<int> dims_of(const talsh_tens_t *t) {
 
 }  // namespace
 
+namespace {
+
+// Whether the GEMM writes a permuted D in place through a C map instead of
+// transposing D into a temporary and back: the leading dims of D' must
+// multiply to exactly M and the next ones to N (no batch, and the split R4
+// GEMM has no C map), and D's stride-1 dim must lead D' with an extent of at
+// least 16, so that the GEMM's and the K-split summing pass's stores (runs
+// along m) stay contiguous.  Sets the number of D' dims in each group.
+bool d_in_place(const ContractPlan &plan, int dk, int *nm, int *nn) {
+  if (!plan.d_transpose || plan.batch != 1 || (dk == R4 && rt().r4mode == R4_SPLIT_BF16)) return false;
+  if (plan.d_o2n.empty() || plan.d_o2n[0] != 0 || plan.dprime_dims[0] < 16) return false;
+  const int r = (int)plan.dprime_dims.size();
+  if (r > kMaxMapDims) return false;
+  int t = 0;
+  int64_t pm = 1, pn = 1;
+  while (t < r && pm < plan.M) pm *= plan.dprime_dims[t++];
+  *nm = t;
+  while (t < r && pn < plan.N) pn *= plan.dprime_dims[t++];
+  *nn = t - *nm;
+  return pm == plan.M && pn == plan.N;
+}
+
+}  // namespace
+
 TtgtTemps ttgt_temp_bytes(const ContractPlan &plan, int dk, size_t lbytes, size_t rbytes, size_t dbytes) {
   TtgtTemps t;
   if (plan.a.transpose) t.a = plan.swap ? rbytes : lbytes;
   if (plan.b.transpose) t.b = plan.swap ? lbytes : rbytes;
-  if (plan.d_transpose) t.d = dbytes;
+  int nm = 0, nn = 0;
+  if (d_in_place(plan, dk, &nm, &nn)) t.d = (size_t)(plan.M + plan.N) * sizeof(int64_t);  // the C map
+  else if (plan.d_transpose) t.d = dbytes;
   if (dk == R4 && rt().r4mode == R4_SPLIT_BF16) t.w = gemm_split_workspace(rt().split_method, plan.M, plan.N, plan.K);
   else t.s = gemm_tail_workspace(dk, plan.M, plan.N, plan.K, plan.batch);
   return t;
@@ -377,7 +403,29 @@ int enqueue_ttgt(const DigitalPattern &p, const ContractPlan &plan, int dk, bool
       return TALSH_FAILURE;
     B = tmp.b;
   }
-  if (plan.d_transpose) {
+  int nm = 0, nn = 0;
+  const bool in_place = d_in_place(plan, dk, &nm, &nn);
+  const int64_t *cmap = nullptr;
+  if (in_place) {
+    // D' dim t is D dim i with d_o2n[i] == t; its D stride is the product of the extents before i
+    if (!tmp.d || (int)plan.d_o2n.size() != p.drank) return TALSH_FAILURE;
+    OffsetMap map;
+    map.nm = nm;
+    map.nn = nn;
+    map.M = plan.M;
+    map.N = plan.N;
+    int64_t stride = 1;
+    for (int i = 0; i < p.drank; ++i) {
+      const int t = plan.d_o2n[i];
+      if (t < nm + nn) {
+        map.ext[t] = ddims[i];
+        map.str[t] = stride;
+      }
+      stride *= ddims[i];
+    }
+    if (launch_offset_map(map, (int64_t *)tmp.d, s)) return TALSH_FAILURE;
+    cmap = (const int64_t *)tmp.d;
+  } else if (plan.d_transpose) {
     if (!tmp.d) return TALSH_FAILURE;
     C = tmp.d;
     if (acc && launch_permute(dk, D, C, p.drank, ddims, plan.d_o2n.data(), PERM_COPY, 1, 0, 0, s))
@@ -399,11 +447,11 @@ int enqueue_ttgt(const DigitalPattern &p, const ContractPlan &plan, int dk, bool
         return TALSH_FAILURE;
   } else if (launch_gemm(dk, R4_FP32, plan.a.op, plan.b.op, plan.swap ? rconj : lconj, plan.swap ? lconj : rconj,
                          plan.M, plan.N, plan.K, ar, cplx ? ai : 0.0, A, plan.a.ld, B, plan.b.ld, acc ? 1 : 0, C,
-                         std::max<int64_t>(1, plan.M), s, plan.batch, sa, sb, sc, tmp.s, tmp.s_bytes)) {
+                         std::max<int64_t>(1, plan.M), s, plan.batch, sa, sb, sc, tmp.s, tmp.s_bytes, cmap)) {
     return TALSH_FAILURE;
   }
   if (gemm_end) (void)hipEventRecord(gemm_end, s);
-  if (plan.d_transpose &&
+  if (plan.d_transpose && !in_place &&
       launch_permute(dk, C, D, (int)plan.dprime_dims.size(), plan.dprime_dims.data(), plan.d_back_o2n.data(),
                      PERM_COPY, 1, 0, 0, s))
     return TALSH_FAILURE;

@@ -223,6 +223,7 @@ int plan_contraction(const DigitalPattern &p, const int *ddims, const int *ldims
   bool have = false;
   ContractPlan best;
   int best_ntrans = 99;
+  bool best_d0_first = false;
   for (int swap = 0; swap < 2; ++swap) {
     const std::vector<int> &sx = swap ? sr : sl, &sy = swap ? sl : sr;
     const int *xdims = swap ? rdims : ldims, *ydims = swap ? ldims : rdims;
@@ -279,10 +280,15 @@ int plan_contraction(const DigitalPattern &p, const int *ddims, const int *ldims
             for (int i = 0; i < nd; ++i) c.dprime_dims[i] = (int)ext_of[dseq[i]];
           }
           const int ntrans = (int)c.a.transpose + (int)c.b.transpose + (int)c.d_transpose;
+          // among equal plans, one whose D' starts with D's first (stride-1) index lets the GEMM
+          // write D in place with m-contiguous stores
+          const bool d0_first = !c.d_transpose || c.d_o2n[0] == 0;
           if (!have || c.moved_elems < best.moved_elems ||
-              (c.moved_elems == best.moved_elems && ntrans < best_ntrans)) {
+              (c.moved_elems == best.moved_elems &&
+               (ntrans < best_ntrans || (ntrans == best_ntrans && d0_first && !best_d0_first)))) {
             best = c;
             best_ntrans = ntrans;
+            best_d0_first = d0_first;
             have = true;
           }
           (void)nx;
