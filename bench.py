@@ -86,16 +86,20 @@ def timed_contractions(pat, d, l, r, steps, warmup, alpha=1.0):
 
 
 def sampled_error(oracle, dk, pat, dims, D0, L, R, Dout, alpha, nsamples=64, seed=5):
-    """Normwise relative error on sampled elements (each a K-term dot product in long double)."""
+    """Errors on sampled elements (each a K-term dot product in long double): normwise relative error
+    ||D - D*|| / ||D*|| and max |D - D*| / max |D*| over the samples."""
     rng = np.random.default_rng(seed)
     idx = rng.choice(D0.size, size=nsamples, replace=False)
     num = den = 0.0
+    dmax = wmax = 0.0
     for off in idx:
         el = oracle.contract_element(dk, pat, dims, int(off), dims, L, dims, R)
         want = complex(D0[off]) + alpha * el
-        num += abs(complex(Dout[off]) - want) ** 2
+        diff = abs(complex(Dout[off]) - want)
+        num += diff ** 2
         den += abs(want) ** 2
-    return float(np.sqrt(num / den))
+        dmax, wmax = max(dmax, diff), max(wmax, abs(want))
+    return float(np.sqrt(num / den)), float(dmax / wmax)
 
 
 def bench_contraction(oracle, dk, pat, n, steps, warmup, check=True, seed=100):
@@ -113,7 +117,8 @@ def bench_contraction(oracle, dk, pat, n, steps, warmup, check=True, seed=100):
         # D now holds D0 + (warmup+steps) * L*R: check against the oracle on samples
         out = d.host_array()
         reps = warmup + steps
-        res["rel_err_vs_fp64"] = sampled_error(oracle, dk, pat, dims, D0, L, R, out, float(reps))
+        res["rel_err_vs_fp64"], res["max_abs_err_ratio"] = sampled_error(oracle, dk, pat, dims, D0, L, R, out,
+                                                                         float(reps))
     for t in (l, r, d):
         t.destruct()
     return res
@@ -178,7 +183,8 @@ def bench_xl(oracle, n=96, grant=2 << 30, reps=3, seed=300):
     ts.set_xl_memory(0)
     # D = D0 + (2 + 2 * reps) * L*R after the calls above
     calls = 2 * (1 + reps)
-    out["rel_err_vs_fp64"] = sampled_error(oracle, R8, pat, dims, D0, L, R, d.host_array(), float(calls))
+    out["rel_err_vs_fp64"], out["max_abs_err_ratio"] = sampled_error(oracle, R8, pat, dims, D0, L, R,
+                                                                     d.host_array(), float(calls))
     for x in (l, r, d):
         x.destruct()
     return out
@@ -680,7 +686,14 @@ def cpu_baseline(budget_s=30.0):
         run()
         times.append(time.perf_counter() - t0)
     t = float(np.median(times))
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((x.split(":", 1)[1].strip() for x in f if x.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": flop / t / 1e12, "unit": "TFLOP/s", "cores": cores, "kind": kind,
+            "nproc": os.cpu_count(), "cpu_model": model,
             "sample": "D(a,b,c,d)+=L(a,i,c,j)*R(j,b,i,d) R8 with c=d=32, others 96 (M=N=3072, K=9216; "
                       "1.74e11 FLOP), median of %d runs, %.2f s each, OMP threads %d" % (len(times), t, cores)}
 
@@ -788,6 +801,7 @@ def main():
                                "(GEMM M=N=K=9216), GPU-resident operands, accumulative, alpha 1",
                    "parallelism": "single GPU"},
         "rel_err_vs_fp64": head.get("rel_err_vs_fp64"),
+        "max_abs_err_ratio": head.get("max_abs_err_ratio"),
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
                      "traffic": profiled_traffic("gemm_kernel<double, 1, 1, 0, 0, 0, 128, 128, 2>"),
