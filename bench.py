@@ -698,6 +698,14 @@ def cpu_baseline(budget_s=30.0):
                       "1.74e11 FLOP), median of %d runs, %.2f s each, OMP threads %d" % (len(times), t, cores)}
 
 
+def gemm_io_lower_bound(m, n, k, esize, xcds=8, l2_bytes=4 << 20):
+    """Bytes an m x n x k GEMM must move between L2 and memory with `xcds` L2 caches of l2_bytes: each XCD
+    computes mnk / xcds multiply-adds, and a processor with S words of fast memory moves at least
+    2 * (its multiply-adds) / sqrt(S) words (classical algorithm); plus C read and written once."""
+    s = l2_bytes / esize
+    return int(xcds * 2.0 * (m * n * k / xcds) / np.sqrt(s) * esize + 2 * m * n * esize)
+
+
 def check_init(err):
     if err:
         raise RuntimeError("talshInit failed: %d" % err)
@@ -807,6 +815,8 @@ def main():
                      "traffic": profiled_traffic("gemm_kernel<double, 1, 1, 0, 0, 0, 128, 128, 2>"),
                      "traffic_source": "profiles/contract_r8_96_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
                                        "copy-calibrated; bytes per GEMM stage)",
+                     # classical-GEMM I/O lower bound with 4 MiB of L2 per XCD (8 XCDs), plus D read and written once
+                     "traffic_io_lower_bound": gemm_io_lower_bound(9216, 9216, 9216, 8),
                      "kernel": "FP64 MFMA GEMM stage: whole-tile launch + K-split launch of the last round + "
                                "summing pass (2*M*N*K FLOP / HIP-event duration of the stage)"},
         "cpu_baseline": cpu,
