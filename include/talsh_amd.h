@@ -58,9 +58,8 @@ int talsh_get_contr_ptrn_str2dig(const char *c_str, int *dig_ptrn, int *drank, i
  * operands (A, then B, then D->D'; -1 where not transposed); moved = elements
  * copied by the transposes.  When D' begins with D's first index (extent at
  * least 16) and there is no hyper index, the GEMM reads and writes D in place
- * through per-row and per-column offsets instead of copying it (except for the
- * split R4 GEMM of fast math); `moved` still counts D.  Host-only query, no
- * device work. */
+ * through per-row and per-column offsets instead of copying it; `moved` still
+ * counts D.  Host-only query, no device work. */
 int talshAmdPlanContraction(const char *pattern, const int *ddims, const int *ldims, const int *rdims,
                             int accumulative, long long *mnk, int *flags, char *ops, int *perms, double *moved);
 

@@ -703,7 +703,8 @@ __device__ __forceinline__ void presplit_gemm(int64_t M, int64_t N, int64_t Kp, 
                                               float *__restrict__ C, int64_t ldc, float alpha, int beta_one,
                                               int64_t gx, int64_t gy, int64_t t0, int64_t count, int ksplit,
                                               int64_t ksteps, float *__restrict__ part,
-                                              const uint32_t *__restrict__ UA, const uint32_t *__restrict__ UB) {
+                                              const uint32_t *__restrict__ UA, const uint32_t *__restrict__ UB,
+                                              const int64_t *__restrict__ cmap) {
   __shared__ uint4 As[2][NP][PBM * 4];
   __shared__ uint4 Bs[2][NP][PBN * 4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -840,7 +841,7 @@ __device__ __forceinline__ void presplit_gemm(int64_t M, int64_t N, int64_t Kp, 
         const int64_t m = bx * PBM + wm * 64 + i * 16 + 4 * (lane >> 4) + r;
         const int64_t n = by * PBN + wn * 64 + j * 16 + (lane & 15);
         if (m < M && n < N) {
-          float *c = C + m + n * ldc;
+          float *c = C + (cmap ? cmap[m] + cmap[M + n] : m + n * ldc);
           const float s = big[i][j][r] + small[i][j][r];
           const float v = alpha * (NP == 2 ? ldexpf(s, ea[i][r] + eb[j]) : s);
           *c = beta_one ? *c + v : v;
@@ -854,9 +855,9 @@ __global__ __launch_bounds__(512) void gemm_bf16x6p_kernel(int64_t M, int64_t N,
                                                            float *__restrict__ C, int64_t ldc, float alpha,
                                                            int beta_one, int64_t gx, int64_t gy, int64_t t0,
                                                            int64_t count, int ksplit, int64_t ksteps,
-                                                           float *__restrict__ part) {
+                                                           float *__restrict__ part, const int64_t *__restrict__ cmap) {
   presplit_gemm<3>(M, N, Kp, Ap, aplane, Bp, bplane, C, ldc, alpha, beta_one, gx, gy, t0, count, ksplit, ksteps,
-                   part, nullptr, nullptr);
+                   part, nullptr, nullptr, cmap);
 }
 
 __global__ __launch_bounds__(512) void gemm_f16x3p_kernel(int64_t M, int64_t N, int64_t Kp,
@@ -866,9 +867,10 @@ __global__ __launch_bounds__(512) void gemm_f16x3p_kernel(int64_t M, int64_t N, 
                                                           int beta_one, int64_t gx, int64_t gy, int64_t t0,
                                                           int64_t count, int ksplit, int64_t ksteps,
                                                           float *__restrict__ part, const uint32_t *__restrict__ UA,
-                                                          const uint32_t *__restrict__ UB) {
+                                                          const uint32_t *__restrict__ UB,
+                                                          const int64_t *__restrict__ cmap) {
   presplit_gemm<2>(M, N, Kp, Ap, aplane, Bp, bplane, C, ldc, alpha, beta_one, gx, gy, t0, count, ksplit, ksteps,
-                   part, UA, UB);
+                   part, UA, UB, cmap);
 }
 
 int64_t round_up(int64_t x, int64_t q) { return (x + q - 1) / q * q; }
@@ -1132,7 +1134,8 @@ static SplitLayout split_layout(int64_t M, int64_t N, int64_t K) {
 
 template <int NP>
 static int launch_presplit(int opa, int opb, int64_t M, int64_t N, int64_t K, double ar, const void *A, int64_t lda,
-                           const void *B, int64_t ldb, int beta_one, void *C, int64_t ldc, void *ws, hipStream_t s) {
+                           const void *B, int64_t ldb, int beta_one, void *C, int64_t ldc, void *ws,
+                           const int64_t *cmap, hipStream_t s) {
   const SplitLayout l = split_layout<NP>(M, N, K);
   char *w = (char *)ws;
   uint16_t *pa = (uint16_t *)(w + l.a), *pb = (uint16_t *)(w + l.b);
@@ -1151,18 +1154,18 @@ static int launch_presplit(int opa, int opb, int64_t M, int64_t N, int64_t K, do
     if (NP == 3)
       gemm_bf16x6p_kernel<<<dim3((unsigned)blocks), dim3(512), 0, s>>>(
           M, N, l.Kp, (const uint4 *)pa, apl, (const uint4 *)pb, bpl, (float *)C, ldc, (float)ar, beta_one, gx, gy,
-          t0, count, ksplit, kst, pt);
+          t0, count, ksplit, kst, pt, cmap);
     else
       gemm_f16x3p_kernel<<<dim3((unsigned)blocks), dim3(512), 0, s>>>(
           M, N, l.Kp, (const uint4 *)pa, apl, (const uint4 *)pb, bpl, (float *)C, ldc, (float)ar, beta_one, gx, gy,
-          t0, count, ksplit, kst, pt, ua, ub);
+          t0, count, ksplit, kst, pt, ua, ub, cmap);
   };
   if (full > 0) gemm(full, 0, full, 1, 0, nullptr);
   if (ts.ksplit > 1) {
     gemm(ts.rem * ts.ksplit, full, ts.rem, ts.ksplit, ksteps, part);
     const int64_t nel = ts.rem * PBM * PBN;
     gemm_tail_reduce<float, 1, PBM, PBN><<<dim3((unsigned)((nel + kThreads - 1) / kThreads)), dim3(kThreads), 0, s>>>(
-        M, N, part, ts.ksplit, (float *)C, ldc, (float)ar, 0.0f, beta_one, gx, gy, full, ts.rem, nullptr);
+        M, N, part, ts.ksplit, (float *)C, ldc, (float)ar, 0.0f, beta_one, gx, gy, full, ts.rem, cmap);
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
@@ -1183,7 +1186,7 @@ size_t gemm_split_workspace(int method, int64_t M, int64_t N, int64_t K) {
 
 int launch_gemm_split(int method, char ta, char tb, int64_t M, int64_t N, int64_t K, double ar, const void *A,
                       int64_t lda, const void *B, int64_t ldb, int beta_one, void *C, int64_t ldc, void *ws,
-                      hipStream_t s) {
+                      hipStream_t s, const int64_t *cmap) {
   const int opa = op_code(ta), opb = op_code(tb);
   if (opa < 0 || opb < 0 || M < 0 || N < 0 || K < 0 || !A || !B || !C || !ws || (uintptr_t)ws % 16) return -1;
   if (method != SPLIT_BF16X6 && method != SPLIT_FP16X3) return -1;
@@ -1191,8 +1194,9 @@ int launch_gemm_split(int method, char ta, char tb, int64_t M, int64_t N, int64_
   if (ldc < std::max<int64_t>(1, M)) return -1;
   if (lda < std::max<int64_t>(1, opa == 0 ? M : K)) return -1;
   if (ldb < std::max<int64_t>(1, opb == 0 ? K : N)) return -1;
-  return method == SPLIT_FP16X3 ? launch_presplit<2>(opa, opb, M, N, K, ar, A, lda, B, ldb, beta_one, C, ldc, ws, s)
-                                : launch_presplit<3>(opa, opb, M, N, K, ar, A, lda, B, ldb, beta_one, C, ldc, ws, s);
+  return method == SPLIT_FP16X3
+             ? launch_presplit<2>(opa, opb, M, N, K, ar, A, lda, B, ldb, beta_one, C, ldc, ws, cmap, s)
+             : launch_presplit<3>(opa, opb, M, N, K, ar, A, lda, B, ldb, beta_one, C, ldc, ws, cmap, s);
 }
 
 size_t gemm_tail_workspace(int dk, int64_t M, int64_t N, int64_t K, int64_t batch) {

@@ -45,12 +45,13 @@ bool gemm_split_bf16_available();
 enum SplitMethod { SPLIT_BF16X6 = 0, SPLIT_FP16X3 = 1 };
 
 // Split R4 GEMM with pre-split operands (same contract as launch_gemm for
-// dk = R4); ws is device scratch of gemm_split_workspace(method, M, N, K)
-// bytes, 16-byte aligned, that holds the planes of both operands.
+// dk = R4, batch 1, including cmap); ws is device scratch of
+// gemm_split_workspace(method, M, N, K) bytes, 16-byte aligned, that holds
+// the planes of both operands.
 size_t gemm_split_workspace(int method, int64_t M, int64_t N, int64_t K);
 int launch_gemm_split(int method, char ta, char tb, int64_t M, int64_t N, int64_t K, double ar, const void *A,
                       int64_t lda, const void *B, int64_t ldb, int beta_one, void *C, int64_t ldc, void *ws,
-                      hipStream_t s);
+                      hipStream_t s, const int64_t *cmap = nullptr);
 
 // C(MxN, col-major, ldc) = alpha * op(A) * op(B) + beta * C, beta in {0,1}.
 // ta: 'N' A is MxK (lda>=M); 'T' A is KxM (lda>=K).
@@ -61,7 +62,7 @@ int launch_gemm_split(int method, char ta, char tb, int64_t M, int64_t N, int64_
 // ws (ws_bytes >= gemm_tail_workspace(...)) lets a GEMM whose last round of
 // tiles would leave most of the GPU idle compute those tiles in K ranges;
 // without it every tile runs whole.  The result depends only on the shapes.
-// cmap (device, M + N entries; batch 1, not the split R4 GEMM): element (m, n)
+// cmap (device, M + N entries; batch 1): element (m, n)
 // of C is at C[cmap[m] + cmap[M + n]] (elements), so the GEMM can read and
 // write a tensor whose M and N indices interleave in memory (ldc unused).
 size_t gemm_tail_workspace(int dk, int64_t M, int64_t N, int64_t K, int64_t batch);

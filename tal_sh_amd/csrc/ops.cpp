@@ -352,12 +352,12 @@ namespace {
 
 // Whether the GEMM writes a permuted D in place through a C map instead of
 // transposing D into a temporary and back: the leading dims of D' must
-// multiply to exactly M and the next ones to N (no batch, and the split R4
-// GEMM has no C map), and D's stride-1 dim must lead D' with an extent of at
-// least 16, so that the GEMM's and the K-split summing pass's stores (runs
-// along m) stay contiguous.  Sets the number of D' dims in each group.
-bool d_in_place(const ContractPlan &plan, int dk, int *nm, int *nn) {
-  if (!plan.d_transpose || plan.batch != 1 || (dk == R4 && rt().r4mode == R4_SPLIT_BF16)) return false;
+// multiply to exactly M and the next ones to N (no batch), and D's stride-1
+// dim must lead D' with an extent of at least 16, so that the GEMM's and the
+// K-split summing pass's stores (runs along m) stay contiguous.  Sets the
+// number of D' dims in each group.
+bool d_in_place(const ContractPlan &plan, int *nm, int *nn) {
+  if (!plan.d_transpose || plan.batch != 1) return false;
   if (plan.d_o2n.empty() || plan.d_o2n[0] != 0 || plan.dprime_dims[0] < 16) return false;
   const int r = (int)plan.dprime_dims.size();
   if (r > kMaxMapDims) return false;
@@ -377,7 +377,7 @@ TtgtTemps ttgt_temp_bytes(const ContractPlan &plan, int dk, size_t lbytes, size_
   if (plan.a.transpose) t.a = plan.swap ? rbytes : lbytes;
   if (plan.b.transpose) t.b = plan.swap ? lbytes : rbytes;
   int nm = 0, nn = 0;
-  if (d_in_place(plan, dk, &nm, &nn)) t.d = (size_t)(plan.M + plan.N) * sizeof(int64_t);  // the C map
+  if (d_in_place(plan, &nm, &nn)) t.d = (size_t)(plan.M + plan.N) * sizeof(int64_t);  // the C map
   else if (plan.d_transpose) t.d = dbytes;
   if (dk == R4 && rt().r4mode == R4_SPLIT_BF16) t.w = gemm_split_workspace(rt().split_method, plan.M, plan.N, plan.K);
   else t.s = gemm_tail_workspace(dk, plan.M, plan.N, plan.K, plan.batch);
@@ -404,7 +404,7 @@ int enqueue_ttgt(const DigitalPattern &p, const ContractPlan &plan, int dk, bool
     B = tmp.b;
   }
   int nm = 0, nn = 0;
-  const bool in_place = d_in_place(plan, dk, &nm, &nn);
+  const bool in_place = d_in_place(plan, &nm, &nn);
   const int64_t *cmap = nullptr;
   if (in_place) {
     // D' dim t is D dim i with d_o2n[i] == t; its D stride is the product of the extents before i
@@ -443,7 +443,7 @@ int enqueue_ttgt(const DigitalPattern &p, const ContractPlan &plan, int dk, bool
     for (int64_t q = 0; q < plan.batch; ++q)  // the planes workspace is reused in stream order
       if (launch_gemm_split(rt().split_method, plan.a.op, plan.b.op, plan.M, plan.N, plan.K, ar, (const float *)A + q * sa, plan.a.ld,
                             (const float *)B + q * sb, plan.b.ld, acc ? 1 : 0, (float *)C + q * sc,
-                            std::max<int64_t>(1, plan.M), tmp.w, s))
+                            std::max<int64_t>(1, plan.M), tmp.w, s, cmap))
         return TALSH_FAILURE;
   } else if (launch_gemm(dk, R4_FP32, plan.a.op, plan.b.op, plan.swap ? rconj : lconj, plan.swap ? lconj : rconj,
                          plan.M, plan.N, plan.K, ar, cplx ? ai : 0.0, A, plan.a.ld, B, plan.b.ld, acc ? 1 : 0, C,
