@@ -421,7 +421,7 @@ int talshTensorContractXL(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *
       const size_t fa = plan.swap ? rv : lv, fb = plan.swap ? lv : rv;
       grow(want.a, have.a, tmp.a, fa);
       grow(want.b, have.b, tmp.b, fb);
-      grow(want.d, have.d, tmp.d, dv);
+      grow(want.d, have.d, tmp.d, want.d == pdv ? dv : 0);  // a D' temporary, or the GEMM's small offset table
       grow(want.w, have.w, tmp.w, 0);
       grow(want.s, have.s, tmp.s, 0);
       tmp.s_bytes = have.s;
