@@ -197,6 +197,51 @@ def test_contract_random_all_kinds(talsh_gpu, oracle):
             assert relerr(dt.host_array(), want) <= tol, (dk, pat)
 
 
+def test_contract_interleaved_tile_sized_all_kinds(talsh_gpu, oracle):
+    """Interleaved patterns at extents that fill GEMM tiles and K steps (16, 32, 48, with some small ones):
+    the configs' patterns (D written in place through offset tables), a K-split last round (small M, N),
+    random patterns, conjugation, accumulative and not.  Same bar as the other contractions; L and R are
+    left unchanged."""
+    ts = talsh_gpu
+    rng = random.Random(4242)
+    cases = [("D(a,b,c,d)+=L(a,i,c,j)*R(j,b,i,d)", 16), ("D(a,b,c,d)+=L(c,i,b,j)*R(d,j,a,i)", 16),
+             ("D(a,b)+=L(i,a,j)*R(j,b,i)", 32), ("D(a,b,c)+=L(c,i,a)*R(i,b)", 48)]
+    for t in range(len(cases) + 24):
+        dk = KINDS[t % 4]
+        if t < len(cases):
+            pat, n = cases[t]
+            syms = lambda part: pat.split(part + "(")[1].split(")")[0].split(",")
+            size = {"a": n, "b": n, "c": n, "d": n, "i": 32 if n == 48 else n, "j": n}
+            if pat.startswith("D(a,b)+=L(i,a,j)"):
+                size = {"a": 24, "b": 40, "i": 32, "j": 64}  # small M, N: K split of the last round
+            dd, ld, rd = ([size[x] for x in syms(part)] for part in ("D", "L", "R"))
+        else:
+            pat, dd, ld, rd = random_contraction(rng, max_rank=4, ext_lo=1, ext_hi=3)
+            # replace extents by multiples of 16 (most) or small ones, consistently per index
+            labels = {}
+            for part, dims in (("D", dd), ("L", ld), ("R", rd)):
+                for x in pat.split(part + "(")[1].split(")")[0].split(","):
+                    if x and x not in labels:
+                        labels[x] = rng.choice([16, 32, 16, 48, 3, 5]) if len(labels) < 4 else rng.choice([2, 3])
+            dd, ld, rd = ([labels[x] for x in pat.split(part + "(")[1].split(")")[0].split(",") if x]
+                          for part in ("D", "L", "R"))
+        if dk in (C4, C8) and t % 3 == 1:
+            pat = pat.replace("L(", "L+(") if t % 2 else pat.replace("R(", "R+(")
+        acc = t % 2
+        alpha = complex(0.5, 0.25) if dk in (C4, C8) else 0.75
+        say("interleaved", dk, pat, dd, ld, rd, "acc", acc)
+        L, R, D = tensor(dk, ld, t), tensor(dk, rd, t + 1), tensor(dk, dd, t + 2)
+        lt, rt, dt = on_gpu(ts, dk, ld, L), on_gpu(ts, dk, rd, R), on_gpu(ts, dk, dd, D)
+        ts.check(ts.contract(pat, dt, lt, rt, alpha=alpha, accumulative=acc), "contract")
+        want = oracle.contract(dk, pat, dd, D, ld, L, rd, R, (alpha.real, alpha.imag) if dk in (C4, C8)
+                               else (alpha, 0.0), acc)
+        tol = 1e-12 if dk in (R8, C8) else 1e-5
+        assert relerr(dt.host_array(), want) <= tol, (dk, pat, dd, ld, rd)
+        assert np.array_equal(lt.host_array(), L) and np.array_equal(rt.host_array(), R)
+        for x in (lt, rt, dt):
+            x.destruct()
+
+
 def test_contract_hyper_all_kinds(talsh_gpu, oracle):
     """Hyper indices (in D, L and R) run as a strided batched MFMA GEMM; compared with the oracle's
     direct loop nest.  Includes a batch of 70000 entries (more than one launch's grid.y)."""
