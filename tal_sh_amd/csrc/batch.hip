@@ -171,8 +171,10 @@ __device__ __forceinline__ void tile_body(const BatchTask &T, long long local, c
 }
 
 // The index of the next work item is fetched (one atomic by thread 0) while
-// the current tile is computed, and published through a two-slot LDS buffer,
-// so neither the atomic's round trip nor a task search sits between tiles.
+// the current tile is computed: the atomic's result stays in a register until
+// the tile is done (the tile's barriers wait for LDS traffic only) and is then
+// published through a two-slot LDS buffer, so neither the atomic's round trip
+// nor a task search sits between tiles.
 template <typename S>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void batch_contract_kernel(
     const BatchTask *__restrict__ tasks, const long long *__restrict__ prefix, const int *__restrict__ item_task,
@@ -184,8 +186,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
   __syncthreads();
   long long item = s_item[0];
   for (int it = 1; item < total; ++it) {
-    // slot it & 1 was last read before the barriers of the previous tile
-    if (tid == 0) s_item[it & 1] = (long long)atomicAdd(counter, 1ull);
+    unsigned long long next = 0;
+    if (tid == 0) next = atomicAdd(counter, 1ull);
     const int t = item_task[item];
     const BatchTask &T = tasks[t];
     const long long local = item - prefix[t];
@@ -195,6 +197,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
       case 2: tile_body<S, 1, 0>(T, local, tabs, sm, tid); break;
       default: tile_body<S, 1, 1>(T, local, tabs, sm, tid); break;
     }
+    // slot it & 1 was last read before the barriers of the previous tile
+    if (tid == 0) s_item[it & 1] = (long long)next;
     __syncthreads();  // the next index is visible; every thread leaves the loop together
     item = s_item[it & 1];
   }
