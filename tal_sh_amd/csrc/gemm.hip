@@ -626,6 +626,60 @@ __global__ __launch_bounds__(256) void row_absmax_kernel(const float *__restrict
   }
 }
 
+// fp16x3 planes of op(X) when its rows are K-contiguous (X(r, k) at
+// X[r*ld + k]): one block per padded row finds the row's largest magnitude
+// (U[r], 0 for padding rows), then splits the row, whose second read comes
+// from L2, so HBM sees each element once.  vec as for row_absmax_kernel.
+__global__ __launch_bounds__(256) void split_rows_f16_kernel(const float *__restrict__ X, int64_t ld, int64_t R,
+                                                             int64_t K, int64_t Kp, uint16_t *__restrict__ P,
+                                                             int64_t plane, int vec, uint32_t *__restrict__ U) {
+  __shared__ uint32_t red[4];
+  const int64_t r = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const float *x = X + r * ld;
+  uint32_t m = 0;
+  if (r < R) {
+    int64_t k = 0;
+    if (vec) {
+      const int64_t kv = K & ~(int64_t)3;
+      for (int64_t q = 4 * tid; q < kv; q += 1024) {
+        const float4 v = *reinterpret_cast<const float4 *>(x + q);
+        m = max(m, max(max(abs_bits(v.x), abs_bits(v.y)), max(abs_bits(v.z), abs_bits(v.w))));
+      }
+      k = kv;
+    }
+    for (k += tid; k < K; k += 256) m = max(m, abs_bits(x[k]));
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o));
+  if (lane == 0) red[wave] = m;
+  __syncthreads();
+  m = max(max(red[0], red[1]), max(red[2], red[3]));
+  if (tid == 0) U[r] = m;
+  const int e = row_exp(m);
+  for (int64_t kc = tid; kc < Kp / 8; kc += 256) {
+    const int64_t k0 = kc * 8;
+    float v[8];
+    if (vec && r < R && k0 + 8 <= K) {
+      const float4 a = *reinterpret_cast<const float4 *>(x + k0);
+      const float4 b = *reinterpret_cast<const float4 *>(x + k0 + 4);
+      v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+      v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    } else {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) v[i] = (r < R && k0 + i < K) ? x[k0 + i] : 0.0f;
+    }
+    _Float16 h[8], l[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) split2_scaled(v[i], e, h[i], l[i]);
+    uint4 u;
+    __builtin_memcpy(&u, h, 16);
+    *reinterpret_cast<uint4 *>(P + r * Kp + k0) = u;
+    __builtin_memcpy(&u, l, 16);
+    *reinterpret_cast<uint4 *>(P + plane + r * Kp + k0) = u;
+  }
+}
+
 // KC = 1: source element (r, k) at X[r*ld + k]; KC = 0: at X[r + k*ld].
 // One thread per 8 consecutive k of one row (KC = 1: lanes along k, KC = 0:
 // lanes along r, so source reads are coalesced either way).  NP = 3: BF16
@@ -896,6 +950,11 @@ int split_operand(bool kcont, const float *X, int64_t ld, int64_t R, int64_t K, 
   const int64_t items = Rp * (Kp / 8), nblk = (items + 255) / 256;
   if (nblk > 0x7fffffffLL) return -4;
   const int vec = ((uintptr_t)X % 16 == 0) && ld % 4 == 0;
+  if (NP == 2 && kcont) {  // maxima and planes in one pass over HBM
+    if (Rp > 0x7fffffffLL) return -4;
+    split_rows_f16_kernel<<<dim3((unsigned)Rp), dim3(256), 0, s>>>(X, ld, R, K, Kp, P, Rp * Kp, vec, U);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+  }
   if constexpr (NP == 2) {
     fill_u32_kernel<<<dim3((unsigned)((Rp + 255) / 256)), dim3(256), 0, s>>>(U, Rp);
     if (R > 0 && K > 0) {
