@@ -97,7 +97,8 @@ __device__ inline void tile_of(int64_t gx, int64_t gy, int64_t t0, int64_t count
 // layout codes: 0 = N, 1 = T; CJA/CJB conjugate the complex operand; W is the
 // vector width in scalars along each operand's contiguous axis.
 template <typename S, int NC, int OPA, int OPB, int CJA, int CJB, int BM, int BN, int W>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void gemm_kernel(int64_t M, int64_t N, int64_t K, const S *__restrict__ A,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void gemm_kernel(
+                                                        int64_t M, int64_t N, int64_t K, const S *__restrict__ A,
                                                         int64_t lda, const S *__restrict__ B, int64_t ldb,
                                                         S *__restrict__ C, int64_t ldc, S alpha_r, S alpha_i,
                                                         int beta_one, int64_t gx, int64_t gy, int64_t bsa,
@@ -1159,7 +1160,8 @@ static TailSplit presplit_tail(int64_t M, int64_t N, int64_t K) {
   if (slots <= 0) return t;
   const int64_t rem = T % slots;
   if (rem == 0 || rem * 4 > slots) return t;
-  const int64_t ks = std::min<int64_t>(std::min<int64_t>(16, slots / rem), (round_up(std::max<int64_t>(K, 1), PBK) / PBK) / 8);
+  const int64_t ksteps = round_up(std::max<int64_t>(K, 1), PBK) / PBK;
+  const int64_t ks = std::min<int64_t>(std::min<int64_t>(16, slots / rem), ksteps / 8);
   if (ks < 2) return t;
   t.rem = rem;
   t.ksplit = (int)ks;
@@ -1295,13 +1297,17 @@ int launch_gemm(int dk, int r4mode, char ta, char tb, int conja, int conjb, int6
     }
     return 0;
   }
+#define TAL_GEMM_KIND(S, NC, BM, BN)                                                                            \
+  dispatch_ops<S, NC, BM, BN>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, sa, sb, \
+                              sc, ws, ws_bytes, cmap, s)
   switch (dk) {
-    case 8: return dispatch_ops<double, 1, 128, 128>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, sa, sb, sc, ws, ws_bytes, cmap, s);
-    case 4: return dispatch_ops<float, 1, 128, 128>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, sa, sb, sc, ws, ws_bytes, cmap, s);
-    case 18: return dispatch_ops<double, 2, 64, 64>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, sa, sb, sc, ws, ws_bytes, cmap, s);
-    case 14: return dispatch_ops<float, 2, 64, 64>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, sa, sb, sc, ws, ws_bytes, cmap, s);
+    case 8: return TAL_GEMM_KIND(double, 1, 128, 128);
+    case 4: return TAL_GEMM_KIND(float, 1, 128, 128);
+    case 18: return TAL_GEMM_KIND(double, 2, 64, 64);
+    case 14: return TAL_GEMM_KIND(float, 2, 64, 64);
     default: return -1;
   }
+#undef TAL_GEMM_KIND
 }
 
 }  // namespace tal

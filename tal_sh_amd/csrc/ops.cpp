@@ -441,7 +441,8 @@ int enqueue_ttgt(const DigitalPattern &p, const ContractPlan &plan, int dk, bool
   if (split) {
     if (!tmp.w) return TALSH_FAILURE;
     for (int64_t q = 0; q < plan.batch; ++q)  // the planes workspace is reused in stream order
-      if (launch_gemm_split(rt().split_method, plan.a.op, plan.b.op, plan.M, plan.N, plan.K, ar, (const float *)A + q * sa, plan.a.ld,
+      if (launch_gemm_split(rt().split_method, plan.a.op, plan.b.op, plan.M, plan.N, plan.K, ar,
+                            (const float *)A + q * sa, plan.a.ld,
                             (const float *)B + q * sb, plan.b.ld, acc ? 1 : 0, (float *)C + q * sc,
                             std::max<int64_t>(1, plan.M), tmp.w, s, cmap))
         return TALSH_FAILURE;
