@@ -324,6 +324,8 @@ def test_contract_split_r4(talsh_gpu, oracle, split):
         cases.append(("D(a,h,b)+=L(a,k,h)*R(k,h,b)", [70, 9, 50], [70, 200, 9], [200, 9, 50]))
         # 32 tiles of 256 x 128: too few for the GPU, run as K ranges plus a summing pass
         cases.append(("D(a,b)+=L(k,a)*R(k,b)", [1024, 1024], [2048, 1024], [2048, 1024]))
+        # D permuted (M = (a,c), N = (b,d)) and written in place through offset tables, one tile split over K
+        cases.append(("D(a,b,c,d)+=L(c,i,a,j)*R(j,d,i,b)", [16, 16, 2, 8], [2, 32, 16, 64], [64, 8, 32, 16]))
         for t, (pat, dd, ld, rd) in enumerate(cases):
             say("split", split, pat, dd, ld, rd)
             L, R, D = tensor(R4, ld, t), tensor(R4, rd, t + 1), tensor(R4, dd, t + 2)
