@@ -643,6 +643,7 @@ __global__ __launch_bounds__(256) void split_rows_f16_kernel(const float *__rest
     int64_t k = 0;
     if (vec) {
       const int64_t kv = K & ~(int64_t)3;
+#pragma unroll 8
       for (int64_t q = 4 * tid; q < kv; q += 1024) {
         const float4 v = *reinterpret_cast<const float4 *>(x + q);
         m = max(m, max(max(abs_bits(v.x), abs_bits(v.y)), max(abs_bits(v.z), abs_bits(v.w))));
@@ -658,6 +659,7 @@ __global__ __launch_bounds__(256) void split_rows_f16_kernel(const float *__rest
   m = max(max(red[0], red[1]), max(red[2], red[3]));
   if (tid == 0) U[r] = m;
   const int e = row_exp(m);
+#pragma unroll 4
   for (int64_t kc = tid; kc < Kp / 8; kc += 256) {
     const int64_t k0 = kc * 8;
     float v[8];
