@@ -263,3 +263,25 @@ def test_host_memory_policy():
     finally:
         lib.talshSetMemAllocPolicyHost(ts.MEM_ALLOC_TMP_BUF, 1, ctypes.byref(err))
         assert lib.talshShutdown() == 0
+
+
+def test_fast_math_split_selection():
+    """talshAmdSetFastMathSplit / talshAmdGetFastMathSplit (include/talsh_amd.h): bf16x6 by default, fp16x3 on
+    request, anything else refused and leaving the choice unchanged; fast math itself (talshEnableFastMath,
+    talsh.h:167-174) is queried per device kind.  No GPU needed."""
+    lib = ts.load()
+    assert lib.talshAmdGetFastMathSplit() == ts.SPLIT_BF16X6
+    try:
+        assert lib.talshAmdSetFastMathSplit(ts.SPLIT_FP16X3) == 0
+        assert lib.talshAmdGetFastMathSplit() == ts.SPLIT_FP16X3
+        assert lib.talshAmdSetFastMathSplit(7) == ts.TALSH_INVALID_ARGS
+        assert lib.talshAmdSetFastMathSplit(-1) == ts.TALSH_INVALID_ARGS
+        assert lib.talshAmdGetFastMathSplit() == ts.SPLIT_FP16X3
+        assert ts.fast_math(True, ts.SPLIT_BF16X6) == 0
+        assert lib.talshQueryFastMath(ts.DEV_NVIDIA_GPU, 0) == ts.YEP
+        assert lib.talshAmdGetFastMathSplit() == ts.SPLIT_BF16X6
+        assert ts.fast_math(False) == 0
+        assert lib.talshQueryFastMath(ts.DEV_NVIDIA_GPU, 0) == ts.NOPE
+    finally:
+        ts.fast_math(False)
+        lib.talshAmdSetFastMathSplit(ts.SPLIT_BF16X6)
