@@ -298,11 +298,17 @@ def bench_transposes(oracle, steps):
     for t in (l, d, s, sm):
         t.destruct()
     lib.talshAmdDeviceSynchronize(0)
+    kernel_gbs = profiled_value("transposing_copy_kernel_gbs_mean")
     return {"copy_gbs_mean": float(np.mean(copy_gbs)), "copy_gbs_min": float(np.min(copy_gbs)),
             "copy_gbs_identity": copy_gbs[0], "copy_gbs_reversal": copy_gbs[1],
             "copy_frac_hbm_mean": float(np.mean(copy_gbs)) / HBM_PEAK_GBS, "copy_bit_exact": bool(exact),
             "add_gbs": add_gbs, "norm2_gbs": norm_gbs, "permutations": len(perms),
-            "slice_gbs": slice_gbs, "insert_accumulate_gbs": insert_gbs}
+            "slice_gbs": slice_gbs, "insert_accumulate_gbs": insert_gbs,
+            # the figures above time each copy with task events, which bracket the launch as well; the
+            # kernels' own time over the same sweep comes from the committed rocprofv3 summary
+            "copy_gbs_mean_kernel_profiled": kernel_gbs,
+            "copy_frac_hbm_mean_kernel_profiled": kernel_gbs / HBM_PEAK_GBS if kernel_gbs else None,
+            "kernel_profile": "profiles/transpose_16p6_r8.md"}
 
 
 def batch_contraction(rng, lo=2, hi=24, max_vol=400000):
@@ -709,6 +715,17 @@ def gemm_io_lower_bound(m, n, k, esize, xcds=8, l2_bytes=4 << 20):
 def check_init(err):
     if err:
         raise RuntimeError("talshInit failed: %d" % err)
+
+
+def profiled_value(key, path=os.path.join(REPO, "profiles", "transpose_16p6_r8.md")):
+    """A `key: value` line of a committed rocprofv3 summary (None when absent)."""
+    try:
+        for line in open(path):
+            if line.startswith(key + ":"):
+                return float(line.split(":", 1)[1])
+    except (OSError, ValueError):
+        pass
+    return None
 
 
 def profiled_traffic(kernel_prefix, path=os.path.join(REPO, "profiles", "contract_r8_96_traffic.md")):
