@@ -689,6 +689,8 @@ int talshTensorContract(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *lt
   std::lock_guard<std::recursive_mutex> lk(rt().mu);
   if (!rt().on) return TALSH_NOT_INITIALIZED;
   if (!cptrn || !dtens || !ltens || !rtens) return TALSH_INVALID_ARGS;
+  // D is written while L and R are read (in place, when the GEMM writes D through offsets): no aliasing
+  if (dtens == ltens || dtens == rtens) return TALSH_INVALID_ARGS;
   if (int e = check_task(talsh_task)) return e;
   if (talshTensorIsEmpty(dtens) == YEP || talshTensorIsEmpty(ltens) == YEP || talshTensorIsEmpty(rtens) == YEP)
     return TALSH_OBJECT_IS_EMPTY;

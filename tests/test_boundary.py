@@ -204,8 +204,11 @@ def test_host_side_errors_without_gpu():
         assert lib.talshInit(ctypes.byref(hb), ctypes.byref(ham), 0, None, 0, None, 0, None) == \
             ts.TALSH_ALREADY_INITIALIZED
         d, l, r = (ts.Tensor(R8, dims, 0) for dims in ([2, 3], [2, 4], [4, 3]))
-        # invalid pattern / shape mismatch are rejected before any device work
+        # invalid pattern / shape mismatch / D aliasing an input are rejected before any device work
         assert ts.contract("D(a,b)+=L(a,i)*R(i", d, l, r) == ts.TALSH_INVALID_ARGS
+        sq = ts.Tensor(R8, [2, 2], 0)
+        assert ts.contract("D(a,b)+=L(a,i)*R(i,b)", sq, sq, sq) == ts.TALSH_INVALID_ARGS
+        sq.destruct()
         assert ts.contract("D(a,b)+=L(a,i)*R(b,i)", d, l, r) == ts.TALSH_INVALID_ARGS
         # host execution is not provided by this build
         assert ts.contract("D(a,b)+=L(a,i)*R(i,b)", d, l, r, dev_kind=ts.DEV_HOST, dev_id=0) == \
