@@ -54,6 +54,13 @@ constexpr int kThreads = 256;
 constexpr int BK = 16;
 constexpr int kXcds = 8;
 
+// Waves per workgroup of gemm_kernel: 8 (4 per SIMD, 64 x 32 per wave) for
+// real FP64, 4 (2 per SIMD, 64 x 64 per wave) otherwise.
+template <typename S, int NC>
+constexpr int gemm_waves() {
+  return (sizeof(S) == 8 && NC == 1) ? 8 : 4;
+}
+
 // W scalars moved as one access (W*sizeof(S) is 16 bytes when vectorised).
 template <typename S, int W>
 struct Vec {
@@ -97,7 +104,8 @@ __device__ inline void tile_of(int64_t gx, int64_t gy, int64_t t0, int64_t count
 // layout codes: 0 = N, 1 = T; CJA/CJB conjugate the complex operand; W is the
 // vector width in scalars along each operand's contiguous axis.
 template <typename S, int NC, int OPA, int OPB, int CJA, int CJB, int BM, int BN, int W>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void gemm_kernel(
+__global__ __launch_bounds__((64 * gemm_waves<S, NC>()))
+__attribute__((amdgpu_waves_per_eu(gemm_waves<S, NC>() / 2))) void gemm_kernel(
                                                         int64_t M, int64_t N, int64_t K, const S *__restrict__ A,
                                                         int64_t lda, const S *__restrict__ B, int64_t ldb,
                                                         S *__restrict__ C, int64_t ldc, S alpha_r, S alpha_i,
@@ -116,16 +124,18 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
   B += (int64_t)blockIdx.y * bsb;
   C += (int64_t)blockIdx.y * bsc;
   using acc_t = typename MF::acc_t;
-  constexpr int SM = BM / 32, SN = BN / 32;  // 16x16 tiles per wave along M, N
+  // threads; columns of C per wave (waves form 2 rows x gemm_waves/2 columns)
+  constexpr int NT = 64 * gemm_waves<S, NC>(), WBN = BN / (gemm_waves<S, NC>() / 2);
+  constexpr int SM = BM / 32, SN = WBN / 16;  // 16x16 tiles per wave along M, N
   constexpr int PAD = W;                     // keeps rows 16-byte aligned
   constexpr int RS = BK + PAD;               // LDS row stride (scalars)
   // A tile: BM x BK elements (BM*BK*NC scalars) moved in vectors of W scalars
   // along the contiguous axis (m for 'N', k for 'T'; re/im interleaved).
   constexpr int ACONT = (OPA == 0 ? BM : BK) * NC, BCONT = (OPB == 0 ? BK : BN) * NC;
-  constexpr int AVEC = BM * BK * NC / W / kThreads, BVEC = BN * BK * NC / W / kThreads;
+  constexpr int AVEC = BM * BK * NC / W / NT, BVEC = BN * BK * NC / W / NT;
   static_assert(AVEC >= 1 && BVEC >= 1, "tile too small");
   constexpr int ATPL = ACONT / W, BTPL = BCONT / W;  // threads per contiguous line
-  static_assert(kThreads % ATPL == 0 && kThreads % BTPL == 0, "line split");
+  static_assert(NT % ATPL == 0 && NT % BTPL == 0, "line split");
   __shared__ S As[2][NC][BM][RS];
   __shared__ S Bs[2][NC][BN][RS];
 
@@ -144,7 +154,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
   // per-thread load coordinates: contiguous-axis scalar offset and line index
   const int a_c = (tid % ATPL) * W, a_l = tid / ATPL;
   const int b_c = (tid % BTPL) * W, b_l = tid / BTPL;
-  constexpr int ALS = kThreads / ATPL, BLS = kThreads / BTPL;  // line step per vector
+  constexpr int ALS = NT / ATPL, BLS = NT / BTPL;  // line step per vector
 
   Vec<S, W> ra[AVEC], rb[BVEC];
 
@@ -237,7 +247,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
 #pragma unroll
       for (int j = 0; j < SN; ++j)
 #pragma unroll
-        for (int c = 0; c < NC; ++c) fb[c][j] = Bs[buf][c][wn * (BN / 2) + j * 16 + (lane & 15)][kk];
+        for (int c = 0; c < NC; ++c) fb[c][j] = Bs[buf][c][wn * WBN + j * 16 + (lane & 15)][kk];
 #pragma unroll
       for (int i = 0; i < SM; ++i)
 #pragma unroll
@@ -266,7 +276,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
       for (int j = 0; j < SN; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int ml = wm * (BM / 2) + i * 16 + MF::row(lane, r), nl = wn * (BN / 2) + j * 16 + (lane & 15);
+          const int ml = wm * (BM / 2) + i * 16 + MF::row(lane, r), nl = wn * WBN + j * 16 + (lane & 15);
 #pragma unroll
           for (int c = 0; c < NC; ++c) pb[(ml + nl * BM) * NC + c] = acc[c][i][j][r];
         }
@@ -279,7 +289,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int64_t m = m0 + wm * (BM / 2) + i * 16 + MF::row(lane, r);
-        const int64_t n = n0 + wn * (BN / 2) + j * 16 + (lane & 15);
+        const int64_t n = n0 + wn * WBN + j * 16 + (lane & 15);
         if (m < M && n < N) {
           S *c = C + (cmap ? cmap[m] + cmap[M + n] : m + n * ldc) * NC;
           if (NC == 1) {
@@ -1010,7 +1020,8 @@ int64_t gemm_slots() {
     int cus = 0, per = 0;
     constexpr int WV = (int)(16 / sizeof(S));
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, gemm_kernel<S, NC, 1, 0, 0, 0, BM, BN, WV>, kThreads, 0) !=
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, gemm_kernel<S, NC, 1, 0, 0, 0, BM, BN, WV>,
+                                                     64 * gemm_waves<S, NC>(), 0) !=
             hipSuccess)
       return 0;
     cache[dev] = (int64_t)cus * std::max(per, 1);
@@ -1069,11 +1080,11 @@ int launch_typed(int64_t M, int64_t N, int64_t K, const void *A, int64_t lda, co
     S *c = (S *)C + b0 * sc * NC;
     if (full > 0) {
       if (vec)
-        gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, WV><<<dim3((unsigned)full, nb), dim3(kThreads), 0, s>>>(
+        gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, WV><<<dim3((unsigned)full, nb), dim3(64 * gemm_waves<S, NC>()), 0, s>>>(
             M, N, K, a, lda, b, ldb, c, ldc, (S)ar, (S)ai, beta_one, gx, gy, sa * NC, sb * NC, sc * NC, 0, full, 1,
             0, nullptr, cmap);
       else
-        gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, 1><<<dim3((unsigned)full, nb), dim3(kThreads), 0, s>>>(
+        gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, 1><<<dim3((unsigned)full, nb), dim3(64 * gemm_waves<S, NC>()), 0, s>>>(
             M, N, K, a, lda, b, ldb, c, ldc, (S)ar, (S)ai, beta_one, gx, gy, sa * NC, sb * NC, sc * NC, 0, full, 1,
             0, nullptr, cmap);
     }
@@ -1081,11 +1092,11 @@ int launch_typed(int64_t M, int64_t N, int64_t K, const void *A, int64_t lda, co
   if (ts.ksplit > 1) {
     const unsigned nw = (unsigned)(ts.rem * ts.ksplit);
     if (vec)
-      gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, WV><<<dim3(nw), dim3(kThreads), 0, s>>>(
+      gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, WV><<<dim3(nw), dim3(64 * gemm_waves<S, NC>()), 0, s>>>(
           M, N, K, (const S *)A, lda, (const S *)B, ldb, (S *)C, ldc, (S)ar, (S)ai, beta_one, gx, gy, 0, 0, 0, full,
           ts.rem, ts.ksplit, kchunk, (S *)ws, cmap);
     else
-      gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, 1><<<dim3(nw), dim3(kThreads), 0, s>>>(
+      gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, 1><<<dim3(nw), dim3(64 * gemm_waves<S, NC>()), 0, s>>>(
           M, N, K, (const S *)A, lda, (const S *)B, ldb, (S *)C, ldc, (S)ar, (S)ai, beta_one, gx, gy, 0, 0, 0, full,
           ts.rem, ts.ksplit, kchunk, (S *)ws, cmap);
     const int64_t nel = ts.rem * BM * BN;
