@@ -54,11 +54,12 @@ constexpr int kThreads = 256;
 constexpr int BK = 16;
 constexpr int kXcds = 8;
 
-// Waves per workgroup of gemm_kernel: 8 (4 per SIMD, 64 x 32 per wave) for
-// real FP64, 4 (2 per SIMD, 64 x 64 per wave) otherwise.
+// Waves per workgroup of gemm_kernel: 8 in 2 rows x 4 columns, 4 per SIMD
+// (at 9216^3, against 4 waves of a 2 x 2 layout: FP64 64.5 -> 65.6, FP32
+// 124.8 -> 127.3, complex FP64 65.5 -> 66.8 TFLOP/s).
 template <typename S, int NC>
 constexpr int gemm_waves() {
-  return (sizeof(S) == 8 && NC == 1) ? 8 : 4;
+  return 8;
 }
 
 // W scalars moved as one access (W*sizeof(S) is 16 bytes when vectorised).
