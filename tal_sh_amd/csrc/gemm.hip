@@ -54,13 +54,12 @@ constexpr int kThreads = 256;
 constexpr int BK = 16;
 constexpr int kXcds = 8;
 
-// Waves per workgroup of gemm_kernel: 8 in 2 rows x 4 columns, 4 per SIMD
-// (at 9216^3, against 4 waves of a 2 x 2 layout: FP64 64.5 -> 65.6, FP32
-// 124.8 -> 127.3, complex FP64 65.5 -> 66.8 TFLOP/s).
-template <typename S, int NC>
-constexpr int gemm_waves() {
-  return 8;
-}
+// Waves per workgroup of gemm_kernel (template parameter NW), in 2 rows x
+// NW/2 columns: 8 (4 per SIMD) for single GEMMs — at 9216^3, against 4 waves
+// of 2 x 2: FP64 64.5 -> 65.6, FP32 124.8 -> 127.3, complex FP64 65.5 -> 66.8
+// TFLOP/s — and 4 (2 per SIMD) for batched ones, whose 16 x 1024^3 case ran
+// 48.1 TFLOP/s with 4 waves and 42.8 with 8.
+constexpr int kWavesSingle = 8, kWavesBatched = 4;
 
 // W scalars moved as one access (W*sizeof(S) is 16 bytes when vectorised).
 template <typename S, int W>
@@ -104,9 +103,8 @@ __device__ inline void tile_of(int64_t gx, int64_t gy, int64_t t0, int64_t count
 
 // layout codes: 0 = N, 1 = T; CJA/CJB conjugate the complex operand; W is the
 // vector width in scalars along each operand's contiguous axis.
-template <typename S, int NC, int OPA, int OPB, int CJA, int CJB, int BM, int BN, int W>
-__global__ __launch_bounds__((64 * gemm_waves<S, NC>()))
-__attribute__((amdgpu_waves_per_eu(gemm_waves<S, NC>() / 2))) void gemm_kernel(
+template <typename S, int NC, int OPA, int OPB, int CJA, int CJB, int BM, int BN, int W, int NW>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2))) void gemm_kernel(
                                                         int64_t M, int64_t N, int64_t K, const S *__restrict__ A,
                                                         int64_t lda, const S *__restrict__ B, int64_t ldb,
                                                         S *__restrict__ C, int64_t ldc, S alpha_r, S alpha_i,
@@ -125,8 +123,8 @@ __attribute__((amdgpu_waves_per_eu(gemm_waves<S, NC>() / 2))) void gemm_kernel(
   B += (int64_t)blockIdx.y * bsb;
   C += (int64_t)blockIdx.y * bsc;
   using acc_t = typename MF::acc_t;
-  // threads; columns of C per wave (waves form 2 rows x gemm_waves/2 columns)
-  constexpr int NT = 64 * gemm_waves<S, NC>(), WBN = BN / (gemm_waves<S, NC>() / 2);
+  // threads; columns of C per wave (waves form 2 rows x NW/2 columns)
+  constexpr int NT = 64 * NW, WBN = BN / (NW / 2);
   constexpr int SM = BM / 32, SN = WBN / 16;  // 16x16 tiles per wave along M, N
   constexpr int PAD = W;                     // keeps rows 16-byte aligned
   constexpr int RS = BK + PAD;               // LDS row stride (scalars)
@@ -1021,8 +1019,8 @@ int64_t gemm_slots() {
     int cus = 0, per = 0;
     constexpr int WV = (int)(16 / sizeof(S));
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, gemm_kernel<S, NC, 1, 0, 0, 0, BM, BN, WV>,
-                                                     64 * gemm_waves<S, NC>(), 0) !=
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, gemm_kernel<S, NC, 1, 0, 0, 0, BM, BN, WV, kWavesSingle>,
+                                                     64 * kWavesSingle, 0) !=
             hipSuccess)
       return 0;
     cache[dev] = (int64_t)cus * std::max(per, 1);
@@ -1080,24 +1078,34 @@ int launch_typed(int64_t M, int64_t N, int64_t K, const void *A, int64_t lda, co
     const S *a = (const S *)A + b0 * sa * NC, *b = (const S *)B + b0 * sb * NC;
     S *c = (S *)C + b0 * sc * NC;
     if (full > 0) {
-      if (vec)
-        gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, WV><<<dim3((unsigned)full, nb), dim3(64 * gemm_waves<S, NC>()), 0, s>>>(
-            M, N, K, a, lda, b, ldb, c, ldc, (S)ar, (S)ai, beta_one, gx, gy, sa * NC, sb * NC, sc * NC, 0, full, 1,
-            0, nullptr, cmap);
+      const dim3 grid((unsigned)full, nb);
+      const int64_t bsa = sa * NC, bsb = sb * NC, bsc = sc * NC;
+      if (batch > 1 && vec)
+        gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, WV, kWavesBatched><<<grid, dim3(64 * kWavesBatched), 0, s>>>(
+            M, N, K, a, lda, b, ldb, c, ldc, (S)ar, (S)ai, beta_one, gx, gy, bsa, bsb, bsc, 0, full, 1, 0, nullptr,
+            cmap);
+      else if (batch > 1)
+        gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, 1, kWavesBatched><<<grid, dim3(64 * kWavesBatched), 0, s>>>(
+            M, N, K, a, lda, b, ldb, c, ldc, (S)ar, (S)ai, beta_one, gx, gy, bsa, bsb, bsc, 0, full, 1, 0, nullptr,
+            cmap);
+      else if (vec)
+        gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, WV, kWavesSingle><<<grid, dim3(64 * kWavesSingle), 0, s>>>(
+            M, N, K, a, lda, b, ldb, c, ldc, (S)ar, (S)ai, beta_one, gx, gy, bsa, bsb, bsc, 0, full, 1, 0, nullptr,
+            cmap);
       else
-        gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, 1><<<dim3((unsigned)full, nb), dim3(64 * gemm_waves<S, NC>()), 0, s>>>(
-            M, N, K, a, lda, b, ldb, c, ldc, (S)ar, (S)ai, beta_one, gx, gy, sa * NC, sb * NC, sc * NC, 0, full, 1,
-            0, nullptr, cmap);
+        gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, 1, kWavesSingle><<<grid, dim3(64 * kWavesSingle), 0, s>>>(
+            M, N, K, a, lda, b, ldb, c, ldc, (S)ar, (S)ai, beta_one, gx, gy, bsa, bsb, bsc, 0, full, 1, 0, nullptr,
+            cmap);
     }
   }
   if (ts.ksplit > 1) {
     const unsigned nw = (unsigned)(ts.rem * ts.ksplit);
     if (vec)
-      gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, WV><<<dim3(nw), dim3(64 * gemm_waves<S, NC>()), 0, s>>>(
+      gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, WV, kWavesSingle><<<dim3(nw), dim3(64 * kWavesSingle), 0, s>>>(
           M, N, K, (const S *)A, lda, (const S *)B, ldb, (S *)C, ldc, (S)ar, (S)ai, beta_one, gx, gy, 0, 0, 0, full,
           ts.rem, ts.ksplit, kchunk, (S *)ws, cmap);
     else
-      gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, 1><<<dim3(nw), dim3(64 * gemm_waves<S, NC>()), 0, s>>>(
+      gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, 1, kWavesSingle><<<dim3(nw), dim3(64 * kWavesSingle), 0, s>>>(
           M, N, K, (const S *)A, lda, (const S *)B, ldb, (S *)C, ldc, (S)ar, (S)ai, beta_one, gx, gy, 0, 0, 0, full,
           ts.rem, ts.ksplit, kchunk, (S *)ws, cmap);
     const int64_t nel = ts.rem * BM * BN;
