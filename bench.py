@@ -238,97 +238,115 @@ def bench_transposes(oracle, steps):
         if p not in perms:
             perms.append(p)
     lib = ts.load()
+
+    def kernel_time(submit, n):
+        """Shortest of n kernel spans (talshTaskTime "comput": dispatch timestamps of the operation's
+        first and last kernel, so the host's launch latency is not counted)."""
+        tms = []
+        for _ in range(n):
+            tk = ts.Task()
+            ts.check(submit(tk), "task")
+            tk.wait()
+            tms.append(tk.times()["comput"])
+            tk.destruct()
+        return float(np.min(tms))
+
     per = []
     exact = True
     for p in perms:
         pat = "D(a,b,c,d,e,f)=L(%s)" % ",".join(p)
         ts.check(ts.copy(pat, d, l), "copy")  # warm-up
-        tms = []
-        for _ in range(steps):
-            tk = ts.Task()
-            ts.check(ts.copy(pat, d, l, task=tk), "copy")
-            tk.wait()
-            tms.append(tk.times()["comput"])
-            tk.destruct()
-        per.append(float(np.median(tms)))
+        per.append(kernel_time(lambda tk: ts.copy(pat, d, l, task=tk), steps))
     # bit-exact check of the last permutation
     o2n = [labels.index(x) + 1 for x in perms[-1]]
     exact = d.host_array().tobytes() == oracle.copy(R8, dims, o2n, L).tobytes()
     d.place(ts.gpu_flat(0))
     copy_gbs = [2 * nbytes / t / 1e9 for t in per]
     # permuted add D += 0.5 * perm(L)
-    tms = []
-    for _ in range(steps + 1):
-        tk = ts.Task()
-        ts.check(ts.add("D(a,b,c,d,e,f)+=L(c,a,f,b,e,d)", d, l, alpha=0.5, task=tk), "add")
-        tk.wait()
-        tms.append(tk.times()["comput"])
-        tk.destruct()
-    add_gbs = 3 * nbytes / float(np.median(tms[1:])) / 1e9
-    # norm2
+    add_s = kernel_time(lambda tk: ts.add("D(a,b,c,d,e,f)+=L(c,a,f,b,e,d)", d, l, alpha=0.5, task=tk), steps + 1)
+    add_gbs = 3 * nbytes / add_s / 1e9
+    # norm2 (both stages: partial sums, then their fixed-order total)
     s = ts.Tensor(R8, [], ts.gpu_flat(0))
-    tms = []
-    for _ in range(steps + 1):
-        tk = ts.Task()
-        ts.check(ts.norm2(s, l, copy_ctrl=ts.coh("M", "T"), task=tk), "norm2")
-        tk.wait()
-        tms.append(tk.times()["comput"])
-        tk.destruct()
-    norm_gbs = nbytes / float(np.median(tms[1:])) / 1e9
+    norm_s = kernel_time(lambda tk: ts.norm2(s, l, copy_ctrl=ts.coh("M", "T"), task=tk), steps + 1)
+    norm_gbs = nbytes / norm_s / 1e9
     # slice / insert of an 8 x 16^4 x 8 box (1/4 of the tensor, offsets 4 and 8 on the outer dims)
     sdims, off = [8, 16, 16, 16, 16, 8], [4, 0, 0, 0, 0, 8]
     sbytes = int(np.prod(sdims)) * 8
     sm = ts.Tensor(R8, sdims, ts.gpu_flat(0))
-    tms = []
-    for _ in range(steps + 1):
-        tk = ts.Task()
-        ts.check(ts.slice(sm, l, off, task=tk), "slice")
-        tk.wait()
-        tms.append(tk.times()["comput"])
-        tk.destruct()
-    slice_gbs = 2 * sbytes / float(np.median(tms[1:])) / 1e9
-    tms = []
-    for _ in range(steps + 1):
-        tk = ts.Task()
-        ts.check(ts.insert(d, sm, off, accumulative=ts.YEP, task=tk), "insert")
-        tk.wait()
-        tms.append(tk.times()["comput"])
-        tk.destruct()
-    insert_gbs = 3 * sbytes / float(np.median(tms[1:])) / 1e9
+    slice_gbs = 2 * sbytes / kernel_time(lambda tk: ts.slice(sm, l, off, task=tk), steps + 1) / 1e9
+    insert_gbs = 3 * sbytes / kernel_time(lambda tk: ts.insert(d, sm, off, accumulative=ts.YEP, task=tk),
+                                          steps + 1) / 1e9
     for t in (l, d, s, sm):
         t.destruct()
     lib.talshAmdDeviceSynchronize(0)
-    kernel_gbs = profiled_value("transposing_copy_kernel_gbs_mean")
-    return {"copy_gbs_mean": float(np.mean(copy_gbs)), "copy_gbs_min": float(np.min(copy_gbs)),
+    worst = int(np.argmin(copy_gbs))
+    return {"timing": "kernel dispatch timestamps (talshTaskTime comput), shortest of %d runs per case" % steps,
+            "copy_gbs_mean": float(np.mean(copy_gbs)), "copy_gbs_min": float(np.min(copy_gbs)),
+            "copy_gbs_max": float(np.max(copy_gbs)), "copy_worst_permutation": ",".join(perms[worst]),
             "copy_gbs_identity": copy_gbs[0], "copy_gbs_reversal": copy_gbs[1],
-            "copy_frac_hbm_mean": float(np.mean(copy_gbs)) / HBM_PEAK_GBS, "copy_bit_exact": bool(exact),
-            "add_gbs": add_gbs, "norm2_gbs": norm_gbs, "permutations": len(perms),
+            "copy_frac_hbm_mean": float(np.mean(copy_gbs)) / HBM_PEAK_GBS,
+            "copy_frac_hbm_min": float(np.min(copy_gbs)) / HBM_PEAK_GBS, "copy_bit_exact": bool(exact),
+            "add_gbs": add_gbs, "add_frac_hbm": add_gbs / HBM_PEAK_GBS, "norm2_gbs": norm_gbs,
+            "norm2_frac_hbm": norm_gbs / HBM_PEAK_GBS, "permutations": len(perms),
             "slice_gbs": slice_gbs, "insert_accumulate_gbs": insert_gbs,
-            # the figures above time each copy with task events, which bracket the launch as well; the
-            # kernels' own time over the same sweep comes from the committed rocprofv3 summary
-            "copy_gbs_mean_kernel_profiled": kernel_gbs,
-            "copy_frac_hbm_mean_kernel_profiled": kernel_gbs / HBM_PEAK_GBS if kernel_gbs else None,
-            "kernel_profile": "profiles/transpose_16p6_r8.md"}
+            "slice_frac_hbm": slice_gbs / HBM_PEAK_GBS, "insert_accumulate_frac_hbm": insert_gbs / HBM_PEAK_GBS,
+            "per_permutation_gbs": [round(x, 1) for x in copy_gbs]}
 
 
-def batch_contraction(rng, lo=2, hi=24, max_vol=400000):
-    """Config 4 task: ranks 2-4 per operand, 1-2 contracted indices, extents uniform in lo..hi."""
-    while True:
-        rl, rr = rng.randint(2, 4), rng.randint(2, 4)
-        nc = rng.randint(1, min(2, rl, rr))
-        nl, nr = rl - nc, rr - nc
-        labels = [chr(ord("a") + i) for i in range(nl + nr + nc)]
-        ext = {x: rng.randint(lo, hi) for x in labels}
-        lu, ru, cc = labels[:nl], labels[nl:nl + nr], labels[nl + nr:]
-        d, l, r = lu + ru, lu + cc, ru + cc
-        rng.shuffle(d)
-        rng.shuffle(l)
-        rng.shuffle(r)
-        vol = lambda s: int(np.prod([ext[x] for x in s])) if s else 1
-        if max(vol(d), vol(l), vol(r)) > max_vol:
-            continue
-        pat = "D(%s)+=L(%s)*R(%s)" % (",".join(d), ",".join(l), ",".join(r))
-        return pat, [ext[x] for x in d], [ext[x] for x in l], [ext[x] for x in r]
+def bench_reference_shapes(steps=3):
+    """The 48 shapes of the reference's benchmark list (tensor_contractions.txt, driven by
+    main.F90:957-1014: D = 0, L = 0.01, R = 0.001, one contraction per shape), R8, operands resident on
+    the GPU, overwriting D.  Shapes come from the committed fixture tests/golden/patterns.json.  Each
+    shape: the shortest of `steps` runs by talshTaskTime (compute: transposes + GEMM; mmul: GEMM), and
+    every element of D checked against the closed form K * 1e-5 through its squared norm."""
+    with open(os.path.join(REPO, "tests", "golden", "patterns.json")) as f:
+        shapes = json.load(f)["tensor_contractions_txt"]
+    gpu = ts.gpu_flat(0)
+    rows, worst_err = [], 0.0
+    tot_flop = tot_comp = tot_mm = 0.0
+    for sh in shapes:
+        dd, ld, rd = sh["ddims"], sh["ldims"], sh["rdims"]
+        vd, vl, vr = (float(np.prod(x)) for x in (dd, ld, rd))
+        k = float(np.sqrt(vl * vr / vd))
+        flop = 2.0 * vd * k
+        d = ts.Tensor(R8, dd, gpu, init_val=0.0)
+        l = ts.Tensor(R8, ld, gpu, init_val=0.01)
+        r = ts.Tensor(R8, rd, gpu, init_val=0.001)
+        ts.check(ts.contract(sh["pattern"], d, l, r, accumulative=ts.NOPE), "contract")  # warm-up
+        comp, mm = [], []
+        for _ in range(steps):
+            tk = ts.Task()
+            ts.check(ts.contract(sh["pattern"], d, l, r, accumulative=ts.NOPE, task=tk), "contract")
+            tk.wait()
+            tm = tk.times()
+            comp.append(tm["comput"])
+            mm.append(tm["mmul"])
+            tk.destruct()
+        s2 = ts.Tensor(R8, [], gpu)
+        ts.check(ts.norm2(s2, d, copy_ctrl=ts.coh("M", "T")), "norm2")
+        want = vd * (k * 1e-5) ** 2
+        err = abs(s2.scalar().real - want) / want
+        worst_err = max(worst_err, err)
+        for t in (d, l, r, s2):
+            t.destruct()
+        c, m = float(np.min(comp)), float(np.min(mm))
+        tot_flop += flop
+        tot_comp += c
+        tot_mm += m
+        rows.append({"pattern": sh["pattern"], "gflop": flop / 1e9, "tflops": flop / c / 1e12,
+                     "gemm_tflops": flop / m / 1e12})
+    tf = [x["tflops"] for x in rows]
+    return {"shapes": len(rows), "tflop_total": tot_flop / 1e12, "tflops_aggregate": tot_flop / tot_comp / 1e12,
+            "gemm_tflops_aggregate": tot_flop / tot_mm / 1e12,
+            "frac_fp64_peak_aggregate": tot_flop / tot_comp / 1e12 / FP64_PEAK_TFLOPS,
+            "tflops_min": float(np.min(tf)), "tflops_median": float(np.median(tf)), "tflops_max": float(np.max(tf)),
+            "max_rel_err_norm2_vs_closed_form": worst_err,
+            "per_shape": [{k2: (round(v, 3) if isinstance(v, float) else v) for k2, v in x.items()} for x in rows]}
+
+
+def batch_contraction(rng):
+    from tal_common import batch_contraction as gen
+    return gen(rng)
 
 
 def bench_batch(oracle, dk, ntask=4096, reps=5, check_sample=128):
@@ -663,35 +681,32 @@ def bench_serve(oracle, ntask=4000, rate=None, budget=200 * 10 ** 9, max_infligh
             "checked_requests": len(checked), "max_rel_err_sampled": errs}
 
 
-def cpu_baseline(budget_s=30.0):
-    """TAL-SH's own CPU path on a bounded sample of the headline contraction.
+def cpu_baseline():
+    """TAL-SH's own CPU path on the headline contraction at its full shape.
 
-    Sample: the same pattern with extents a=b=i=j=96 and c=d=32 (M=N=3072,
-    K=9216 kept whole): 1.74e11 FLOP, 1/9 of one full step."""
+    One call of D(a,b,c,d)+=L(a,i,c,j)*R(j,b,i,d), all extents 96 (M=N=K=9216, 1.5655e12 FLOP; about
+    25 s on 16 cores), after an untimed call on a small shape that starts the OpenMP thread pool."""
     from tal_common import REF_SO, Oracle, Reference
 
     pat = "D(a,b,c,d)+=L(a,i,c,j)*R(j,b,i,d)"
-    dd, ld, rd = [96, 96, 32, 32], [96, 96, 32, 96], [96, 96, 96, 32]
-    L = uniform(11, int(np.prod(ld)))
-    R = uniform(12, int(np.prod(rd)))
-    D = uniform(13, int(np.prod(dd)))
-    flop = 2.0 * 3072 * 3072 * 9216
+    dims = [96] * 4
+    n = 96 ** 4
+    L, R, D = uniform(11, n), uniform(12, n), uniform(13, n)
+    flop = 2.0 * 9216.0 ** 3
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     if os.path.exists(REF_SO):
         ref = Reference()
         kind = "reference"
-        run = lambda: ref.contract(R8, pat, dd, D, ld, L, rd, R)
+        run = lambda dd, a, b, c: ref.contract(R8, pat, dd, c, dd, a, dd, b)
     else:
         orc = Oracle()
         kind = "port"
-        run = lambda: orc.contract(R8, pat, dd, D, ld, L, rd, R)
-    times = []
-    t_start = time.perf_counter()
-    while len(times) < 3 and time.perf_counter() - t_start < budget_s:
-        t0 = time.perf_counter()
-        run()
-        times.append(time.perf_counter() - t0)
-    t = float(np.median(times))
+        run = lambda dd, a, b, c: orc.contract(R8, pat, dd, c, dd, a, dd, b)
+    small = uniform(14, 16 ** 4)
+    run([16] * 4, small, small, small)
+    t0 = time.perf_counter()
+    run(dims, L, R, D)
+    t = time.perf_counter() - t0
     model = ""
     try:
         with open("/proc/cpuinfo") as f:
@@ -699,9 +714,10 @@ def cpu_baseline(budget_s=30.0):
     except OSError:
         pass
     return {"value": flop / t / 1e12, "unit": "TFLOP/s", "cores": cores, "kind": kind,
-            "nproc": os.cpu_count(), "cpu_model": model,
-            "sample": "D(a,b,c,d)+=L(a,i,c,j)*R(j,b,i,d) R8 with c=d=32, others 96 (M=N=3072, K=9216; "
-                      "1.74e11 FLOP), median of %d runs, %.2f s each, OMP threads %d" % (len(times), t, cores)}
+            "nproc": os.cpu_count(), "cpu_model": model, "seconds": t,
+            "sample": "the headline contraction at full size: D(a,b,c,d)+=L(a,i,c,j)*R(j,b,i,d) R8, extents 96 "
+                      "(M=N=K=9216, 1.5655e12 FLOP), one call of %.1f s after a 16^4 warm-up call, "
+                      "OMP threads %d" % (t, cores)}
 
 
 def gemm_io_lower_bound(m, n, k, esize, xcds=8, l2_bytes=4 << 20):
@@ -717,24 +733,13 @@ def check_init(err):
         raise RuntimeError("talshInit failed: %d" % err)
 
 
-def profiled_value(key, path=os.path.join(REPO, "profiles", "transpose_16p6_r8.md")):
-    """A `key: value` line of a committed rocprofv3 summary (None when absent)."""
-    try:
-        for line in open(path):
-            if line.startswith(key + ":"):
-                return float(line.split(":", 1)[1])
-    except (OSError, ValueError):
-        pass
-    return None
-
-
-def profiled_traffic(kernel_prefix, path=os.path.join(REPO, "profiles", "contract_r8_96_traffic.md")):
+def profiled_traffic(path=os.path.join(REPO, "profiles", "contract_r8_96_traffic.md")):
     """HBM-side bytes per GEMM stage (the whole-tile launch, the K-split launch and its summing pass), from
     the committed counter summary (FETCH_SIZE and WRITE_SIZE passes scaled by a plain copy of known size);
     None when the summary has no line for it."""
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_gemm_stage " + kernel_prefix):
+            if line.startswith("traffic_bytes_per_gemm_stage"):
                 return int(line.rsplit(":", 1)[1])
     except OSError:
         pass
@@ -749,8 +754,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--only", default="",
-                    help="comma list of parts to run (head,r4,bf16,fp16x3,c8,config1,transpose,batch,hyper,xl,serve)")
-    ap.add_argument("--serve-requests", type=int, default=4000, help="serving stream length (full: 20000)")
+                    help="comma list of parts to run "
+                         "(head,r4,bf16,fp16x3,c8,config1,transpose,shapes,batch,hyper,xl,serve)")
+    ap.add_argument("--serve-requests", type=int, default=20000, help="serving stream length")
     ap.add_argument("--serve-rate", type=float, default=None, help="arrivals per second (default 0.8 x capacity)")
     ap.add_argument("--serve-streams", type=int, default=16, help="streams per GPU for the serving run")
     ap.add_argument("--serve-inflight", type=int, default=32, help="requests in flight (server slots)")
@@ -788,6 +794,8 @@ def main():
         secondary["r8_config1"] = bench_contraction(oracle, R8, "D(a,b,c,d)+=L(c,i,b,j)*R(d,j,a,i)", 32, 50, 5)
     if want("transpose"):
         secondary["transpose_config2"] = bench_transposes(oracle, 5)
+    if want("shapes"):
+        secondary["reference_shapes_r8"] = bench_reference_shapes()
     if want("batch"):
         secondary["batched_config4_r8"] = bench_batch(oracle, R8)
         secondary["batched_config4_r4"] = bench_batch(oracle, R4)
@@ -829,7 +837,7 @@ def main():
         "max_abs_err_ratio": head.get("max_abs_err_ratio"),
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                     "traffic": profiled_traffic("gemm_kernel<double, 1, 1, 0, 0, 0, 128, 128, 2>"),
+                     "traffic": profiled_traffic(),
                      "traffic_source": "profiles/contract_r8_96_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
                                        "copy-calibrated; bytes per GEMM stage)",
                      # classical-GEMM I/O lower bound with 4 MiB of L2 per XCD (8 XCDs), plus D read and written once

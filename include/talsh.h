@@ -30,12 +30,15 @@
 #define TALSH_DEFAULT(x)
 #endif
 
+/* parameters (talsh.h:17-23) */
 #define TALSH_MAX_DEV_PRESENT 16
+#define TALSH_MEM_ALLOC_POLICY_HOST MEM_ALLOC_TMP_BUF /* default Host allocation policy */
+#define TALSH_MEM_ALLOC_FALLBACK_HOST 1               /* default fallback to regular memory: {0|1} */
+#define TALSH_CPTAL_MIN_BUF_SIZE 1073741824           /* smallest Host buffer CP-TAL uses effectively */
 #define TALSH_NO_HOST_BUFFER 16777216
 #define TALSH_GFLOP_THRESH_GPU 4
 
-/* error codes (talsh.h:25-41) */
-/* Tensor operation kinds and stages (talsh.h:53-82). */
+/* tensor operation kinds and stages (talsh.h:53-82) */
 #define TALSH_TENSOR_NOOP -1
 #define TALSH_TENSOR_INIT 68
 #define TALSH_TENSOR_NORM1 69
@@ -65,6 +68,7 @@
 #define TALSH_OP_STORED 7
 #define TALSH_OP_RETIRED 8
 
+/* error codes (talsh.h:25-41) */
 #define TALSH_SUCCESS 0
 #define TALSH_FAILURE -666
 #define TALSH_NOT_AVAILABLE -888
@@ -90,17 +94,6 @@
 #define TALSH_TASK_INPUT_READY 2000003
 #define TALSH_TASK_OUTPUT_READY 2000004
 #define TALSH_TASK_COMPLETED 2000005
-
-/* operation kinds used by this library (talsh.h:53-70) */
-#define TALSH_TENSOR_NOOP -1
-#define TALSH_TENSOR_INIT 68
-#define TALSH_TENSOR_NORM1 69
-#define TALSH_TENSOR_NORM2 70
-#define TALSH_TENSOR_COPY 77
-#define TALSH_TENSOR_PERMUTE 78
-#define TALSH_TENSOR_SCALE 79
-#define TALSH_TENSOR_ADD 80
-#define TALSH_TENSOR_CONTRACT 82
 
 /* Dense tensor with up to TALSH_MAX_DEV_PRESENT body images (talsh.h:87-94, 40 B). */
 typedef struct {

@@ -5,7 +5,7 @@
  * against TAL-SH pass these structs and codes across the C ABI:
  *   data kinds            tensor_algebra.h:64-73
  *   device kinds / ids    tensor_algebra.h:75-85
- *   aliases, TRY_LATER..  tensor_algebra.h:92-103
+ *   aliases, TRY_LATER..  tensor_algebra.h:62, 92-117
  *   coherence codes       tensor_algebra.h:119-203 (2 bits per operand,
  *                         D most significant: D=0 discard, M=1 move,
  *                         T=2 temporary, K=3 keep)
@@ -21,6 +21,7 @@
 #define MAX_TENSOR_RANK 56
 #define MAX_TENSOR_OPERANDS 4
 #define MAX_CONTRACTION_PATTERN_LEN 1024
+#define MAX_MLNDS_PER_TENS 4 /* multi-indices per tensor block: dims, divs, grps, prmn */
 
 /* data kinds */
 #define NO_TYPE 0
@@ -48,6 +49,11 @@
 
 #define NOPE 0
 #define YEP 1
+#define BLAS_ON 0
+#define BLAS_OFF 1
+#define EFF_TRN_OFF 0
+#define EFF_TRN_ON 1
+#define EFF_TRN_ON_CUTT 2
 #define DEVICE_UNABLE -546372819
 #define TRY_LATER -918273645
 #define NOT_CLEAN -192837465
@@ -61,6 +67,11 @@
 #define DEV_OFF 0
 #define DEV_ON 1
 #define DEV_ON_BLAS 2
+#define GPU_OFF 0
+#define GPU_MINE 1
+#define GPU_MINE_CUBLAS 2
+#define NO_COPY_BACK 0
+#define COPY_BACK 1
 
 /* coherence control: one letter per operand, 2 bits each */
 #define COPY_D 0

@@ -55,56 +55,67 @@ __global__ void batch_reset_kernel(unsigned long long *counter) { *counter = 0ul
 // tid % tm, k = tid / tm + i * 256 / tm) when an uncontracted index is the
 // contiguous one of L, along k (AM = 1: k = tid % 16, rows tid / 16 + 16 i)
 // when a contracted index is, so consecutive lanes read consecutive
-// addresses either way; likewise for B with n (BM).  Row / column offsets
-// come from the tables once per item; only the contracted-index table is read
-// per element.  The next K step is gathered into registers while the current
-// one is multiplied from LDS (double-buffered, one barrier per step).
+// addresses either way; likewise for B with n (BM).  At the start of a tile
+// the workgroup copies the tile's row and column offsets and, when K <= kKC
+// (KL = 1), the task's contracted-index offsets from the task tables into LDS
+// with one load per thread and entry, all in flight at once; the K loop and
+// the epilogue then read offsets from LDS, so a data gather waits for one
+// global load, not two dependent ones.  The next K step is gathered into registers while the
+// current one is multiplied from LDS (double-buffered, one barrier per step).
+constexpr int kKC = 512;  // tasks with K <= kKC keep their contracted-index offsets in LDS
 template <typename S>
 struct TileSmem {
   S A[2][kMaxTile][kBK + 1];
   S B[2][kMaxTile][kBK + 1];
+  int lm[kMaxTile], rn[kMaxTile], dm[kMaxTile], dn[kMaxTile];
+  int lk[kKC], rk[kKC];
 };
 
-template <typename S, int AM, int BM>
+template <typename S, int AM, int BM, int KL>
 __device__ __forceinline__ void tile_body(const BatchTask &T, long long local, const int *__restrict__ tabs,
                                           TileSmem<S> &sm, int tid) {
   using MF = BMfma<S>;
   using acc_t = typename MF::acc_t;
   constexpr int NR = kMaxTile * kBK / kThreads;  // elements per thread and operand (tile 64 x 16)
   const int lane = tid & 63, wave = tid >> 6;
-  const int tm = T.tm, tn = T.tn, K = T.K;
+  const int tm = T.tm, tn = T.tn, K = T.K, M = T.M, N = T.N;
   const int m0 = (int)(local % T.tiles_m) * tm, n0 = (int)(local / T.tiles_m) * tn;
-  const int *lM = tabs + T.tab, *lK = lM + T.M, *rK = lK + T.K, *rN = rK + T.K, *dM = rN + T.N, *dN = dM + T.M;
+  const int *lM = tabs + T.tab, *lK = lM + M, *rK = lK + K, *rN = rK + K, *dM = rN + N, *dN = dM + M;
   const S *L = (const S *)T.L, *R = (const S *)T.R;
   const int na = tm * kBK / kThreads, nb = tn * kBK / kThreads;
   const int tml = __builtin_ctz(tm), tnl = __builtin_ctz(tn);  // tm, tn are powers of two
+  // row / column offsets of the tile (-1 past the edge); the previous tile's
+  // epilogue read these arrays before the barrier that ended its item
+  if (tid < kMaxTile) {
+    const int m = m0 + tid, n = n0 + tid;
+    sm.lm[tid] = (tid < tm && m < M) ? lM[m] : -1;
+    sm.dm[tid] = (tid < tm && m < M) ? dM[m] : 0;
+    sm.rn[tid] = (tid < tn && n < N) ? rN[n] : -1;
+    sm.dn[tid] = (tid < tn && n < N) ? dN[n] : 0;
+  }
   auto am = [&](int i) { return AM ? (tid >> 4) + kBK * i : tid & (tm - 1); };
   auto ak = [&](int i) { return AM ? tid & (kBK - 1) : (tid >> tml) + i * (kThreads >> tml); };
   auto bn = [&](int i) { return BM ? (tid >> 4) + kBK * i : tid & (tn - 1); };
   auto bk = [&](int i) { return BM ? tid & (kBK - 1) : (tid >> tnl) + i * (kThreads >> tnl); };
-  int aoff[NR], boff[NR];
-#pragma unroll
-  for (int i = 0; i < NR; ++i) {
-    aoff[i] = (i < na && m0 + am(i) < T.M) ? lM[m0 + am(i)] : -1;
-    boff[i] = (i < nb && n0 + bn(i) < T.N) ? rN[n0 + bn(i)] : -1;
-  }
-  // two register sets: step s is gathered into set s & 1 two steps ahead
-  S ra0[NR], rb0[NR], ra1[NR], rb1[NR];
-#define TAL_GATHER(RA, RB, k0)                                           \
-  _Pragma("unroll") for (int i = 0; i < NR; ++i) {                       \
-    const int ka = (k0) + ak(i), kb = (k0) + bk(i);                      \
-    RA[i] = (aoff[i] >= 0 && ka < K) ? L[aoff[i] + lK[ka]] : (S)0;       \
-    RB[i] = (boff[i] >= 0 && kb < K) ? R[rK[kb] + boff[i]] : (S)0;       \
-  }
-#define TAL_STAGE(RA, RB, buf)                                           \
-  _Pragma("unroll") for (int i = 0; i < NR; ++i) {                       \
-    if (i < na) sm.A[buf][am(i)][ak(i)] = RA[i];                         \
-    if (i < nb) sm.B[buf][bn(i)][bk(i)] = RB[i];                         \
-  }
   const int subm = tm >> 4, nsub = subm * (tn >> 4);
   acc_t acc[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) acc[q] = acc_t{0, 0, 0, 0};
+  // two register sets: step s is gathered into set s & 1 two steps ahead
+  S ra0[NR], rb0[NR], ra1[NR], rb1[NR];
+  int aoff[NR], boff[NR];
+  const int *lk = KL ? sm.lk : lK, *rk = KL ? sm.rk : rK;
+#define TAL_GATHER(RA, RB, k0)                                             \
+  _Pragma("unroll") for (int i = 0; i < NR; ++i) {                         \
+    const int ka = (k0) + ak(i), kb = (k0) + bk(i);                        \
+    RA[i] = (aoff[i] >= 0 && ka < K) ? L[aoff[i] + lk[ka]] : (S)0;         \
+    RB[i] = (boff[i] >= 0 && kb < K) ? R[rk[kb] + boff[i]] : (S)0;         \
+  }
+#define TAL_STAGE(RA, RB, buf)                                             \
+  _Pragma("unroll") for (int i = 0; i < NR; ++i) {                         \
+    if (i < na) sm.A[buf][am(i)][ak(i)] = RA[i];                           \
+    if (i < nb) sm.B[buf][bn(i)][bk(i)] = RB[i];                           \
+  }
   constexpr int UK = sizeof(S) == 8 ? 1 : 2;  // fewer live fragments for FP64 (register budget)
 #define TAL_COMPUTE(buf)                                                                                   \
   _Pragma("unroll UK") for (int ks = 0; ks < kBK / 4; ++ks) {                                              \
@@ -116,6 +127,17 @@ __device__ __forceinline__ void tile_body(const BatchTask &T, long long local, c
         acc[q] = MF::op(sm.A[buf][si * 16 + (lane & 15)][kk], sm.B[buf][sj * 16 + (lane & 15)][kk], acc[q]); \
       }                                                                                                    \
     }                                                                                                      \
+  }
+  if (KL)  // K <= kKC: the contracted-index offsets of the whole task in LDS
+    for (int k = tid; k < K; k += kThreads) {
+      sm.lk[k] = lK[k];
+      sm.rk[k] = rK[k];
+    }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NR; ++i) {
+    aoff[i] = i < na ? sm.lm[am(i)] : -1;
+    boff[i] = i < nb ? sm.rn[bn(i)] : -1;
   }
   const int nk = (K + kBK - 1) / kBK;
   TAL_GATHER(ra0, rb0, 0)
@@ -160,9 +182,9 @@ __device__ __forceinline__ void tile_body(const BatchTask &T, long long local, c
     const int si = sidx % subm, sj = sidx / subm;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int m = m0 + si * 16 + MF::row(lane, r), n = n0 + sj * 16 + (lane & 15);
-      if (m < T.M && n < T.N) {
-        S *d = D + dM[m] + dN[n];
+      const int mi = si * 16 + MF::row(lane, r), ni = sj * 16 + (lane & 15);
+      if (m0 + mi < M && n0 + ni < N) {
+        S *d = D + sm.dm[mi] + sm.dn[ni];
         const S v = alpha * acc[q][r];
         *d = T.beta_one ? *d + v : v;
       }
@@ -191,11 +213,15 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     const int t = item_task[item];
     const BatchTask &T = tasks[t];
     const long long local = item - prefix[t];
-    switch (T.amode * 2 + T.bmode) {  // uniform over the block
-      case 0: tile_body<S, 0, 0>(T, local, tabs, sm, tid); break;
-      case 1: tile_body<S, 0, 1>(T, local, tabs, sm, tid); break;
-      case 2: tile_body<S, 1, 0>(T, local, tabs, sm, tid); break;
-      default: tile_body<S, 1, 1>(T, local, tabs, sm, tid); break;
+    switch (T.amode * 2 + T.bmode + (T.K <= kKC ? 4 : 0)) {  // uniform over the block
+      case 0: tile_body<S, 0, 0, 0>(T, local, tabs, sm, tid); break;
+      case 1: tile_body<S, 0, 1, 0>(T, local, tabs, sm, tid); break;
+      case 2: tile_body<S, 1, 0, 0>(T, local, tabs, sm, tid); break;
+      case 3: tile_body<S, 1, 1, 0>(T, local, tabs, sm, tid); break;
+      case 4: tile_body<S, 0, 0, 1>(T, local, tabs, sm, tid); break;
+      case 5: tile_body<S, 0, 1, 1>(T, local, tabs, sm, tid); break;
+      case 6: tile_body<S, 1, 0, 1>(T, local, tabs, sm, tid); break;
+      default: tile_body<S, 1, 1, 1>(T, local, tabs, sm, tid); break;
     }
     // slot it & 1 was last read before the barriers of the previous tile
     if (tid == 0) s_item[it & 1] = (long long)next;

@@ -35,6 +35,7 @@ struct talshAmdBatch {
   long long total = 0;
   double flops = 0.0;
   std::vector<const void *> sig;
+  std::vector<size_t> sig_bytes;  // body bytes of each sig entry
   void *dev = nullptr;
   int dev_handle = -1;
   tal::BatchTask *d_tasks = nullptr;
@@ -434,17 +435,23 @@ int build(talshAmdBatch *b, int g) {
   const int dk = hts.empty() ? R8 : (hts[0].t.D ? hts[0].cls / 10000 : R8);
   for (auto &h : hts)
     if (h.cls / 10000 != dk) return TALSH_NOT_IMPLEMENTED;  // one data kind per batch
-  // outputs must be distinct and must not be read by another task of the batch
-  std::vector<const void *> outs, ins;
-  for (auto &h : hts) {
-    outs.push_back(h.t.D);
-    ins.push_back(h.t.L);
-    ins.push_back(h.t.R);
+  // outputs must not overlap one another nor any input of the batch (byte ranges)
+  const size_t esz = (size_t)dk_size(dk);
+  std::vector<std::pair<uintptr_t, uintptr_t>> outs, ins;
+  for (size_t i = 0; i < hts.size(); ++i) {
+    const auto &e = b->entries[i];
+    const uintptr_t d = (uintptr_t)hts[i].t.D, l = (uintptr_t)hts[i].t.L, r = (uintptr_t)hts[i].t.R;
+    outs.emplace_back(d, d + shape_volume(e.d->shape_p) * esz);
+    ins.emplace_back(l, l + shape_volume(e.l->shape_p) * esz);
+    ins.emplace_back(r, r + shape_volume(e.r->shape_p) * esz);
   }
   std::sort(outs.begin(), outs.end());
-  if (std::adjacent_find(outs.begin(), outs.end()) != outs.end()) return TALSH_INVALID_ARGS;
-  for (const void *p : ins)
-    if (std::binary_search(outs.begin(), outs.end(), p)) return TALSH_INVALID_ARGS;
+  for (size_t i = 1; i < outs.size(); ++i)
+    if (outs[i].first < outs[i - 1].second) return TALSH_INVALID_ARGS;
+  for (const auto &in : ins) {  // the output starting last before in's end is the only candidate
+    auto it = std::lower_bound(outs.begin(), outs.end(), std::make_pair(in.second, (uintptr_t)0));
+    if (it != outs.begin() && std::prev(it)->second > in.first) return TALSH_INVALID_ARGS;
+  }
   Device &dev = rt().gpus[g];
   hipStream_t s = dev.streams[0];
   if (int e = autotune(g, dk, hts, s, &b->tuned_classes, &b->tune_ms)) return e;
@@ -485,6 +492,9 @@ int build(talshAmdBatch *b, int g) {
     return err;
   }
   b->sig = signature(b, g);
+  b->sig_bytes.clear();
+  for (auto &e : b->entries)
+    for (const talsh_tens_t *t : {e.d, e.l, e.r}) b->sig_bytes.push_back(shape_volume(t->shape_p) * esz);
   b->built = true;
   return TALSH_SUCCESS;
 }
@@ -542,9 +552,9 @@ int talshAmdBatchExecute(talshAmdBatch *b, int dev_id, int dev_kind, talsh_task_
     (void)hipStreamWaitEvent(s, b->hz->ev, 0);
   } else {
     for (size_t i = 0; i < b->entries.size(); ++i) {
-      hazard_wait_write(b->sig[3 * i], s);
-      hazard_wait_read(b->sig[3 * i + 1], s);
-      hazard_wait_read(b->sig[3 * i + 2], s);
+      hazard_wait_write(b->sig[3 * i], b->sig_bytes[3 * i], s);
+      hazard_wait_read(b->sig[3 * i + 1], b->sig_bytes[3 * i + 1], s);
+      hazard_wait_read(b->sig[3 * i + 2], b->sig_bytes[3 * i + 2], s);
     }
   }
   (void)hipEventRecord(task->ev_in->ev, s);
@@ -561,9 +571,9 @@ int talshAmdBatchExecute(talshAmdBatch *b, int dev_id, int dev_kind, talsh_task_
   if (!fast) {
     if (!b->hz) b->hz = new_event(g);
     for (size_t i = 0; i < b->entries.size(); ++i) {
-      hazard_set_writer(b->sig[3 * i], b->hz);
-      hazard_add_reader(b->sig[3 * i + 1], b->hz);
-      hazard_add_reader(b->sig[3 * i + 2], b->hz);
+      hazard_set_writer(b->sig[3 * i], b->sig_bytes[3 * i], b->hz);
+      hazard_add_reader(b->sig[3 * i + 1], b->sig_bytes[3 * i + 1], b->hz);
+      hazard_add_reader(b->sig[3 * i + 2], b->sig_bytes[3 * i + 2], b->hz);
       talsh_tens_t *d = b->entries[i].d;  // other images of each output become stale
       const int gi = image_find(d, flat_gpu(g), NO_TYPE, true);
       if (gi >= 0) image_discard_other(d, gi);
@@ -607,7 +617,7 @@ int talshAmdBatchDestroy(talshAmdBatch *b) {
   {
     std::lock_guard<std::recursive_mutex> lk(rt().mu);
     if (b->built && rt().on && b->g >= 0) {
-      for (const void *p : b->sig) hazard_drain(p);
+      for (size_t i = 0; i < b->sig.size(); ++i) hazard_drain(b->sig[i], b->sig_bytes[i]);
       release_build(b);
     }
   }

@@ -15,12 +15,16 @@ namespace tal {
 // Bytes of the temporaries a plan needs (0 = not needed): transposed A, B,
 // D', the split-BF16 planes of both GEMM operands, and the partial sums of a
 // GEMM whose last round of tiles is split along K.
+// `split` is the fast-math split method of an R4 GEMM (-1: plain GEMM), read
+// once per operation so the workspace and the kernel always agree.
 struct TtgtTemps {
   size_t a = 0, b = 0, d = 0, w = 0, s = 0;
+  int split = -1;
 };
 struct TtgtBuffers {
   void *a = nullptr, *b = nullptr, *d = nullptr, *w = nullptr, *s = nullptr;
   size_t s_bytes = 0;
+  int split = -1;
 };
 TtgtTemps ttgt_temp_bytes(const ContractPlan &plan, int dk, size_t lbytes, size_t rbytes, size_t dbytes);
 

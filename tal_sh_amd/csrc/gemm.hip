@@ -8,8 +8,10 @@
 // so that the contraction planner can feed tensor operands in whichever
 // matrix form their storage already has.
 //
-// Kernel shape: 256 threads = 4 wave64s in a 2x2 grid; each wave owns a
-// (BM/2)x(BN/2) slab of C built from 16x16 MFMA tiles:
+// Kernel shape: NW wave64s per workgroup (template parameter): single GEMMs
+// run 8 waves in a 2x4 grid (512 threads), batched GEMMs 4 waves in a 2x2
+// grid; each wave owns a (BM/2)x(BN/(NW/2)) slab of C built from 16x16
+// MFMA tiles:
 //   R8/C8: v_mfma_f64_16x16x4_f64   R4/C4: v_mfma_f32_16x16x4_f32
 // Operand tiles (BK=16) are double-buffered in LDS as K-contiguous rows padded
 // by one 16-byte vector (conflict-free fragment reads, aligned rows); the next
@@ -1010,7 +1012,7 @@ int launch_bf16x6(int64_t M, int64_t N, int64_t K, const void *A, int64_t lda, c
 
 // Workgroups of gemm_kernel<S, NC, ..., BM, BN> resident at once on the
 // current device (CUs x blocks per CU), cached per device.
-template <typename S, int NC, int BM, int BN>
+template <typename S, int NC, int BM, int BN, int NWS>
 int64_t gemm_slots() {
   static int64_t cache[64] = {0};
   int dev = 0;
@@ -1019,8 +1021,8 @@ int64_t gemm_slots() {
     int cus = 0, per = 0;
     constexpr int WV = (int)(16 / sizeof(S));
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, gemm_kernel<S, NC, 1, 0, 0, 0, BM, BN, WV, kWavesSingle>,
-                                                     64 * kWavesSingle, 0) !=
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, gemm_kernel<S, NC, 1, 0, 0, 0, BM, BN, WV, NWS>,
+                                                     64 * NWS, 0) !=
             hipSuccess)
       return 0;
     cache[dev] = (int64_t)cus * std::max(per, 1);
@@ -1036,11 +1038,11 @@ struct TailSplit {
   int64_t rem = 0;
   int ksplit = 1;
 };
-template <typename S, int NC, int BM, int BN>
+template <typename S, int NC, int BM, int BN, int NWS>
 TailSplit tail_split(int64_t M, int64_t N, int64_t K, int64_t batch) {
   TailSplit t;
   if (batch != 1 || M <= 0 || N <= 0) return t;
-  const int64_t T = ((M + BM - 1) / BM) * ((N + BN - 1) / BN), slots = gemm_slots<S, NC, BM, BN>();
+  const int64_t T = ((M + BM - 1) / BM) * ((N + BN - 1) / BN), slots = gemm_slots<S, NC, BM, BN, NWS>();
   if (slots <= 0) return t;
   const int64_t rem = T % slots;
   if (rem == 0 || rem * 4 > slots) return t;
@@ -1055,7 +1057,7 @@ size_t tail_bytes(const TailSplit &t) {
   return t.ksplit > 1 ? (size_t)t.rem * t.ksplit * BM * BN * NC * sizeof(S) : 0;
 }
 
-template <typename S, int NC, int OPA, int OPB, int CJA, int CJB, int BM, int BN>
+template <typename S, int NC, int OPA, int OPB, int CJA, int CJB, int BM, int BN, int NWS>
 int launch_typed(int64_t M, int64_t N, int64_t K, const void *A, int64_t lda, const void *B, int64_t ldb,
                  void *C, int64_t ldc, double ar, double ai, int beta_one, int64_t batch, int64_t sa, int64_t sb,
                  int64_t sc, void *ws, size_t ws_bytes, const int64_t *cmap, hipStream_t s) {
@@ -1068,7 +1070,7 @@ int launch_typed(int64_t M, int64_t N, int64_t K, const void *A, int64_t lda, co
   const bool vec = ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0) && acont % WV == 0 &&
                    bcont % WV == 0 && (lda * NC) % WV == 0 && (ldb * NC) % WV == 0 &&
                    (batch == 1 || ((sa * NC) % WV == 0 && (sb * NC) % WV == 0));
-  TailSplit ts = tail_split<S, NC, BM, BN>(M, N, K, batch);
+  TailSplit ts = tail_split<S, NC, BM, BN, NWS>(M, N, K, batch);
   if (!ws || tail_bytes<S, NC, BM, BN>(ts) > ws_bytes) ts = TailSplit{};
   const int64_t full = nblk - ts.rem;  // tiles computed whole, in the first launch
   const int64_t kchunk = ts.ksplit > 1 ? (((K + BK - 1) / BK + ts.ksplit - 1) / ts.ksplit) * BK : 0;
@@ -1089,11 +1091,11 @@ int launch_typed(int64_t M, int64_t N, int64_t K, const void *A, int64_t lda, co
             M, N, K, a, lda, b, ldb, c, ldc, (S)ar, (S)ai, beta_one, gx, gy, bsa, bsb, bsc, 0, full, 1, 0, nullptr,
             cmap);
       else if (vec)
-        gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, WV, kWavesSingle><<<grid, dim3(64 * kWavesSingle), 0, s>>>(
+        gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, WV, NWS><<<grid, dim3(64 * NWS), 0, s>>>(
             M, N, K, a, lda, b, ldb, c, ldc, (S)ar, (S)ai, beta_one, gx, gy, bsa, bsb, bsc, 0, full, 1, 0, nullptr,
             cmap);
       else
-        gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, 1, kWavesSingle><<<grid, dim3(64 * kWavesSingle), 0, s>>>(
+        gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, 1, NWS><<<grid, dim3(64 * NWS), 0, s>>>(
             M, N, K, a, lda, b, ldb, c, ldc, (S)ar, (S)ai, beta_one, gx, gy, bsa, bsb, bsc, 0, full, 1, 0, nullptr,
             cmap);
     }
@@ -1101,11 +1103,11 @@ int launch_typed(int64_t M, int64_t N, int64_t K, const void *A, int64_t lda, co
   if (ts.ksplit > 1) {
     const unsigned nw = (unsigned)(ts.rem * ts.ksplit);
     if (vec)
-      gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, WV, kWavesSingle><<<dim3(nw), dim3(64 * kWavesSingle), 0, s>>>(
+      gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, WV, NWS><<<dim3(nw), dim3(64 * NWS), 0, s>>>(
           M, N, K, (const S *)A, lda, (const S *)B, ldb, (S *)C, ldc, (S)ar, (S)ai, beta_one, gx, gy, 0, 0, 0, full,
           ts.rem, ts.ksplit, kchunk, (S *)ws, cmap);
     else
-      gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, 1, kWavesSingle><<<dim3(nw), dim3(64 * kWavesSingle), 0, s>>>(
+      gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, 1, NWS><<<dim3(nw), dim3(64 * NWS), 0, s>>>(
           M, N, K, (const S *)A, lda, (const S *)B, ldb, (S *)C, ldc, (S)ar, (S)ai, beta_one, gx, gy, 0, 0, 0, full,
           ts.rem, ts.ksplit, kchunk, (S *)ws, cmap);
     const int64_t nel = ts.rem * BM * BN;
@@ -1115,14 +1117,14 @@ int launch_typed(int64_t M, int64_t N, int64_t K, const void *A, int64_t lda, co
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-template <typename S, int NC, int BM, int BN>
+template <typename S, int NC, int BM, int BN, int NWS>
 int dispatch_ops(int opa, int opb, int cja, int cjb, int64_t M, int64_t N, int64_t K, const void *A, int64_t lda,
                  const void *B, int64_t ldb, void *C, int64_t ldc, double ar, double ai, int beta_one,
                  int64_t batch, int64_t sa, int64_t sb, int64_t sc, void *ws, size_t ws_bytes,
                  const int64_t *cmap, hipStream_t s) {
 #define TAL_GEMM_CASE(a, b, ca, cb)                                                                      \
   if (opa == a && opb == b && cja == ca && cjb == cb)                                                    \
-    return launch_typed<S, NC, a, b, ca, cb, BM, BN>(M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, \
+    return launch_typed<S, NC, a, b, ca, cb, BM, BN, NWS>(M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, \
                                                      sa, sb, sc, ws, ws_bytes, cmap, s);
 #define TAL_GEMM_LAYOUTS(ca, cb) \
   TAL_GEMM_CASE(0, 0, ca, cb)    \
@@ -1282,12 +1284,28 @@ int launch_gemm_split(int method, char ta, char tb, int64_t M, int64_t N, int64_
              : launch_presplit<3>(opa, opb, M, N, K, ar, A, lda, B, ldb, beta_one, C, ldc, ws, cmap, s);
 }
 
+// Real single GEMMs whose 128 x 128 tiles would not fill one round of the
+// resident workgroups run 64 x 64 tiles with 4 waves (32 x 32 of C each):
+// four times the tiles, so a 1024^3 product spreads over the whole GPU.
+template <typename S>
+bool small_real(int64_t M, int64_t N, int64_t batch) {
+  if (batch != 1) return false;
+  const int64_t t128 = ((M + 127) / 128) * ((N + 127) / 128);
+  return t128 < gemm_slots<S, 1, 128, 128, kWavesSingle>();
+}
+
 size_t gemm_tail_workspace(int dk, int64_t M, int64_t N, int64_t K, int64_t batch) {
   switch (dk) {
-    case 8: return tail_bytes<double, 1, 128, 128>(tail_split<double, 1, 128, 128>(M, N, K, batch));
-    case 4: return tail_bytes<float, 1, 128, 128>(tail_split<float, 1, 128, 128>(M, N, K, batch));
-    case 18: return tail_bytes<double, 2, 64, 64>(tail_split<double, 2, 64, 64>(M, N, K, batch));
-    case 14: return tail_bytes<float, 2, 64, 64>(tail_split<float, 2, 64, 64>(M, N, K, batch));
+    case 8:
+      return small_real<double>(M, N, batch)
+                 ? tail_bytes<double, 1, 64, 64>(tail_split<double, 1, 64, 64, 4>(M, N, K, batch))
+                 : tail_bytes<double, 1, 128, 128>(tail_split<double, 1, 128, 128, kWavesSingle>(M, N, K, batch));
+    case 4:
+      return small_real<float>(M, N, batch)
+                 ? tail_bytes<float, 1, 64, 64>(tail_split<float, 1, 64, 64, 4>(M, N, K, batch))
+                 : tail_bytes<float, 1, 128, 128>(tail_split<float, 1, 128, 128, kWavesSingle>(M, N, K, batch));
+    case 18: return tail_bytes<double, 2, 64, 64>(tail_split<double, 2, 64, 64, kWavesSingle>(M, N, K, batch));
+    case 14: return tail_bytes<float, 2, 64, 64>(tail_split<float, 2, 64, 64, kWavesSingle>(M, N, K, batch));
     default: return 0;
   }
 }
@@ -1319,14 +1337,18 @@ int launch_gemm(int dk, int r4mode, char ta, char tb, int conja, int conjb, int6
     }
     return 0;
   }
-#define TAL_GEMM_KIND(S, NC, BM, BN)                                                                            \
-  dispatch_ops<S, NC, BM, BN>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, sa, sb, \
-                              sc, ws, ws_bytes, cmap, s)
+#define TAL_GEMM_KIND(S, NC, BM, BN, NWS)                                                                        \
+  dispatch_ops<S, NC, BM, BN, NWS>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, sa, \
+                                   sb, sc, ws, ws_bytes, cmap, s)
   switch (dk) {
-    case 8: return TAL_GEMM_KIND(double, 1, 128, 128);
-    case 4: return TAL_GEMM_KIND(float, 1, 128, 128);
-    case 18: return TAL_GEMM_KIND(double, 2, 64, 64);
-    case 14: return TAL_GEMM_KIND(float, 2, 64, 64);
+    case 8:
+      return small_real<double>(M, N, batch) ? TAL_GEMM_KIND(double, 1, 64, 64, 4)
+                                             : TAL_GEMM_KIND(double, 1, 128, 128, kWavesSingle);
+    case 4:
+      return small_real<float>(M, N, batch) ? TAL_GEMM_KIND(float, 1, 64, 64, 4)
+                                            : TAL_GEMM_KIND(float, 1, 128, 128, kWavesSingle);
+    case 18: return TAL_GEMM_KIND(double, 2, 64, 64, kWavesSingle);
+    case 14: return TAL_GEMM_KIND(float, 2, 64, 64, kWavesSingle);
     default: return -1;
   }
 #undef TAL_GEMM_KIND

@@ -3,12 +3,35 @@
 // before launching; a nonzero return means nothing was launched.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
 #include <cstdint>
 
 namespace tal {
+
+// Optional dispatch timestamps around the kernels of one operation (per host
+// thread).  While `on`, launch_clocked gives the first kernel `t0` as its
+// start event and every kernel `t1` as its stop event, so the span (t0, t1)
+// covers the kernels and what runs between them, but not the host's launch
+// latency before the first one.
+struct KernelClock {
+  bool on = false, started = false;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+};
+KernelClock &kernel_clock();
+
+template <typename F, typename... A>
+inline void launch_clocked(F kernel, dim3 grid, dim3 block, hipStream_t s, A... args) {
+  KernelClock &c = kernel_clock();
+  if (c.on) {
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, s, c.started ? nullptr : c.t0, c.t1, 0, args...);
+    c.started = true;
+  } else {
+    hipLaunchKernelGGL(kernel, grid, block, 0, s, args...);
+  }
+}
 
 enum PermOp { PERM_COPY = 0, PERM_ADD = 1 };
 

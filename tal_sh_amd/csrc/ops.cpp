@@ -46,6 +46,7 @@ class Op {
  public:
   Op(int g, talsh_task_t *user_task, const char *name) : g_(g), user_(user_task), name_(name) {}
   ~Op() {
+    kernel_clock() = KernelClock();
     if (!committed_) rollback();
   }
 
@@ -103,7 +104,9 @@ class Op {
   void *temp(int i) const { return temps_[i]; }
 
   // Starts the task: stream, ordering against earlier work, input transfers.
-  int begin(int data_kind, int coh_ctrl) {
+  // clock: the kernels launched until end() carry dispatch timestamps, which
+  // talshTaskTime reports as the compute time.
+  int begin(int data_kind, int coh_ctrl, bool clock = false) {
     Device &d = rt().gpus[g_];
     (void)hipSetDevice(d.hip_id);
     // stream choice weight: host-link bytes, plus device bytes at the HBM/link speed ratio
@@ -119,10 +122,10 @@ class Op {
     (void)hipEventRecord(task_->ev_start->ev, s_);
     for (auto &o : ops_) {
       if (o.resident) {
-        if (o.output) hazard_wait_write(o.dptr, s_);
-        else hazard_wait_read(o.dptr, s_);
+        if (o.output) hazard_wait_write(o.dptr, o.bytes, s_);
+        else hazard_wait_read(o.dptr, o.bytes, s_);
       } else {
-        hazard_wait_read(o.src_p, s_);
+        hazard_wait_read(o.src_p, o.bytes, s_);
         if (o.read) {
           if (hipMemcpyAsync(o.dptr, o.src_p, o.bytes, hipMemcpyDefault, s_) != hipSuccess) return TALSH_FAILURE;
           task_->bytes += (double)o.bytes;
@@ -130,8 +133,17 @@ class Op {
         }
       }
     }
-    for (auto p : temps_) hazard_wait_write(p, s_);
+    for (size_t i = 0; i < temps_.size(); ++i) hazard_wait_write(temps_[i], temp_bytes_[i], s_);
     (void)hipEventRecord(task_->ev_in->ev, s_);
+    if (clock) {
+      task_->ev_k0 = new_event(g_);
+      task_->ev_k1 = new_event(g_);
+      KernelClock &c = kernel_clock();
+      c.on = true;
+      c.started = false;
+      c.t0 = task_->ev_k0->ev;
+      c.t1 = task_->ev_k1->ev;
+    }
     data_kind_ = data_kind;
     coh_ctrl_ = coh_ctrl;
     return TALSH_SUCCESS;
@@ -149,10 +161,13 @@ class Op {
   // Ends the task: output transfers, completion event, coherence bookkeeping.
   int end(double flops) {
     Device &d = rt().gpus[g_];
+    KernelClock &clk = kernel_clock();
+    task_->kclock = clk.on && clk.started;
+    clk = KernelClock();
     (void)hipEventRecord(task_->ev_comp->ev, s_);
     for (auto &o : ops_) {
-      if (o.output && !o.resident && (o.coh == COPY_T || o.coh == COPY_K)) {
-        hazard_wait_write(o.src_p, s_);
+      if (copied_back(o)) {
+        hazard_wait_write(o.src_p, o.bytes, s_);
         if (hipMemcpyAsync(o.src_p, o.dptr, o.bytes, hipMemcpyDefault, s_) != hipSuccess) return TALSH_FAILURE;
         task_->bytes += (double)o.bytes;
         d.traffic_out += (double)o.bytes;
@@ -160,17 +175,16 @@ class Op {
     }
     (void)hipEventRecord(task_->ev_done->ev, s_);
     if (hipGetLastError() != hipSuccess) return TALSH_FAILURE;
-    // ordering state of every buffer touched
+    // ordering state of every buffer touched; a non-resident output's source
+    // image is written only when the result is copied back to it
     for (auto &o : ops_) {
-      if (o.output) {
-        hazard_set_writer(o.dptr, task_->ev_done);
-        if (!o.resident) hazard_set_writer(o.src_p, task_->ev_done);
-      } else {
-        hazard_add_reader(o.dptr, task_->ev_done);
-        if (!o.resident) hazard_add_reader(o.src_p, task_->ev_done);
-      }
+      if (o.output) hazard_set_writer(o.dptr, o.bytes, task_->ev_done);
+      else hazard_add_reader(o.dptr, o.bytes, task_->ev_done);
+      if (o.resident) continue;
+      if (copied_back(o)) hazard_set_writer(o.src_p, o.bytes, task_->ev_done);
+      else hazard_add_reader(o.src_p, o.bytes, task_->ev_done);
     }
-    for (auto p : temps_) hazard_set_writer(p, task_->ev_done);
+    for (size_t i = 0; i < temps_.size(); ++i) hazard_set_writer(temps_[i], temp_bytes_[i], task_->ev_done);
     // output images other than the source become stale
     for (auto &o : ops_)
       if (o.output) {
@@ -192,6 +206,9 @@ class Op {
       for (size_t i = 0; i < temps.size(); ++i) dev_free(g, temps[i], th[i]);
       for (const auto &o : ops) finalize_operand(g, o);
     });
+    std::vector<talsh_tens_t *> tens;
+    for (auto &o : ops_) tens.push_back(o.t);
+    task_register(task_, tens);
     task_->flops = flops;
     d.tasks_submitted++;
     d.active++;
@@ -228,15 +245,26 @@ class Op {
   }
 
  private:
+  static bool copied_back(const Operand &o) {
+    return o.output && !o.resident && (o.coh == COPY_T || o.coh == COPY_K);
+  }
+
   static int find_image(const talsh_tens_t *t, const void *p) {
     for (int i = 0; i < t->ndev; ++i)
       if (t->dev_rsc[i].gmem_p == p) return i;
     return -1;
   }
 
+  // Coherence work of one operand.  The tensor is alive here: destructing a
+  // tensor first finalizes the tasks that name it.  A buffer the task owns is
+  // released even if the tensor was emptied by other means meanwhile.
   static void finalize_operand(int g, const Operand &o) {
     talsh_tens_t *t = o.t;
-    if (talshTensorIsEmpty(t) == YEP) return;
+    const bool owns = !o.resident && o.dptr && o.dhandle >= 0;
+    if (talshTensorIsEmpty(t) == YEP || tensor_healthy(t) != YEP) {
+      if (owns) dev_free(g, o.dptr, o.dhandle);
+      return;
+    }
     const int si = find_image(t, o.src_p);
     if (o.resident) {
       if (si >= 0) {
@@ -379,8 +407,12 @@ TtgtTemps ttgt_temp_bytes(const ContractPlan &plan, int dk, size_t lbytes, size_
   int nm = 0, nn = 0;
   if (d_in_place(plan, &nm, &nn)) t.d = (size_t)(plan.M + plan.N) * sizeof(int64_t);  // the C map
   else if (plan.d_transpose) t.d = dbytes;
-  if (dk == R4 && rt().r4mode == R4_SPLIT_BF16) t.w = gemm_split_workspace(rt().split_method, plan.M, plan.N, plan.K);
-  else t.s = gemm_tail_workspace(dk, plan.M, plan.N, plan.K, plan.batch);
+  if (dk == R4 && rt().r4mode == R4_SPLIT_BF16) {
+    t.split = rt().split_method;
+    t.w = gemm_split_workspace(t.split, plan.M, plan.N, plan.K);
+  } else {
+    t.s = gemm_tail_workspace(dk, plan.M, plan.N, plan.K, plan.batch);
+  }
   return t;
 }
 
@@ -435,13 +467,13 @@ int enqueue_ttgt(const DigitalPattern &p, const ContractPlan &plan, int dk, bool
   if (p.conj & 1) { lconj ^= 1; rconj ^= 1; }
   if (!cplx) lconj = rconj = 0;
   if (gemm_start) (void)hipEventRecord(gemm_start, s);
-  const bool split = (dk == R4 && rt().r4mode == R4_SPLIT_BF16);
+  const bool split = tmp.split >= 0;
   // hyper indices: plan.batch products of M x N x K, each operand strided by one matrix
   const int64_t sa = plan.M * plan.K, sb = plan.K * plan.N, sc = plan.M * plan.N;
   if (split) {
     if (!tmp.w) return TALSH_FAILURE;
     for (int64_t q = 0; q < plan.batch; ++q)  // the planes workspace is reused in stream order
-      if (launch_gemm_split(rt().split_method, plan.a.op, plan.b.op, plan.M, plan.N, plan.K, ar,
+      if (launch_gemm_split(tmp.split, plan.a.op, plan.b.op, plan.M, plan.N, plan.K, ar,
                             (const float *)A + q * sa, plan.a.ld,
                             (const float *)B + q * sb, plan.b.ld, acc ? 1 : 0, (float *)C + q * sc,
                             std::max<int64_t>(1, plan.M), tmp.w, s, cmap))
@@ -476,7 +508,7 @@ int talshTensorInit(talsh_tens_t *dtens, double val_real, double val_imag, int d
   if (int e = op.add(dtens, coh_letter(copy_ctrl, 1, 0), true, false)) return e;
   if (int e = op.acquire({})) return deferred(g, e);
   Operand &d = op.operand(0);
-  if (int e = op.begin(d.dk, copy_ctrl)) return e;
+  if (int e = op.begin(d.dk, copy_ctrl, true)) return e;
   if (launch_init(d.dk, d.dptr, shape_volume(dtens->shape_p), val_real, val_imag, op.stream())) return TALSH_FAILURE;
   if (int e = op.end(0.0)) return e;
   return op.publish(1);
@@ -497,7 +529,7 @@ int talshTensorScale(talsh_tens_t *dtens, double val_real, double val_imag, int 
   if (int e = op.add(dtens, coh_letter(copy_ctrl, 1, 0), true, true)) return e;
   if (int e = op.acquire({})) return deferred(g, e);
   Operand &d = op.operand(0);
-  if (int e = op.begin(d.dk, copy_ctrl)) return e;
+  if (int e = op.begin(d.dk, copy_ctrl, true)) return e;
   const size_t vol = shape_volume(dtens->shape_p);
   if (launch_scale(d.dk, d.dptr, vol, val_real, val_imag, op.stream())) return TALSH_FAILURE;
   if (int e = op.end((d.dk == C4 || d.dk == C8) ? 6.0 * vol : 1.0 * vol)) return e;
@@ -542,7 +574,7 @@ int talshTensorCopy(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens,
   if (op.operand(0).dk != op.operand(1).dk) return TALSH_INVALID_ARGS;
   if (int e = op.acquire({})) return deferred(g, e);
   const int dk = op.operand(0).dk;
-  if (int e = op.begin(dk, copy_ctrl)) return e;
+  if (int e = op.begin(dk, copy_ctrl, true)) return e;
   std::vector<int> ld = dims_of(ltens);
   if (launch_permute(dk, op.operand(1).dptr, op.operand(0).dptr, (int)ld.size(), ld.data(), o2n.data(), PERM_COPY,
                      1.0, 0.0, conj, op.stream()))
@@ -572,7 +604,7 @@ int talshTensorAdd(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *ltens, 
   if (op.operand(0).dk != op.operand(1).dk) return TALSH_INVALID_ARGS;
   if (int e = op.acquire({})) return deferred(g, e);
   const int dk = op.operand(0).dk;
-  if (int e = op.begin(dk, copy_ctrl)) return e;
+  if (int e = op.begin(dk, copy_ctrl, true)) return e;
   std::vector<int> ld = dims_of(ltens);
   if (launch_permute(dk, op.operand(1).dptr, op.operand(0).dptr, (int)ld.size(), ld.data(), o2n.data(), PERM_ADD,
                      scale_real, scale_imag, conj, op.stream()))
@@ -613,7 +645,7 @@ static int slice_insert(int mode, talsh_tens_t *dtens, talsh_tens_t *ltens, cons
   if (op.operand(0).dk != op.operand(1).dk) return TALSH_INVALID_ARGS;
   if (int e = op.acquire({})) return deferred(g, e);
   const int dk = op.operand(0).dk;
-  if (int e = op.begin(dk, copy_ctrl)) return e;
+  if (int e = op.begin(dk, copy_ctrl, true)) return e;
   std::vector<int> sd = dims_of(small), bd = dims_of(big);
   if (launch_slice(dk, mode, op.operand(1).dptr, op.operand(0).dptr, rank, sd.data(), bd.data(), offsets,
                    acc ? 1 : 0, op.stream()))
@@ -730,6 +762,7 @@ int talshTensorContract(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *lt
   bufs.w = iw >= 0 ? op.temp(iw) : nullptr;
   bufs.s = is >= 0 ? op.temp(is) : nullptr;
   bufs.s_bytes = tb.s;
+  bufs.split = tb.split;
   op.gemm_events();
   if (int e = enqueue_ttgt(p, plan, dk, acc, scale_real, scale_imag, op.operand(0).dptr, op.operand(1).dptr,
                            op.operand(2).dptr, dtens->shape_p->dims, ltens->shape_p->dims, rtens->shape_p->dims,
@@ -762,7 +795,7 @@ int talshTensorNorm2(talsh_tens_t *dtens, talsh_tens_t *ltens, int dev_id, int d
   const int dk = op.operand(1).dk;
   if (op.operand(0).dk != dk) return TALSH_INVALID_ARGS;
   if (int e = op.acquire({norm2_scratch_elems() * sizeof(double)})) return deferred(g, e);
-  if (int e = op.begin(dk, copy_ctrl)) return e;
+  if (int e = op.begin(dk, copy_ctrl, true)) return e;
   const size_t vol = shape_volume(ltens->shape_p);
   if (launch_norm2(dk, op.operand(1).dptr, vol, (double *)op.temp(0), op.operand(0).dptr, op.stream()))
     return TALSH_FAILURE;
@@ -809,27 +842,33 @@ int talshTensorPlace(talsh_tens_t *tens, int dev_id, int dev_kind, void *dev_mem
   hipStream_t s = task->stream;
   void *sp = tens->dev_rsc[src].gmem_p;
   (void)hipEventRecord(task->ev_start->ev, s);
-  hazard_wait_read(sp, s);
-  hazard_wait_write(dst, s);
+  hazard_wait_read(sp, bytes, s);
+  hazard_wait_write(dst, bytes, s);
   (void)hipEventRecord(task->ev_in->ev, s);
   if (hipMemcpyAsync(dst, sp, bytes, hipMemcpyDefault, s) != hipSuccess) {
     task_free(task);
     if (!attached) {
-      if (target == DEV_HOST) host_free(dst, h);
+      if (target == DEV_HOST) host_free(dst, h, bytes);
       else dev_free(tg, dst, h);
     }
     return TALSH_FAILURE;
   }
   (void)hipEventRecord(task->ev_comp->ev, s);
   (void)hipEventRecord(task->ev_done->ev, s);
-  hazard_add_reader(sp, task->ev_done);
-  hazard_set_writer(dst, task->ev_done);
+  hazard_add_reader(sp, bytes, task->ev_done);
+  hazard_set_writer(dst, bytes, task->ev_done);
   if (target == DEV_HOST) d.traffic_out += (double)bytes;
   else d.traffic_in += (double)bytes;
   task->bytes = (double)bytes;
   if (copy_ctrl == COPY_M) tens->avail[src] = NOPE;
-  task->on_finalize.push_back([tens, sp, dst, h, attached, target, dk, copy_ctrl]() {
-    if (talshTensorIsEmpty(tens) == YEP) return;
+  task->on_finalize.push_back([tens, sp, dst, h, attached, target, tg, bytes, dk, copy_ctrl]() {
+    if (talshTensorIsEmpty(tens) == YEP) {  // the new image has no owner: release it
+      if (!attached) {
+        if (target == DEV_HOST) host_free(dst, h, bytes);
+        else dev_free(tg, dst, h);
+      }
+      return;
+    }
     int si = -1;
     for (int i = 0; i < tens->ndev; ++i)
       if (tens->dev_rsc[i].gmem_p == sp) si = i;
@@ -839,6 +878,7 @@ int talshTensorPlace(talsh_tens_t *tens, int dev_id, int dev_kind, void *dev_mem
     }
     image_add(tens, target, dst, h, attached, dk);
   });
+  task_register(task, {tens});
   d.tasks_submitted++;
   d.active++;
   talsh_task_t local;
@@ -906,12 +946,14 @@ int talshEnableFastMath(int dev_kind, int dev_id) {
   (void)dev_kind;
   (void)dev_id;
   if (!gemm_split_bf16_available()) return TALSH_NOT_AVAILABLE;
+  std::lock_guard<std::recursive_mutex> lk(rt().mu);  // not while an operation plans
   rt().r4mode = R4_SPLIT_BF16;
   return TALSH_SUCCESS;
 }
 int talshDisableFastMath(int dev_kind, int dev_id) {
   (void)dev_kind;
   (void)dev_id;
+  std::lock_guard<std::recursive_mutex> lk(rt().mu);
   rt().r4mode = R4_FP32;
   return TALSH_SUCCESS;
 }

@@ -389,10 +389,10 @@ template <typename T, int TA, int V>
 void launch_rows(const T *src, T *dst, const PermArgs &p, int op, typename Real<T>::type ar,
                  typename Real<T>::type ai, int conj, int64_t nblk, hipStream_t s) {
   if (op == PERM_COPY)
-    perm_rows_kernel<T, TA, V, PERM_COPY><<<dim3((unsigned)nblk), dim3(kThreads), 0, s>>>(src, dst, p, ar, ai,
+    launch_clocked(perm_rows_kernel<T, TA, V, PERM_COPY>, dim3((unsigned)nblk), dim3(kThreads), s, src, dst, p, ar, ai,
                                                                                           conj);
   else
-    perm_rows_kernel<T, TA, V, PERM_ADD><<<dim3((unsigned)nblk), dim3(kThreads), 0, s>>>(src, dst, p, ar, ai,
+    launch_clocked(perm_rows_kernel<T, TA, V, PERM_ADD>, dim3((unsigned)nblk), dim3(kThreads), s, src, dst, p, ar, ai,
                                                                                          conj);
 }
 
@@ -400,10 +400,10 @@ template <typename T, int TA, int TB, int V>
 void launch_tile(const T *src, T *dst, const PermArgs &p, int op, typename Real<T>::type ar,
                  typename Real<T>::type ai, int conj, int64_t nblk, hipStream_t s) {
   if (op == PERM_COPY)
-    perm_tile_kernel<T, TA, TB, V, PERM_COPY><<<dim3((unsigned)nblk), dim3(kThreads), 0, s>>>(src, dst, p, ar, ai,
+    launch_clocked(perm_tile_kernel<T, TA, TB, V, PERM_COPY>, dim3((unsigned)nblk), dim3(kThreads), s, src, dst, p, ar, ai,
                                                                                               conj);
   else
-    perm_tile_kernel<T, TA, TB, V, PERM_ADD><<<dim3((unsigned)nblk), dim3(kThreads), 0, s>>>(src, dst, p, ar, ai,
+    launch_clocked(perm_tile_kernel<T, TA, TB, V, PERM_ADD>, dim3((unsigned)nblk), dim3(kThreads), s, src, dst, p, ar, ai,
                                                                                              conj);
 }
 
@@ -471,14 +471,14 @@ int launch_perm_typed(const T *src, T *dst, int rank, const int *dims, const int
     const int g = grid_for(vol / VW);
     if (aligned) {
       if (op == PERM_COPY)
-        perm_flat_kernel<T, VW, PERM_COPY><<<g, kThreads, 0, s>>>(src, dst, vol, ar, ai, conj);
+        launch_clocked(perm_flat_kernel<T, VW, PERM_COPY>, dim3(g), dim3(kThreads), s, src, dst, vol, ar, ai, conj);
       else
-        perm_flat_kernel<T, VW, PERM_ADD><<<g, kThreads, 0, s>>>(src, dst, vol, ar, ai, conj);
+        launch_clocked(perm_flat_kernel<T, VW, PERM_ADD>, dim3(g), dim3(kThreads), s, src, dst, vol, ar, ai, conj);
     } else {
       if (op == PERM_COPY)
-        perm_flat_kernel<T, 1, PERM_COPY><<<g, kThreads, 0, s>>>(src, dst, vol, ar, ai, conj);
+        launch_clocked(perm_flat_kernel<T, 1, PERM_COPY>, dim3(g), dim3(kThreads), s, src, dst, vol, ar, ai, conj);
       else
-        perm_flat_kernel<T, 1, PERM_ADD><<<g, kThreads, 0, s>>>(src, dst, vol, ar, ai, conj);
+        launch_clocked(perm_flat_kernel<T, 1, PERM_ADD>, dim3(g), dim3(kThreads), s, src, dst, vol, ar, ai, conj);
     }
     return hipGetLastError() == hipSuccess ? 0 : -2;
   }
@@ -670,8 +670,8 @@ int launch_slice_typed(int mode, const T *src, T *dst, int rank, const int *sdim
   const int64_t nblk = p.nchunk * ((p.nrows + ry - 1) / ry);
   if (nblk > 0x7fffffffLL) return -4;
 #define TAL_SLICE_LAUNCH(VV)                                                                                   \
-  (accumulative ? slice_kernel<T, VV, PERM_ADD><<<dim3((unsigned)nblk), dim3(kThreads), 0, s>>>(src, dst, p)   \
-                : slice_kernel<T, VV, PERM_COPY><<<dim3((unsigned)nblk), dim3(kThreads), 0, s>>>(src, dst, p))
+  (accumulative ? launch_clocked(slice_kernel<T, VV, PERM_ADD>, dim3((unsigned)nblk), dim3(kThreads), s, src, dst, p)   \
+                : launch_clocked(slice_kernel<T, VV, PERM_COPY>, dim3((unsigned)nblk), dim3(kThreads), s, src, dst, p))
   if (V > 1)
     TAL_SLICE_LAUNCH(VW);
   else
@@ -723,10 +723,10 @@ int launch_init(int dk, void *d, size_t vol, double vr, double vi, hipStream_t s
   if (vol == 0) return 0;
   const int g = grid_for((int64_t)vol);
   switch (dk) {
-    case 4: init_kernel<float><<<g, kThreads, 0, s>>>((float *)d, vol, make_val<float>(vr, vi)); break;
-    case 8: init_kernel<double><<<g, kThreads, 0, s>>>((double *)d, vol, make_val<double>(vr, vi)); break;
-    case 14: init_kernel<float2><<<g, kThreads, 0, s>>>((float2 *)d, vol, make_val<float2>(vr, vi)); break;
-    case 18: init_kernel<double2><<<g, kThreads, 0, s>>>((double2 *)d, vol, make_val<double2>(vr, vi)); break;
+    case 4: launch_clocked(init_kernel<float>, dim3(g), dim3(kThreads), s, (float *)d, vol, make_val<float>(vr, vi)); break;
+    case 8: launch_clocked(init_kernel<double>, dim3(g), dim3(kThreads), s, (double *)d, vol, make_val<double>(vr, vi)); break;
+    case 14: launch_clocked(init_kernel<float2>, dim3(g), dim3(kThreads), s, (float2 *)d, vol, make_val<float2>(vr, vi)); break;
+    case 18: launch_clocked(init_kernel<double2>, dim3(g), dim3(kThreads), s, (double2 *)d, vol, make_val<double2>(vr, vi)); break;
     default: return -1;
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
@@ -737,16 +737,21 @@ int launch_scale(int dk, void *d, size_t vol, double ar, double ai, hipStream_t 
   if (vol == 0) return 0;
   const int g = grid_for((int64_t)vol);
   switch (dk) {
-    case 4: scale_kernel<float><<<g, kThreads, 0, s>>>((float *)d, vol, (float)ar, (float)ai); break;
-    case 8: scale_kernel<double><<<g, kThreads, 0, s>>>((double *)d, vol, ar, ai); break;
-    case 14: scale_kernel<float2><<<g, kThreads, 0, s>>>((float2 *)d, vol, (float)ar, (float)ai); break;
-    case 18: scale_kernel<double2><<<g, kThreads, 0, s>>>((double2 *)d, vol, ar, ai); break;
+    case 4: launch_clocked(scale_kernel<float>, dim3(g), dim3(kThreads), s, (float *)d, vol, (float)ar, (float)ai); break;
+    case 8: launch_clocked(scale_kernel<double>, dim3(g), dim3(kThreads), s, (double *)d, vol, ar, ai); break;
+    case 14: launch_clocked(scale_kernel<float2>, dim3(g), dim3(kThreads), s, (float2 *)d, vol, (float)ar, (float)ai); break;
+    case 18: launch_clocked(scale_kernel<double2>, dim3(g), dim3(kThreads), s, (double2 *)d, vol, ar, ai); break;
     default: return -1;
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 size_t norm2_scratch_elems() { return kNormBlocks; }
+
+KernelClock &kernel_clock() {
+  static thread_local KernelClock c;
+  return c;
+}
 
 template <typename T>
 static int launch_norm2_typed(const T *x, size_t vol, double *partials, void *out, hipStream_t s) {
@@ -756,10 +761,10 @@ static int launch_norm2_typed(const T *x, size_t vol, double *partials, void *ou
   const int64_t packs = aligned ? n / VW : n;
   const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(kNormBlocks, (packs + 4 * kThreads - 1) / (4 * kThreads)));
   if (aligned)
-    norm2_stage1<T, VW><<<nb, kThreads, 0, s>>>(x, n, partials);
+    launch_clocked(norm2_stage1<T, VW>, dim3(nb), dim3(kThreads), s, x, n, partials);
   else
-    norm2_stage1<T, 1><<<nb, kThreads, 0, s>>>(x, n, partials);
-  norm2_stage2<T><<<1, kThreads, 0, s>>>(partials, nb, (T *)out);
+    launch_clocked(norm2_stage1<T, 1>, dim3(nb), dim3(kThreads), s, x, n, partials);
+  launch_clocked(norm2_stage2<T>, dim3(1), dim3(kThreads), s, partials, nb, (T *)out);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

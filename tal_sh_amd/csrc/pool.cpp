@@ -40,6 +40,11 @@ void Arena::erase_free(std::map<size_t, size_t>::iterator it) {
 
 size_t Arena::largest_free() const { return by_len_.empty() ? 0 : by_len_.rbegin()->first; }
 
+size_t Arena::length(int handle) const {
+  if (handle < 0 || handle >= static_cast<int>(slots_.size()) || !slots_[handle].used) return 0;
+  return slots_[handle].len;
+}
+
 bool Arena::owns(const void *p) const {
   const char *c = static_cast<const char *>(p);
   return base_ && c >= base_ && c < base_ + size_;

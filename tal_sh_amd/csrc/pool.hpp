@@ -29,6 +29,8 @@ class Arena {
   int alloc(size_t bytes, void **ptr, int *handle);
   // 0 on success, -1 for an unknown handle.
   int release(int handle);
+  // Bytes held by a live handle (its request rounded up to the alignment), 0 otherwise.
+  size_t length(int handle) const;
   bool owns(const void *p) const;
   void *base() const { return base_; }
   size_t size() const { return size_; }

@@ -94,24 +94,68 @@ EvRef new_event(int g) {
 
 static bool ev_done(const EvRef &e) { return !e || !e->ev || hipEventQuery(e->ev) == hipSuccess; }
 
-void hazard_wait_read(const void *p, hipStream_t s) {
-  auto it = rt().hazards.find(p);
-  if (it == rt().hazards.end()) return;
-  if (it->second.writer && !ev_done(it->second.writer)) (void)hipStreamWaitEvent(s, it->second.writer->ev, 0);
-}
+namespace {
 
-void hazard_wait_write(const void *p, hipStream_t s) {
-  auto it = rt().hazards.find(p);
-  if (it == rt().hazards.end()) return;
-  Hazard &h = it->second;
-  if (h.writer && !ev_done(h.writer)) (void)hipStreamWaitEvent(s, h.writer->ev, 0);
+int len_class(size_t n) { return 63 - __builtin_clzll((unsigned long long)(n | 1)); }
+
+bool hazard_idle(const Hazard &h) {
+  if (!ev_done(h.writer)) return false;
   for (auto &e : h.readers)
-    if (!ev_done(e)) (void)hipStreamWaitEvent(s, e->ev, 0);
+    if (!ev_done(e)) return false;
+  return true;
 }
 
-void hazard_add_reader(const void *p, const EvRef &e) {
+// Calls f(hazard) for every filed range overlapping [p, p + n); ranges whose
+// events have all completed are forgotten on the way (they order nothing).
+template <class F>
+void each_overlap(const void *ptr, size_t n, F &&f) {
+  Runtime &r = rt();
+  const uintptr_t p = (uintptr_t)ptr, end = p + (n ? n : 1);
+  for (int c = 0; c < kHazardClasses; ++c) {
+    auto &m = r.hazards[c];
+    if (m.empty()) continue;
+    const uintptr_t span = (uintptr_t)2 << c;  // class-c ranges are shorter than 2^(c+1)
+    const uintptr_t lo = p > span ? p - span : 0;
+    for (auto it = m.lower_bound(HazardKey(lo, 0)); it != m.end() && it->first.first < end;) {
+      if (it->first.first + it->first.second <= p) {
+        ++it;
+        continue;
+      }
+      if (hazard_idle(it->second)) {
+        it = m.erase(it);
+        r.hazard_epoch++;
+        continue;
+      }
+      f(it->second);
+      ++it;
+    }
+  }
+}
+
+Hazard &hazard_entry(const void *p, size_t n) {
+  n = n ? n : 1;
+  return rt().hazards[len_class(n)][HazardKey((uintptr_t)p, n)];
+}
+
+}  // namespace
+
+void hazard_wait_read(const void *p, size_t n, hipStream_t s) {
+  each_overlap(p, n, [s](Hazard &h) {
+    if (h.writer && !ev_done(h.writer)) (void)hipStreamWaitEvent(s, h.writer->ev, 0);
+  });
+}
+
+void hazard_wait_write(const void *p, size_t n, hipStream_t s) {
+  each_overlap(p, n, [s](Hazard &h) {
+    if (h.writer && !ev_done(h.writer)) (void)hipStreamWaitEvent(s, h.writer->ev, 0);
+    for (auto &e : h.readers)
+      if (!ev_done(e)) (void)hipStreamWaitEvent(s, e->ev, 0);
+  });
+}
+
+void hazard_add_reader(const void *p, size_t n, const EvRef &e) {
   rt().hazard_epoch++;
-  Hazard &h = rt().hazards[p];
+  Hazard &h = hazard_entry(p, n);
   for (auto &x : h.readers)
     if (x == e) return;  // the same event (re-recorded by its owner) is listed once
   if (h.readers.size() >= 16) {  // prune completed readers
@@ -123,21 +167,43 @@ void hazard_add_reader(const void *p, const EvRef &e) {
   h.readers.push_back(e);
 }
 
-void hazard_set_writer(const void *p, const EvRef &e) {
+// The writer has waited (hazard_wait_write) on every read of this range, so
+// the range's readers are superseded.  Other overlapping ranges keep their
+// own state: queries visit them too.
+void hazard_set_writer(const void *p, size_t n, const EvRef &e) {
   rt().hazard_epoch++;
-  Hazard &h = rt().hazards[p];
+  Hazard &h = hazard_entry(p, n);
   h.writer = e;
   h.readers.clear();
 }
 
-void hazard_drain(const void *p) {
-  auto it = rt().hazards.find(p);
-  if (it == rt().hazards.end()) return;
-  rt().hazard_epoch++;
-  if (it->second.writer && it->second.writer->ev) (void)hipEventSynchronize(it->second.writer->ev);
-  for (auto &e : it->second.readers)
-    if (e && e->ev) (void)hipEventSynchronize(e->ev);
-  rt().hazards.erase(it);
+void hazard_drain(const void *ptr, size_t n) {
+  Runtime &r = rt();
+  const uintptr_t p = (uintptr_t)ptr, end = p + (n ? n : 1);
+  for (int c = 0; c < kHazardClasses; ++c) {
+    auto &m = r.hazards[c];
+    if (m.empty()) continue;
+    const uintptr_t span = (uintptr_t)2 << c;
+    const uintptr_t lo = p > span ? p - span : 0;
+    for (auto it = m.lower_bound(HazardKey(lo, 0)); it != m.end() && it->first.first < end;) {
+      if (it->first.first + it->first.second <= p) {
+        ++it;
+        continue;
+      }
+      Hazard &h = it->second;
+      if (h.writer && h.writer->ev) (void)hipEventSynchronize(h.writer->ev);
+      for (auto &e : h.readers)
+        if (e && e->ev) (void)hipEventSynchronize(e->ev);
+      it = m.erase(it);
+      r.hazard_epoch++;
+    }
+  }
+}
+
+size_t image_bytes(const talsh_tens_t *t, int i) {
+  if (!t || !t->shape_p || i < 0 || i >= t->ndev) return 0;
+  const int s = dk_size(t->data_kind[i]);
+  return shape_volume(t->shape_p) * (size_t)(s > 0 ? s : 1);
 }
 
 // ------------------------------------------------------------------ memory
@@ -150,7 +216,7 @@ int dev_alloc(int g, size_t bytes, void **p, int *handle) {
 }
 
 void dev_free(int g, void *p, int handle) {
-  hazard_drain(p);
+  hazard_drain(p, rt().gpus[g].arena.length(handle));
   rt().gpus[g].arena.release(handle);
   if (rt().log_mem) std::printf("#TALSH MEM: GPU %d release %p (entry %d)\n", g, p, handle);
 }
@@ -167,8 +233,8 @@ int host_alloc(size_t bytes, void **p, int *handle) {
   return *p ? 0 : TRY_LATER;
 }
 
-void host_free(void *p, int handle) {
-  hazard_drain(p);
+void host_free(void *p, int handle, size_t bytes) {
+  hazard_drain(p, handle >= 0 ? rt().host_arena.length(handle) : bytes);
   if (handle >= 0)
     rt().host_arena.release(handle);
   else
@@ -188,15 +254,15 @@ int image_add(talsh_tens_t *t, int dev_id, void *p, int buf_entry, int attached,
   return i;
 }
 
-void image_release(talsh_dev_rsc_t *r) {
+void image_release(talsh_dev_rsc_t *r, size_t bytes) {
   if (r->gmem_p && !r->mem_attached) {
     const int g = gpu_index(r->dev_id);
     if (r->dev_id == DEV_HOST)
-      host_free(r->gmem_p, r->buf_entry);
+      host_free(r->gmem_p, r->buf_entry, bytes);
     else if (g >= 0)
       dev_free(g, r->gmem_p, r->buf_entry);
   } else if (r->gmem_p) {
-    hazard_drain(r->gmem_p);
+    hazard_drain(r->gmem_p, bytes);
   }
   r->dev_id = DEV_NULL;
   r->gmem_p = nullptr;
@@ -206,7 +272,7 @@ void image_release(talsh_dev_rsc_t *r) {
 
 int image_discard(talsh_tens_t *t, int i) {
   if (i < 0 || i >= t->ndev) return -1;
-  image_release(&t->dev_rsc[i]);
+  image_release(&t->dev_rsc[i], image_bytes(t, i));
   for (int j = i; j + 1 < t->ndev; ++j) {
     t->dev_rsc[j] = t->dev_rsc[j + 1];
     t->data_kind[j] = t->data_kind[j + 1];
@@ -275,30 +341,65 @@ Task *task_new(int g, double weight) {
 
 void task_free(Task *t) { delete t; }
 
+void task_register(Task *t, const std::vector<talsh_tens_t *> &tens) {
+  for (talsh_tens_t *x : tens) {
+    if (!x) continue;
+    bool seen = false;
+    for (talsh_tens_t *y : t->tens) seen = seen || y == x;
+    if (seen) continue;
+    t->tens.push_back(x);
+    rt().tens_tasks[x].push_back(t);
+  }
+}
+
+int task_finalize(Task *t, bool block) {
+  if (t->status == TALSH_TASK_COMPLETED || t->status == TALSH_TASK_ERROR) return t->status;
+  hipError_t q = block ? hipEventSynchronize(t->ev_done->ev) : hipEventQuery(t->ev_done->ev);
+  if (q == hipErrorNotReady) return t->status;
+  Runtime &r = rt();
+  Device &d = r.gpus[t->dev];
+  auto fin = std::move(t->on_finalize);
+  t->on_finalize.clear();
+  for (auto &f : fin) f();
+  for (talsh_tens_t *x : t->tens) {
+    auto it = r.tens_tasks.find(x);
+    if (it == r.tens_tasks.end()) continue;
+    auto &v = it->second;
+    for (size_t i = 0; i < v.size(); ++i)
+      if (v[i] == t) {
+        v.erase(v.begin() + (long)i);
+        break;
+      }
+    if (v.empty()) r.tens_tasks.erase(it);
+  }
+  t->tens.clear();
+  d.active--;
+  if (q == hipSuccess) {
+    t->status = TALSH_TASK_COMPLETED;
+    d.tasks_completed++;
+    d.flops += t->flops;
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, t->ev_start->ev, t->ev_done->ev) == hipSuccess) t->exec_time = ms * 1e-3;
+  } else {
+    t->status = TALSH_TASK_ERROR;
+    t->error = (int)q;
+    d.tasks_failed++;
+  }
+  return t->status;
+}
+
 // Finalises a scheduled task once its device work is done; returns the status.
 int task_finalize_if_done(talsh_task_t *tt, bool block) {
   Task *t = static_cast<Task *>(tt->task_p);
   if (!t) return TALSH_TASK_EMPTY;
-  if (t->status == TALSH_TASK_COMPLETED || t->status == TALSH_TASK_ERROR) return t->status;
-  hipError_t q = block ? hipEventSynchronize(t->ev_done->ev) : hipEventQuery(t->ev_done->ev);
-  if (q == hipErrorNotReady) return t->status;
-  Device &d = rt().gpus[t->dev];
-  for (auto &f : t->on_finalize) f();
-  t->on_finalize.clear();
-  d.active--;
-  if (q == hipSuccess) {
-    t->status = TALSH_TASK_COMPLETED;
+  const int st = task_finalize(t, block);
+  if (st == TALSH_TASK_COMPLETED) {
     tt->task_error = 0;
-    d.tasks_completed++;
-    d.flops += t->flops;
-    float ms = 0;
-    if (hipEventElapsedTime(&ms, t->ev_start->ev, t->ev_done->ev) == hipSuccess) tt->exec_time = ms * 1e-3;
-  } else {
-    t->status = TALSH_TASK_ERROR;
-    tt->task_error = (int)q;
-    d.tasks_failed++;
+    tt->exec_time = t->exec_time;
+  } else if (st == TALSH_TASK_ERROR) {
+    tt->task_error = t->error;
   }
-  return t->status;
+  return st;
 }
 
 }  // namespace tal
@@ -386,7 +487,8 @@ int talshShutdown(void) {
     (void)hipSetDevice(d.hip_id);
     (void)hipDeviceSynchronize();
   }
-  r.hazards.clear();
+  for (auto &m : r.hazards) m.clear();
+  r.tens_tasks.clear();
   r.hazard_epoch++;
   for (auto &d : r.gpus) {
     if (!d.on) continue;
@@ -722,7 +824,7 @@ int talshTensorImportData(talsh_tens_t *t, int data_kind, const void *ext_data) 
   if (talshTensorIsEmpty(t) == YEP) return TALSH_OBJECT_IS_EMPTY;
   const int i = image_find(t, DEV_HOST, data_kind, true);
   if (i < 0) return TALSH_NOT_FOUND;
-  hazard_drain(t->dev_rsc[i].gmem_p);
+  hazard_drain(t->dev_rsc[i].gmem_p, image_bytes(t, i));
   std::memcpy(t->dev_rsc[i].gmem_p, ext_data, shape_volume(t->shape_p) * (size_t)dk_size(data_kind));
   return TALSH_SUCCESS;
 }
@@ -730,8 +832,19 @@ int talshTensorImportData(talsh_tens_t *t, int data_kind, const void *ext_data) 
 int talshTensorDestruct(talsh_tens_t *t) {
   std::lock_guard<std::recursive_mutex> lk(rt().mu);
   if (!t) return TALSH_INVALID_ARGS;
+  // Tasks that still name the tensor finish first: their coherence work
+  // (images, buffers they own) needs it.  Their handles then report COMPLETED.
+  for (auto it = rt().tens_tasks.find(t); it != rt().tens_tasks.end(); it = rt().tens_tasks.find(t)) {
+    Task *task = it->second.front();
+    (void)task_finalize(task, true);
+    it = rt().tens_tasks.find(t);
+    if (it != rt().tens_tasks.end() && it->second.front() == task) {  // still listed: drop the link
+      it->second.erase(it->second.begin());
+      if (it->second.empty()) rt().tens_tasks.erase(it);
+    }
+  }
   if (t->dev_rsc) {
-    for (int i = 0; i < t->ndev && i < t->dev_rsc_len; ++i) image_release(&t->dev_rsc[i]);
+    for (int i = 0; i < t->ndev && i < t->dev_rsc_len; ++i) image_release(&t->dev_rsc[i], image_bytes(t, i));
     std::free(t->dev_rsc);
   }
   if (t->shape_p) {
@@ -841,7 +954,7 @@ int talshTensorGetBodyAccess(talsh_tens_t *t, void **body_p, int data_kind, int 
   const int i = image_find(t, id, data_kind, false);
   if (i < 0) return TALSH_NOT_FOUND;
   if (t->avail[i] != YEP) return TALSH_NOT_AVAILABLE;
-  hazard_drain(t->dev_rsc[i].gmem_p);  // the caller may touch it directly
+  hazard_drain(t->dev_rsc[i].gmem_p, image_bytes(t, i));  // the caller may touch it directly
   *body_p = t->dev_rsc[i].gmem_p;
   return TALSH_SUCCESS;
 }
@@ -865,7 +978,7 @@ int talshTensorGetScalar(talsh_tens_t *t, double *scalar_real, double *scalar_im
     const int dk = t->data_kind[i];
     unsigned char buf[16] = {0};
     const int g = gpu_index(t->dev_rsc[i].dev_id);
-    hazard_drain(t->dev_rsc[i].gmem_p);
+    hazard_drain(t->dev_rsc[i].gmem_p, image_bytes(t, i));
     if (t->dev_rsc[i].dev_id == DEV_HOST)
       std::memcpy(buf, t->dev_rsc[i].gmem_p, dk_size(dk));
     else if (g >= 0) {
@@ -917,7 +1030,7 @@ void talshTensorPrintBody(const talsh_tens_t *t, double thresh) {
     std::printf(" No Host image\n");
     return;
   }
-  hazard_drain(t->dev_rsc[i].gmem_p);
+  hazard_drain(t->dev_rsc[i].gmem_p, image_bytes(t, i));
   const int nd = t->shape_p->num_dim;
   const int *dims = t->shape_p->dims;
   const size_t v = shape_volume(t->shape_p);
@@ -948,7 +1061,7 @@ double talshTensorImageNorm1_cpu(const talsh_tens_t *t) {
   if (!t || !t->shape_p) return -1.0;
   const int i = image_find(t, DEV_HOST, NO_TYPE, false);
   if (i < 0) return -1.0;
-  hazard_drain(t->dev_rsc[i].gmem_p);
+  hazard_drain(t->dev_rsc[i].gmem_p, image_bytes(t, i));
   const size_t v = shape_volume(t->shape_p);
   double s = 0;
   const void *p = t->dev_rsc[i].gmem_p;
@@ -1100,7 +1213,8 @@ int talshTaskTime(talsh_task_t *tt, double *total, double *comput, double *input
   };
   *total = el(t->ev_start, t->ev_done);
   if (input) *input = el(t->ev_start, t->ev_in);
-  if (comput) *comput = el(t->ev_in, t->ev_comp);
+  // compute: the kernels' own span when they carried dispatch timestamps
+  if (comput) *comput = t->kclock ? el(t->ev_k0, t->ev_k1) : el(t->ev_in, t->ev_comp);
   if (output) *output = el(t->ev_comp, t->ev_done);
   if (mmul) *mmul = (t->ev_mm0 && t->ev_mm1) ? el(t->ev_mm0, t->ev_mm1) : 0.0;
   return TALSH_SUCCESS;

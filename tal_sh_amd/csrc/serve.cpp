@@ -96,7 +96,21 @@ void admit(talshAmdServer *s) {
   for (auto it = s->queue.begin(); it != s->queue.end();) {
     talshAmdServer::Req *q = *it;
     if ((int)s->inflight.size() >= s->max_inflight) break;
-    const bool fits = s->in_use + q->need <= s->budget;
+    bool fits = s->in_use + q->need <= s->budget;
+    // The need depends on where the operands are now (a request that reads
+    // another's output was estimated while that output was in flight) and on
+    // the fast-math mode: it is recomputed before the request is charged, and
+    // for the head whenever it is blocked.
+    if (fits || it == s->queue.begin()) {
+      if (estimate(s, q) != TALSH_SUCCESS) {
+        s->failed++;
+        s->rejected.emplace_back(q->r.tag, TALSH_INVALID_ARGS);
+        it = s->queue.erase(it);
+        delete q;
+        continue;
+      }
+      fits = s->in_use + q->need <= s->budget;
+    }
     if (!fits) {
       if (it == s->queue.begin()) {
         head_blocked = true;
@@ -109,8 +123,9 @@ void admit(talshAmdServer *s) {
     const int e = talshTensorContract(q->pattern.c_str(), q->r.dtens, q->r.ltens, q->r.rtens, q->r.scale_real,
                                       q->r.scale_imag, s->g, DEV_NVIDIA_GPU, q->r.copy_ctrl, q->r.accumulative,
                                       &q->task);
-    if (e == TRY_LATER && !s->inflight.empty()) {
-      // device pool fragmented by requests in flight: retry once some of them complete
+    if ((e == TRY_LATER || e == TALSH_NOT_AVAILABLE) && !s->inflight.empty()) {
+      // device pool fragmented by requests in flight, or an operand is the output of a
+      // request still in flight: retry once some of them complete
       s->deferred++;
       if (it == s->queue.begin()) head_blocked = true;
       ++it;

@@ -67,7 +67,7 @@ int load_box(const talsh_tens_slice_t &s, int dk, void *dst, int g) {
   std::vector<int> bd = ivec(big->shape_p->dims, rank), sd = ivec(s.shape.dims, rank), off = offsets_of(s);
   int i = image_find(big, flat_gpu(g), dk, true);
   if (i >= 0) {
-    hazard_wait_read(big->dev_rsc[i].gmem_p, st);
+    hazard_wait_read(big->dev_rsc[i].gmem_p, image_bytes(big, i), st);
     if (launch_slice(dk, 0, big->dev_rsc[i].gmem_p, dst, rank, sd.data(), bd.data(), off.data(), 0, st))
       return TALSH_FAILURE;
   } else {
@@ -77,7 +77,7 @@ int load_box(const talsh_tens_slice_t &s, int dk, void *dst, int g) {
       i = image_find(big, DEV_HOST, dk, true);
       if (i < 0) return TALSH_NOT_AVAILABLE;
     }
-    hazard_drain(big->dev_rsc[i].gmem_p);
+    hazard_drain(big->dev_rsc[i].gmem_p, image_bytes(big, i));
     if (box_dma(true, (char *)big->dev_rsc[i].gmem_p, (char *)dst, rank, bd.data(), off.data(), sd.data(),
                 (size_t)dk_size(dk), st))
       return TALSH_FAILURE;
@@ -96,7 +96,7 @@ int store_box(const talsh_tens_slice_t &s, int dk, const void *src, int g) {
   int i = image_find(big, flat_gpu(g), dk, true);
   if (i >= 0) {
     void *p = big->dev_rsc[i].gmem_p;
-    hazard_wait_write(p, st);
+    hazard_wait_write(p, image_bytes(big, i), st);
     if (launch_slice(dk, 1, src, p, rank, sd.data(), bd.data(), off.data(), 1, st)) return TALSH_FAILURE;
     if (hipStreamSynchronize(st) != hipSuccess) return TALSH_FAILURE;
     image_discard_other(big, image_find(big, flat_gpu(g), dk, true));
@@ -109,7 +109,7 @@ int store_box(const talsh_tens_slice_t &s, int dk, const void *src, int g) {
     if (i < 0) return TALSH_NOT_AVAILABLE;
   }
   char *host = (char *)big->dev_rsc[i].gmem_p;
-  hazard_drain(host);
+  hazard_drain(host, image_bytes(big, i));
   const size_t esz = (size_t)dk_size(dk), bytes = shape_vol(s.shape) * esz;
   void *tmp = nullptr;
   int h = -1;

@@ -190,7 +190,7 @@ int talshTensorContractXL(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *
   if (int e = pick_home(rtens, g, dk, &hr)) return e == TALSH_NOT_AVAILABLE ? TALSH_INVALID_ARGS : e;
   // the call owns the bodies from here on: wait for earlier work on any image
   for (talsh_tens_t *t : {dtens, ltens, rtens})
-    for (int i = 0; i < t->ndev; ++i) hazard_drain(t->dev_rsc[i].gmem_p);
+    for (int i = 0; i < t->ndev; ++i) hazard_drain(t->dev_rsc[i].gmem_p, image_bytes(t, i));
 
   // ---- index space: D indices [0, drank), then one id per contracted pair
   const int nd = p.drank;
@@ -425,6 +425,7 @@ int talshTensorContractXL(const char *cptrn, talsh_tens_t *dtens, talsh_tens_t *
       grow(want.w, have.w, tmp.w, 0);
       grow(want.s, have.s, tmp.s, 0);
       tmp.s_bytes = have.s;
+      tmp.split = want.split;
       if (err) break;
       const void *lp = fetch(hl, l_in_place, p.lrank, ld, ol.data(), pl.data(), lbuf, s);
       const void *rp = fetch(hr, r_in_place, p.rrank, rd, orr.data(), pr.data(), rbuf, s);

@@ -71,6 +71,20 @@ FIXED_PATTERNS = [
 ]
 
 
+# (pattern, D dims, L dims, R dims) with a rank-0 operand, or no contracted index
+RANK0_CASES = [
+    ("D(a)+=L(a)*R()", [7], [7], []),
+    ("D(a,b,c)+=L(c,a,b)*R()", [3, 4, 5], [5, 3, 4], []),
+    ("D(a,b)+=L()*R(b,a)", [4, 6], [], [6, 4]),
+    ("D()+=L()*R()", [], [], []),
+    ("D(a)+=L+(a)*R()", [9], [9], []),
+    ("D()+=L()*R+()", [], [], []),
+    ("D(a,b)+=L(a)*R(b)", [5, 6], [5], [6]),
+    ("D(a,b)+=L(b)*R+(a)", [4, 3], [3], [4]),
+    ("D()+=L(a,b)*R(b,a)", [], [4, 5], [5, 4]),
+]
+
+
 def ref_contractions_txt():
     pats = []
     if os.path.exists(REF_CONTRACTIONS_TXT):
@@ -193,6 +207,19 @@ def main():
                 arrays[key] = fn(dk, sdims, S, bdims, B, offsets, acc)
                 cases.append({"key": key, "op": op, "dk": dk, "sdims": sdims, "bdims": bdims, "offsets": offsets,
                               "seed_b": seed_b, "seed_s": seed_s, "accumulative": acc})
+    # contractions with a rank-0 operand and outer products (the reference GPU path
+    # has dedicated kernels for these, tensor_algebra_gpu_nvidia.cu:6732-6849)
+    for dk in (R4, R8, C4, C8):
+        for t, (pat, dd, ld, rd) in enumerate(RANK0_CASES):
+            sl, sr, sd = 2000 + 7 * t + dk, 2100 + 7 * t + dk, 2200 + 7 * t + dk
+            alpha = (0.5, -0.75) if dk in (C4, C8) else (-1.5, 0.0)
+            acc = (t + dk) % 2
+            key = "c%d" % len(cases)
+            arrays[key] = ref.contract(dk, pat, dd, tensor(dk, dd, sd), ld, tensor(dk, ld, sl), rd,
+                                       tensor(dk, rd, sr), alpha, acc)
+            cases.append({"key": key, "op": "contract", "dk": dk, "pattern": pat, "ddims": dd, "ldims": ld,
+                          "rdims": rd, "seed_l": sl, "seed_r": sr, "seed_d": sd, "alpha": list(alpha),
+                          "accumulative": acc})
     np.savez_compressed(os.path.join(HERE, "numeric.npz"), **arrays)
     with open(os.path.join(HERE, "numeric.json"), "w") as f:
         json.dump(cases, f, separators=(",", ":"))

@@ -95,6 +95,27 @@ def random_hyper_contraction(rng, max_rank=5, ext_lo=1, ext_hi=6, nhyper=None):
         return pat, [ext[x] for x in d], [ext[x] for x in l], [ext[x] for x in r]
 
 
+def batch_contraction(rng, lo=2, hi=24):
+    """Batched small contraction: D, L and R each of rank 2-4, 1-2 contracted indices, extents
+    uniform in lo..hi, labels shuffled in every operand.  Returns (pattern, ddims, ldims, rdims).
+    With every rank at most 4 an operand holds at most hi^4 elements (331,776 for hi = 24)."""
+    while True:
+        rl, rr = rng.randint(2, 4), rng.randint(2, 4)
+        nc = rng.randint(1, min(2, rl, rr))
+        nl, nr = rl - nc, rr - nc
+        if not 2 <= nl + nr <= 4:
+            continue
+        labels = [chr(ord("a") + i) for i in range(nl + nr + nc)]
+        ext = {x: rng.randint(lo, hi) for x in labels}
+        lu, ru, cc = labels[:nl], labels[nl:nl + nr], labels[nl + nr:]
+        d, l, r = lu + ru, lu + cc, ru + cc
+        rng.shuffle(d)
+        rng.shuffle(l)
+        rng.shuffle(r)
+        pat = "D(%s)+=L(%s)*R(%s)" % (",".join(d), ",".join(l), ",".join(r))
+        return pat, [ext[x] for x in d], [ext[x] for x in l], [ext[x] for x in r]
+
+
 def einsum_contract(pat, dd, D, ld, L, rd, R, alpha=1.0, accumulative=1, conj_l=False, conj_r=False):
     """numpy einsum of a TAL-SH pattern over Fortran-ordered (dlf) views; returns flat D."""
     lhs, rhs = pat.split("+=")

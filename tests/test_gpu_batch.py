@@ -10,7 +10,7 @@ import random
 import numpy as np
 import pytest
 
-from tal_common import R4, R8, tensor
+from tal_common import R4, R8, batch_contraction, tensor
 
 pytestmark = pytest.mark.gpu
 
@@ -22,28 +22,6 @@ def say(*a):
 def relerr(got, want):
     den = np.linalg.norm(want.astype(np.float64))
     return np.linalg.norm((got - want).astype(np.float64)) / (den if den > 0 else 1.0)
-
-
-def batch_contraction(rng, lo=2, hi=24):
-    """ranks 2-4 per operand, 1-2 contracted indices, extents uniform in lo..hi."""
-    while True:
-        rl, rr = rng.randint(2, 4), rng.randint(2, 4)
-        nc = rng.randint(1, min(2, rl, rr))
-        nl, nr = rl - nc, rr - nc
-        labels = [chr(ord("a") + i) for i in range(nl + nr + nc)]
-        ext = {x: rng.randint(lo, hi) for x in labels}
-        lu, ru, cc = labels[:nl], labels[nl:nl + nr], labels[nl + nr:]
-        d = lu + ru
-        rng.shuffle(d)
-        l = lu + cc
-        rng.shuffle(l)
-        r = ru + cc
-        rng.shuffle(r)
-        vol = lambda s: int(np.prod([ext[x] for x in s])) if s else 1
-        if max(vol(d), vol(l), vol(r)) > 400000:
-            continue
-        pat = "D(%s)+=L(%s)*R(%s)" % (",".join(d), ",".join(l), ",".join(r))
-        return pat, [ext[x] for x in d], [ext[x] for x in l], [ext[x] for x in r]
 
 
 def upload(ts, dk, dims, data):

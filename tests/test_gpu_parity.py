@@ -533,3 +533,28 @@ def test_optimal_device_and_buffer_base(talsh_gpu):
     # nothing resident on a GPU: operations still go to a GPU on this build
     assert lib.talshDetermineOptimalDevice(h.ptr, h.ptr, h.ptr) == ts.gpu_flat(0)
     assert lib.talshDeviceBufferBasePtr(0, ts.DEV_NVIDIA_GPU)
+
+
+def test_reference_copy_add_round_trip_c4(talsh_gpu, oracle):
+    """The reference's one asserted numeric check (test.cpp:484-568, without its SVD step, which
+    is out of scope): btens C4 5x10x15x20x25 with b[i] = (i*1e-4, -i*1e-5) normalised by its
+    1-norm; dtens = btens reversed (D(a,b,c,d,e)=B(e,d,c,b,a)); btens += -1 * dtens reversed back
+    (B(a,b,c,d,e)+=D(e,d,c,b,a)); the reference requires inf-norm(btens) <= 1e-6.  Operands live
+    on the Host and the operations run on the GPU, as there.  The copy is also bit-exact against
+    the oracle, so the round trip is exactly zero."""
+    ts = talsh_gpu
+    bd, dd = [5, 10, 15, 20, 25], [25, 20, 15, 10, 5]
+    vol = int(np.prod(bd))
+    i = np.arange(vol, dtype=np.float32)
+    b = (i * np.float32(1e-4) - 1j * (i * np.float32(1e-5))).astype(np.complex64)
+    norm1 = float(np.sum(np.abs(b.astype(np.complex128))))
+    b = (b / np.float32(norm1)).astype(np.complex64)
+    bt = ts.Tensor(C4, bd, 0, data=b)
+    dt = ts.Tensor(C4, dd, 0)
+    say("round trip C4", bd)
+    ts.check(ts.copy("D(a,b,c,d,e)=B(e,d,c,b,a)", dt, bt, dev_id=0, dev_kind=ts.DEV_NVIDIA_GPU), "copy")
+    ts.check(ts.add("B(a,b,c,d,e)+=D(e,d,c,b,a)", bt, dt, alpha=-1.0, dev_id=0, dev_kind=ts.DEV_NVIDIA_GPU), "add")
+    assert dt.host_array().tobytes() == oracle.copy(C4, bd, [5, 4, 3, 2, 1], b).tobytes()
+    got = bt.host_array()
+    assert np.max(np.abs(got)) <= 1e-6
+    assert not np.any(got)
