@@ -21,7 +21,8 @@ their task is waited on, and the rank-0 / outer-product contractions that the
 reference runs with dedicated kernels (tensor_algebra_gpu_nvidia.cu:6732-6849).
 
 Bars: copies and initialisations bit-exact; R8/C8 contractions normwise
-relative error <= 1e-12, R4/C4 <= 1e-5, against the oracle.
+relative error <= 1e-12 against the oracle; R4/C4 contractions elementwise
+within 1e-6 * (|D| + |alpha| * sum |l| |r|) of it (dot products cancel).
 """
 import ctypes
 import gc
@@ -43,10 +44,6 @@ def say(*a):
 def relerr(got, want):
     den = np.linalg.norm(want.astype(np.complex128))
     return np.linalg.norm((got - want).astype(np.complex128)) / (den if den > 0 else 1.0)
-
-
-def tol(dk):
-    return 1e-12 if dk in (R8, C8) else 1e-5
 
 
 def labels(n):
@@ -389,6 +386,15 @@ def test_rank0_operands_and_outer_products(talsh_gpu, oracle, dk):
             ts.check(ts.contract(pat, dt, lt, rt, alpha=complex(*alpha), accumulative=acc), "contract")
             want = oracle.contract(dk, pat, dd, D, ld, L, rd, R, alpha, acc)
             got = snapshot(ts, dt)
-            assert relerr(got, want) <= tol(dk), (pat, acc)
+            if dk in (R8, C8):
+                assert relerr(got, want) <= 1e-12, (pat, acc)
+            else:
+                # FP32 arithmetic: elementwise against |D| + |alpha| * sum |l| |r| (a dot product
+                # can cancel far below its terms, so a normwise bound would not hold)
+                plain = pat.replace("+(", "(")
+                mag = oracle.contract(R8, plain, dd, np.zeros(max(1, int(np.prod(dd)))), ld,
+                                      np.abs(L).astype(np.float64), rd, np.abs(R).astype(np.float64))
+                bound = abs(complex(*alpha)) * mag + (np.abs(D) if acc else 0.0)
+                assert np.all(np.abs(got - want) <= 1e-6 * bound), (pat, acc)
             for x in (lt, rt, dt):
                 x.destruct()
