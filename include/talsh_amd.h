@@ -123,6 +123,24 @@ int talshAmdServerPending(const talshAmdServer *server, int *queued, int *inflig
 int talshAmdServerStats(const talshAmdServer *server, double *out, int nout);
 int talshAmdServerDestroy(talshAmdServer *server);
 
+/* Allocation-free norm2.  No counterpart in the reference: this is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same kernels and result
+ * as talshTensorNorm2, on raw device pointers: *result (a device address
+ * holding one element of data_kind, like talshTensorNorm2's rank-0 output)
+ * = sum of |x|^2 over the `volume` elements of kind data_kind (R4, R8, C4,
+ * C8) at device address `body` on GPU `gpu`, enqueued on `stream` (a
+ * hipStream_t of that GPU; NULL = its null stream).  The reduction's partial
+ * sums live in a fixed per-GPU device array of 8 KiB (1024 doubles, set up by
+ * talshInit; enough for any volume); no pool buffer, task, event or device
+ * allocation is made.  The caller orders the call against other work on
+ * `body` and `result` (no hazard tracking).  Calls that overlap on one GPU
+ * share the array and race; serialise them (one stream, or events).
+ * Returns TALSH_NOT_INITIALIZED before talshInit, TALSH_INVALID_ARGS for a
+ * bad kind, GPU, pointer or volume 0. */
+int talshAmdTensorNorm2NoAlloc(int data_kind, const void *body, size_t volume, void *result, int gpu,
+                               void *stream);
+
 /* Copies a short build description into buf. */
 int talshAmdBuildInfo(char *buf, int len);
 

@@ -170,6 +170,7 @@ SIGNATURES = {
     "talshAmdGetFastMathSplit": (_I, []),
     "talshAmdDeviceSynchronize": (_I, [_I]),
     "talshAmdBuildInfo": (_I, [ctypes.c_char_p, _I]),
+    "talshAmdTensorNorm2NoAlloc": (_I, [_I, _P, _S, _P, _I, _P]),
     "talshAmdBatchCreate": (_I, [ctypes.POINTER(_P)]),
     "talshAmdBatchAdd": (_I, [_P, ctypes.c_char_p, _TP, _TP, _TP, _D, _D, _I]),
     "talshAmdBatchSize": (_I, [_P]),

@@ -55,6 +55,15 @@ int launch_scale(int dk, void *d, size_t vol, double ar, double ai, hipStream_t 
 // deterministic reduction; `partials` is device scratch of norm2_scratch_elems() doubles.
 size_t norm2_scratch_elems();
 int launch_norm2(int dk, const void *src, size_t vol, double *partials, void *out, hipStream_t s);
+// The same reduction with its partials in a static per-GPU device array
+// instead of caller scratch.  noalloc_scratch_setup(hip_dev), with that device
+// current, looks up the array's address on it (once per device, at init);
+// launch_norm2_noalloc returns -3 on a device not set up.  Calls that overlap
+// on one GPU share the array and race.
+constexpr int kNoAllocMaxDevices = 64;
+int noalloc_scratch_setup(int hip_dev);
+size_t norm2_noalloc_scratch_bytes();
+int launch_norm2_noalloc(int hip_dev, int dk, const void *src, size_t vol, void *out, hipStream_t s);
 
 // GEMM precision modes for real single precision.
 enum R4Mode { R4_FP32 = 0, R4_SPLIT_BF16 = 1 };

@@ -990,6 +990,27 @@ int talshAmdDeviceSynchronize(int gpu) {
   (void)hipSetDevice(rt().gpus[g].hip_id);
   return hipDeviceSynchronize() == hipSuccess ? TALSH_SUCCESS : TALSH_FAILURE;
 }
+// talshTensorNorm2's kernels on raw device pointers with the per-GPU static
+// scratch: no pool buffer, no task, no event, no allocation of any kind.
+int talshAmdTensorNorm2NoAlloc(int data_kind, const void *body, size_t volume, void *result, int gpu,
+                               void *stream) {
+  std::lock_guard<std::recursive_mutex> lk(rt().mu);
+  if (!rt().on) return TALSH_NOT_INITIALIZED;
+  const int g = gpu_index(flat_gpu(gpu));
+  if (g < 0 || !rt().gpus[g].on) return TALSH_INVALID_ARGS;
+  if (data_kind != R4 && data_kind != R8 && data_kind != C4 && data_kind != C8) return TALSH_INVALID_ARGS;
+  if (!body || !result || volume == 0) return TALSH_INVALID_ARGS;
+  // the reduction's partials never exceed the fixed scratch (one double per stage-1 block)
+  if (norm2_scratch_elems() * sizeof(double) > norm2_noalloc_scratch_bytes()) return TALSH_INVALID_ARGS;
+  int prev = -1;
+  if (hipGetDevice(&prev) != hipSuccess) return TALSH_FAILURE;
+  const int dev = rt().gpus[g].hip_id;
+  if (prev != dev && hipSetDevice(dev) != hipSuccess) return TALSH_FAILURE;
+  const int e = launch_norm2_noalloc(dev, data_kind, body, volume, result, (hipStream_t)stream);
+  if (prev != dev) (void)hipSetDevice(prev);
+  return e == 0 ? TALSH_SUCCESS : TALSH_FAILURE;
+}
+
 int talshAmdBuildInfo(char *buf, int len) {
   const char *s = "libtalsh_amd gfx950 MFMA f64/f32 TTGT";
   if (!buf || len <= 0) return TALSH_INVALID_ARGS;

@@ -748,6 +748,31 @@ int launch_scale(int dk, void *d, size_t vol, double ar, double ai, hipStream_t 
 
 size_t norm2_scratch_elems() { return kNormBlocks; }
 
+// Fixed scratch of the allocation-free norm2 (stage-1 partial sums, one double
+// per block; stage 1 writes every partial that stage 2 reads, so nothing needs
+// clearing between calls).  Each GPU holds its own copy of the array for the
+// life of the process; host code reaches it only through the address that
+// hipGetSymbolAddress returns on that GPU.
+static __device__ __align__(256) double scratch[kNormBlocks];
+static double *g_scratch[kNoAllocMaxDevices] = {};
+
+int noalloc_scratch_setup(int hip_dev) {
+  if (hip_dev < 0 || hip_dev >= kNoAllocMaxDevices) return -1;
+  void *p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return -2;
+  g_scratch[hip_dev] = (double *)p;
+  return 0;
+}
+
+size_t norm2_noalloc_scratch_bytes() { return sizeof(scratch); }
+
+int launch_norm2_noalloc(int hip_dev, int dk, const void *src, size_t vol, void *out, hipStream_t s) {
+  if (hip_dev < 0 || hip_dev >= kNoAllocMaxDevices || !g_scratch[hip_dev]) return -3;
+  // the stage-1 grid never exceeds kNormBlocks partials, the array's length
+  static_assert(sizeof(scratch) == kNormBlocks * sizeof(double), "one partial per stage-1 block");
+  return launch_norm2(dk, src, vol, g_scratch[hip_dev], out, s);
+}
+
 KernelClock &kernel_clock() {
   static thread_local KernelClock c;
   return c;

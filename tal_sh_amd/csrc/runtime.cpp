@@ -440,6 +440,8 @@ int talshInit(size_t *host_buf_size, int *host_arg_max, int ngpus, int gpu_list[
     Device &d = r.gpus[id];
     if (d.on) continue;
     if (hipSetDevice(id) != hipSuccess) return TALSH_FAILURE;
+    // address of this GPU's copy of the allocation-free entry points' scratch
+    if (noalloc_scratch_setup(id) != 0) return TALSH_FAILURE;
     size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) != hipSuccess) return TALSH_FAILURE;
     d.hip_id = id;
