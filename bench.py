@@ -349,9 +349,10 @@ def batch_contraction(rng):
     return gen(rng)
 
 
-def bench_batch(oracle, dk, ntask=4096, reps=5, check_sample=128):
+def bench_batch(oracle, dk, ntask=4096, reps=5, check_sample=128, pertask=True):
     """Config 4: 4096 seeded small contractions; one persistent launch (graph replay)
-    versus one talshTensorContract per task (each its own transpose+GEMM launches)."""
+    versus one talshTensorContract per task (each its own transpose+GEMM launches;
+    pertask=False skips that comparison, e.g. under a counter profiler)."""
     import random
 
     rng = random.Random(4096 + dk)
@@ -397,17 +398,19 @@ def bench_batch(oracle, dk, ntask=4096, reps=5, check_sample=128):
         worst = max(worst, float(np.linalg.norm((got - want).astype(np.float64)) / den))
         dt.place(ts.gpu_flat(0))
     # one launch sequence per task: talshTensorContract, all queued, then waited
-    pertask = []
-    t0 = time.perf_counter()
-    for (pat, dd, ld, rd, L, R, lt, rt, dt) in tasks:
-        tk = ts.Task()
-        ts.check(ts.contract(pat, dt, lt, rt, alpha=1.0, accumulative=0, task=tk), "contract")
-        pertask.append(tk)
-    for tk in pertask:
-        tk.wait()
-        tk.destruct()
-    lib.talshAmdDeviceSynchronize(0)
-    single_s = time.perf_counter() - t0
+    single_s = float("nan")
+    if pertask:
+        queued = []
+        t0 = time.perf_counter()
+        for (pat, dd, ld, rd, L, R, lt, rt, dt) in tasks:
+            tk = ts.Task()
+            ts.check(ts.contract(pat, dt, lt, rt, alpha=1.0, accumulative=0, task=tk), "contract")
+            queued.append(tk)
+        for tk in queued:
+            tk.wait()
+            tk.destruct()
+        lib.talshAmdDeviceSynchronize(0)
+        single_s = time.perf_counter() - t0
     b.destroy()
     for (pat, dd, ld, rd, L, R, lt, rt, dt) in tasks:
         for t in (lt, rt, dt):
@@ -756,6 +759,8 @@ def main():
     ap.add_argument("--only", default="",
                     help="comma list of parts to run "
                          "(head,r4,bf16,fp16x3,c8,config1,transpose,shapes,batch,hyper,xl,serve)")
+    ap.add_argument("--batch-no-pertask", action="store_true",
+                    help="batched config 4 without the one-launch-per-task comparison (for counter profiles)")
     ap.add_argument("--serve-requests", type=int, default=20000, help="serving stream length")
     ap.add_argument("--serve-rate", type=float, default=None, help="arrivals per second (default 0.8 x capacity)")
     ap.add_argument("--serve-streams", type=int, default=16, help="streams per GPU for the serving run")
@@ -797,8 +802,8 @@ def main():
     if want("shapes"):
         secondary["reference_shapes_r8"] = bench_reference_shapes()
     if want("batch"):
-        secondary["batched_config4_r8"] = bench_batch(oracle, R8)
-        secondary["batched_config4_r4"] = bench_batch(oracle, R4)
+        secondary["batched_config4_r8"] = bench_batch(oracle, R8, pertask=not args.batch_no_pertask)
+        secondary["batched_config4_r4"] = bench_batch(oracle, R4, pertask=not args.batch_no_pertask)
     if want("hyper"):
         secondary["hyper_r8_batched"] = bench_hyper(oracle)
     if want("xl"):

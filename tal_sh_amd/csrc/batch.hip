@@ -1,23 +1,22 @@
 // Persistent batched-contraction kernel (gfx950).
 //
 // Thousands of small contractions of uneven shape run in one launch.  Each
-// contraction is described on the device by a BatchTask: operand pointers,
-// the matrix sizes M (uncontracted L dims), N (uncontracted R dims), K
-// (contracted dims), a tile shape chosen by the host autotuner, and offset
-// tables that map a row / column / contracted index to an element offset in
-// each tensor (lM, lK, rK, rN, dM, dN).  With those tables the operands are
-// gathered straight from their dlf storage into LDS tiles, so no operand is
-// transposed: D(m,n) (+)= alpha * sum_k L[lM[m]+lK[k]] * R[rK[k]+rN[n]] is
-// written to D[dM[m]+dN[n]].  The reference runs such tasks one TTGT schedule
-// (up to 4 transposes + a GEMM launch) at a time (tensor_algebra_gpu_nvidia.cu:
-// 5847-7188).
+// wave is an independent worker: the host planner gives it a fixed list of
+// work items (wave tiles of one contraction, self-contained 256-byte
+// descriptors, batch_item.hpp) and the wave computes them one after the
+// other with no barrier, no LDS and no atomic.  Operands are read straight
+// from their dlf storage into the 16x16x4 MFMA operand registers: a row,
+// column or contracted index becomes a byte offset by mixed-radix digits
+// (multiply-high division), and buffer loads with the operand's size as
+// their range return 0 for every index past an edge, so no operand is
+// transposed and no load is masked.  Contracted indices are loaded kDepth
+// steps of 4 ahead of the MFMAs that consume them.  The reference runs such
+// tasks one TTGT schedule (up to 4 transposes + a GEMM launch) at a time
+// (tensor_algebra_gpu_nvidia.cu:5847-7188).
 //
-// Work distribution: every (task, tile) pair is a work item; a fixed grid of
-// workgroups pulls items from a global counter until the counter passes the
-// total, which every workgroup reaches, so the grid always drains.  The
-// counter is cleared by a one-thread kernel in the same stream (also inside
-// a captured graph).  Each D element is produced by exactly one work item, so
-// results are deterministic and independent of the batch composition.
+// Each D element is produced by exactly one item, in a fixed order of the
+// contracted indices, so results are deterministic and independent of the
+// batch composition and of the schedule.
 #include "batch.hpp"
 
 #include <hip/hip_runtime.h>
@@ -28,222 +27,156 @@ namespace {
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
-template <typename S>
-struct BMfma;
-template <>
-struct BMfma<double> {
-  using acc_t = f64x4;
-  __device__ static int row(int lane, int r) { return 4 * r + (lane >> 4); }
-  __device__ static acc_t op(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
-};
-template <>
-struct BMfma<float> {
-  using acc_t = f32x4;
-  __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
-  __device__ static acc_t op(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
-};
-
-constexpr int kThreads = 256;
-constexpr int kBK = 16;
-constexpr int kMaxTile = 64;
-
-__global__ void batch_reset_kernel(unsigned long long *counter) { *counter = 0ull; }
-
-// One work item = one tm x tn tile of one task's D.  Element i of a thread
-// in the A tile is (row am(i), step column ak(i)): along m (AM = 0: one row
-// tid % tm, k = tid / tm + i * 256 / tm) when an uncontracted index is the
-// contiguous one of L, along k (AM = 1: k = tid % 16, rows tid / 16 + 16 i)
-// when a contracted index is, so consecutive lanes read consecutive
-// addresses either way; likewise for B with n (BM).  At the start of a tile
-// the workgroup copies the tile's row and column offsets and, when K <= kKC
-// (KL = 1), the task's contracted-index offsets from the task tables into LDS
-// with one load per thread and entry, all in flight at once; the K loop and
-// the epilogue then read offsets from LDS, so a data gather waits for one
-// global load, not two dependent ones.  The next K step is gathered into registers while the
-// current one is multiplied from LDS (double-buffered, one barrier per step).
-constexpr int kKC = 512;  // tasks with K <= kKC keep their contracted-index offsets in LDS
-template <typename S>
-struct TileSmem {
-  S A[2][kMaxTile][kBK + 1];
-  S B[2][kMaxTile][kBK + 1];
-  int lm[kMaxTile], rn[kMaxTile], dm[kMaxTile], dn[kMaxTile];
-  int lk[kKC], rk[kKC];
-};
-
-template <typename S, int AM, int BM, int KL>
-__device__ __forceinline__ void tile_body(const BatchTask &T, long long local, const int *__restrict__ tabs,
-                                          TileSmem<S> &sm, int tid) {
-  using MF = BMfma<S>;
-  using acc_t = typename MF::acc_t;
-  constexpr int NR = kMaxTile * kBK / kThreads;  // elements per thread and operand (tile 64 x 16)
-  const int lane = tid & 63, wave = tid >> 6;
-  const int tm = T.tm, tn = T.tn, K = T.K, M = T.M, N = T.N;
-  const int m0 = (int)(local % T.tiles_m) * tm, n0 = (int)(local / T.tiles_m) * tn;
-  const int *lM = tabs + T.tab, *lK = lM + M, *rK = lK + K, *rN = rK + K, *dM = rN + N, *dN = dM + M;
-  const S *L = (const S *)T.L, *R = (const S *)T.R;
-  const int na = tm * kBK / kThreads, nb = tn * kBK / kThreads;
-  const int tml = __builtin_ctz(tm), tnl = __builtin_ctz(tn);  // tm, tn are powers of two
-  // row / column offsets of the tile (-1 past the edge); the previous tile's
-  // epilogue read these arrays before the barrier that ended its item
-  if (tid < kMaxTile) {
-    const int m = m0 + tid, n = n0 + tid;
-    sm.lm[tid] = (tid < tm && m < M) ? lM[m] : -1;
-    sm.dm[tid] = (tid < tm && m < M) ? dM[m] : 0;
-    sm.rn[tid] = (tid < tn && n < N) ? rN[n] : -1;
-    sm.dn[tid] = (tid < tn && n < N) ? dN[n] : 0;
-  }
-  auto am = [&](int i) { return AM ? (tid >> 4) + kBK * i : tid & (tm - 1); };
-  auto ak = [&](int i) { return AM ? tid & (kBK - 1) : (tid >> tml) + i * (kThreads >> tml); };
-  auto bn = [&](int i) { return BM ? (tid >> 4) + kBK * i : tid & (tn - 1); };
-  auto bk = [&](int i) { return BM ? tid & (kBK - 1) : (tid >> tnl) + i * (kThreads >> tnl); };
-  const int subm = tm >> 4, nsub = subm * (tn >> 4);
-  acc_t acc[4];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) acc[q] = acc_t{0, 0, 0, 0};
-  // two register sets: step s is gathered into set s & 1 two steps ahead
-  S ra0[NR], rb0[NR], ra1[NR], rb1[NR];
-  int aoff[NR], boff[NR];
-  const int *lk = KL ? sm.lk : lK, *rk = KL ? sm.rk : rK;
-#define TAL_GATHER(RA, RB, k0)                                             \
-  _Pragma("unroll") for (int i = 0; i < NR; ++i) {                         \
-    const int ka = (k0) + ak(i), kb = (k0) + bk(i);                        \
-    RA[i] = (aoff[i] >= 0 && ka < K) ? L[aoff[i] + lk[ka]] : (S)0;         \
-    RB[i] = (boff[i] >= 0 && kb < K) ? R[rk[kb] + boff[i]] : (S)0;         \
-  }
-#define TAL_STAGE(RA, RB, buf)                                             \
-  _Pragma("unroll") for (int i = 0; i < NR; ++i) {                         \
-    if (i < na) sm.A[buf][am(i)][ak(i)] = RA[i];                           \
-    if (i < nb) sm.B[buf][bn(i)][bk(i)] = RB[i];                           \
-  }
-  constexpr int UK = sizeof(S) == 8 ? 1 : 2;  // fewer live fragments for FP64 (register budget)
-#define TAL_COMPUTE(buf)                                                                                   \
-  _Pragma("unroll UK") for (int ks = 0; ks < kBK / 4; ++ks) {                                              \
-    const int kk = ks * 4 + (lane >> 4);                                                                   \
-    _Pragma("unroll") for (int q = 0; q < 4; ++q) {                                                        \
-      const int sidx = wave + 4 * q;                                                                       \
-      if (sidx < nsub) {                                                                                   \
-        const int si = sidx % subm, sj = sidx / subm;                                                      \
-        acc[q] = MF::op(sm.A[buf][si * 16 + (lane & 15)][kk], sm.B[buf][sj * 16 + (lane & 15)][kk], acc[q]); \
-      }                                                                                                    \
-    }                                                                                                      \
-  }
-  if (KL)  // K <= kKC: the contracted-index offsets of the whole task in LDS
-    for (int k = tid; k < K; k += kThreads) {
-      sm.lk[k] = lK[k];
-      sm.rk[k] = rK[k];
-    }
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < NR; ++i) {
-    aoff[i] = i < na ? sm.lm[am(i)] : -1;
-    boff[i] = i < nb ? sm.rn[bn(i)] : -1;
-  }
-  const int nk = (K + kBK - 1) / kBK;
-  TAL_GATHER(ra0, rb0, 0)
-  TAL_STAGE(ra0, rb0, 0)
-  __syncthreads();
-  if (nk > 1) {
-    TAL_GATHER(ra1, rb1, kBK)
-  }
-  if (nk > 2) {
-    TAL_GATHER(ra0, rb0, 2 * kBK)
-  }
-  for (int kt = 0; kt < nk; kt += 2) {
-    // even step kt from buffer 0; step kt+1 (set 1) goes to buffer 1, set 1 refills with kt+3
-    TAL_COMPUTE(0)
-    if (kt + 1 < nk) {
-      TAL_STAGE(ra1, rb1, 1)
-    }
-    __syncthreads();
-    if (kt + 3 < nk) {
-      TAL_GATHER(ra1, rb1, (kt + 3) * kBK)
-    }
-    if (kt + 1 >= nk) break;
-    // odd step kt+1 from buffer 1; step kt+2 (set 0) goes to buffer 0, set 0 refills with kt+4
-    TAL_COMPUTE(1)
-    if (kt + 2 < nk) {
-      TAL_STAGE(ra0, rb0, 0)
-    }
-    __syncthreads();
-    if (kt + 4 < nk) {
-      TAL_GATHER(ra0, rb0, (kt + 4) * kBK)
-    }
-  }
-#undef TAL_COMPUTE
-#undef TAL_GATHER
-#undef TAL_STAGE
-  S *D = (S *)T.D;
-  const S alpha = (S)T.alpha;
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int sidx = wave + 4 * q;
-    if (sidx >= nsub) continue;
-    const int si = sidx % subm, sj = sidx / subm;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int mi = si * 16 + MF::row(lane, r), ni = sj * 16 + (lane & 15);
-      if (m0 + mi < M && n0 + ni < N) {
-        S *d = D + sm.dm[mi] + sm.dn[ni];
-        const S v = alpha * acc[q][r];
-        *d = T.beta_one ? *d + v : v;
-      }
-    }
-  }
+// Raw buffer resource (gfx9 format word) over `bytes` bytes at p: the range
+// check drops every access at or past `bytes`.
+__device__ inline __amdgpu_buffer_rsrc_t rsrc(uint64_t p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)bytes, 0x00020000);
 }
 
-// The index of the next work item is fetched (one atomic by thread 0) while
-// the current tile is computed: the atomic's result stays in a register until
-// the tile is done (the tile's barriers wait for LDS traffic only) and is then
-// published through a two-slot LDS buffer, so neither the atomic's round trip
-// nor a task search sits between tiles.
 template <typename S>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void batch_contract_kernel(
-    const BatchTask *__restrict__ tasks, const long long *__restrict__ prefix, const int *__restrict__ item_task,
-    const int *__restrict__ tabs, unsigned long long *counter, long long total) {
-  __shared__ TileSmem<S> sm;
-  __shared__ long long s_item[2];
-  const int tid = threadIdx.x;
-  if (tid == 0) s_item[0] = (long long)atomicAdd(counter, 1ull);
-  __syncthreads();
-  long long item = s_item[0];
-  for (int it = 1; item < total; ++it) {
-    unsigned long long next = 0;
-    if (tid == 0) next = atomicAdd(counter, 1ull);
-    const int t = item_task[item];
-    const BatchTask &T = tasks[t];
-    const long long local = item - prefix[t];
-    switch (T.amode * 2 + T.bmode + (T.K <= kKC ? 4 : 0)) {  // uniform over the block
-      case 0: tile_body<S, 0, 0, 0>(T, local, tabs, sm, tid); break;
-      case 1: tile_body<S, 0, 1, 0>(T, local, tabs, sm, tid); break;
-      case 2: tile_body<S, 1, 0, 0>(T, local, tabs, sm, tid); break;
-      case 3: tile_body<S, 1, 1, 0>(T, local, tabs, sm, tid); break;
-      case 4: tile_body<S, 0, 0, 1>(T, local, tabs, sm, tid); break;
-      case 5: tile_body<S, 0, 1, 1>(T, local, tabs, sm, tid); break;
-      case 6: tile_body<S, 1, 0, 1>(T, local, tabs, sm, tid); break;
-      default: tile_body<S, 1, 1, 1>(T, local, tabs, sm, tid); break;
+struct Wave;
+template <>
+struct Wave<double> {
+  using acc_t = f64x4;
+  static constexpr int kDepth = kBatchDepthF64;
+  __device__ static double load(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+  }
+  __device__ static void store(__amdgpu_buffer_rsrc_t r, uint32_t off, double v) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)off, 0, 0);
+  }
+  __device__ static acc_t mfma(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+};
+template <>
+struct Wave<float> {
+  using acc_t = f32x4;
+  static constexpr int kDepth = kBatchDepthF32;
+  __device__ static float load(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+  }
+  __device__ static void store(__amdgpu_buffer_rsrc_t r, uint32_t off, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, (int)off, 0, 0);
+  }
+  __device__ static acc_t mfma(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+};
+
+constexpr int kWaves = 4;  // waves per workgroup (independent workers)
+
+// `nx` is the wave's next descriptor (this one for its last item): one
+// scalar load per 64-byte line of it is issued once this item's own
+// descriptor fields are in registers, and consumed after the item, so the
+// next item starts from the scalar cache instead of memory.
+template <typename S, int BI, int BJ>
+__device__ __forceinline__ void item_body(const BatchItem &it, int lane, const BatchItem &nx) {
+  using W = Wave<S>;
+  using acc_t = typename W::acc_t;
+  constexpr int P = W::kDepth;
+  const auto rl = rsrc(it.L, it.lbytes), rr = rsrc(it.R, it.rbytes), rd = rsrc(it.D, it.dbytes);
+  uint32_t arow[BI], bcol[BJ], dcol[BJ];
+#pragma unroll
+  for (int i = 0; i < BI; ++i) arow[i] = a_row_offset(it, i, lane);
+#pragma unroll
+  for (int j = 0; j < BJ; ++j) b_col_offsets(it, j, lane, &bcol[j], &dcol[j]);
+  acc_t acc[BI][BJ];
+#pragma unroll
+  for (int i = 0; i < BI; ++i)
+#pragma unroll
+    for (int j = 0; j < BJ; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+  S a[P][BI], b[P][BJ];
+  // loads of K step c into register set s; a step past K loads zeros (kOutside)
+  auto load = [&](int s, int c) {
+    uint32_t kl, kr;
+    k_offsets(it, c, lane, &kl, &kr);
+#pragma unroll
+    for (int i = 0; i < BI; ++i) a[s][i] = W::load(rl, arow[i] + kl);
+#pragma unroll
+    for (int j = 0; j < BJ; ++j) b[s][j] = W::load(rr, kr + bcol[j]);
+  };
+  const int nsteps = (it.K + 3) / 4;
+#pragma unroll
+  for (int s = 0; s < P; ++s) load(s, s);
+  const int pf0 = nx.M, pf1 = (int)nx.gm.e0, pf2 = nx.gn.s[0][0], pf3 = nx.gk.s[0][0];
+  for (int c0 = 0; c0 < nsteps; c0 += P) {
+#pragma unroll
+    for (int s = 0; s < P; ++s) {
+      if (c0 + s < nsteps) {  // uniform over the wave
+#pragma unroll
+        for (int i = 0; i < BI; ++i)
+#pragma unroll
+          for (int j = 0; j < BJ; ++j) acc[i][j] = W::mfma(a[s][i], b[s][j], acc[i][j]);
+      }
+      load(s, c0 + s + P);
     }
-    // slot it & 1 was last read before the barriers of the previous tile
-    if (tid == 0) s_item[it & 1] = (long long)next;
-    __syncthreads();  // the next index is visible; every thread leaves the loop together
-    item = s_item[it & 1];
+  }
+  const S alpha = (S)it.alpha;
+  const bool accumulate = item_accumulates(it);
+  uint32_t drow[BI][4];  // D offsets of the accumulator rows, computed before the stores
+#pragma unroll
+  for (int i = 0; i < BI; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) drow[i][r] = d_row_offset(it, i, acc_row((int)sizeof(S), lane, r));
+#pragma unroll
+  for (int i = 0; i < BI; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+#pragma unroll
+      for (int j = 0; j < BJ; ++j) {
+        const uint32_t off = drow[i][r] + dcol[j];
+        S v = alpha * acc[i][j][r];
+        if (accumulate) v = W::load(rd, off) + v;
+        W::store(rd, off, v);
+      }
+    }
+  asm volatile("" ::"s"(pf0), "s"(pf1), "s"(pf2), "s"(pf3));
+}
+
+// Workgroup b's wave v is worker w = b * kWaves + v and runs items
+// [start[w], start[w + 1]) of `items`; every worker's list is finite, so the
+// grid drains.
+template <typename S>
+__global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(4))) void batch_contract_kernel(const BatchItem *__restrict__ items,
+                                                                    const int *__restrict__ start, int nworkers) {
+  const int lane = threadIdx.x & 63;
+  const int w = (int)blockIdx.x * kWaves + __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  if (w >= nworkers) return;
+  const int p1 = start[w + 1];
+  for (int p = start[w]; p < p1; ++p) {
+    const BatchItem &it = items[p], &nx = items[p + 1 < p1 ? p + 1 : p];
+    switch (it.shape & 15) {  // (BI - 1) | (BJ - 1) << 2, uniform over the wave
+      case 0: item_body<S, 1, 1>(it, lane, nx); break;
+      case 1: item_body<S, 2, 1>(it, lane, nx); break;
+      case 4: item_body<S, 1, 2>(it, lane, nx); break;
+      case 5: item_body<S, 2, 2>(it, lane, nx); break;
+      case 3: item_body<S, 4, 1>(it, lane, nx); break;
+      case 12: item_body<S, 1, 4>(it, lane, nx); break;
+      default: break;  // not produced by the planner
+    }
   }
 }
 
 }  // namespace
 
-int batch_launch(int dk, const BatchTask *tasks, const long long *prefix, const int *item_task, const int *tabs,
-                 unsigned long long *counter, long long total, int grid, hipStream_t s) {
-  if (!tasks || !prefix || !item_task || !tabs || !counter || total < 0 || grid <= 0) return -1;
-  batch_reset_kernel<<<1, 1, 0, s>>>(counter);
-  if (total == 0) return hipGetLastError() == hipSuccess ? 0 : -2;
+int batch_workers_per_block() { return kWaves; }
+
+int batch_blocks_per_cu(int dk, int *blocks) {
+  int nb = 0;
+  hipError_t e = hipErrorInvalidValue;
+  if (dk == 8)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_contract_kernel<double>, kWaves * 64, 0);
+  else if (dk == 4)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_contract_kernel<float>, kWaves * 64, 0);
+  if (e != hipSuccess || nb <= 0) return -1;
+  *blocks = nb;
+  return 0;
+}
+
+int batch_launch(int dk, const BatchItem *items, const int *start, int nblocks, hipStream_t s) {
+  if (!items || !start || nblocks <= 0) return -1;
   switch (dk) {
-    case 8:
-      batch_contract_kernel<double><<<grid, kThreads, 0, s>>>(tasks, prefix, item_task, tabs, counter, total);
-      break;
-    case 4:
-      batch_contract_kernel<float><<<grid, kThreads, 0, s>>>(tasks, prefix, item_task, tabs, counter, total);
-      break;
+    case 8: batch_contract_kernel<double><<<nblocks, kWaves * 64, 0, s>>>(items, start, nblocks * kWaves); break;
+    case 4: batch_contract_kernel<float><<<nblocks, kWaves * 64, 0, s>>>(items, start, nblocks * kWaves); break;
     default: return -1;
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;

@@ -1,31 +1,21 @@
-// Batched contraction: device task descriptor and kernel launcher.
+// Batched contraction: kernel launcher (work items in batch_item.hpp).
 #pragma once
 
 #include <hip/hip_runtime.h>
 
-#include <cstdint>
+#include "batch_item.hpp"
 
 namespace tal {
 
-// One contraction of a batch as the persistent kernel sees it.  Tables at
-// tabs[tab ...]: lM[M] lK[K] rK[K] rN[N] dM[M] dN[N] (element offsets).
-struct BatchTask {
-  const void *L;
-  const void *R;
-  void *D;
-  int M, N, K;
-  int tm, tn;            // tile shape (multiples of 16, at most 64)
-  int tiles_m, tiles_n;
-  int beta_one;
-  int amode, bmode;      // thread layout of the A / B gathers: 0 along m / n, 1 along k
-  long long tab;
-  double alpha;
-};
+// Waves (independent workers) per workgroup of the persistent kernel.
+int batch_workers_per_block();
 
-// Clears the work counter and runs all tasks in one persistent launch of
-// `grid` workgroups on stream s (both launches are capturable).  Work item
-// `it` is tile it - prefix[t] of task t = item_task[it].
-int batch_launch(int dk, const BatchTask *tasks, const long long *prefix, const int *item_task, const int *tabs,
-                 unsigned long long *counter, long long total, int grid, hipStream_t s);
+// Workgroups of the persistent kernel resident on one CU at once (occupancy).
+int batch_blocks_per_cu(int dk, int *blocks);
+
+// Runs the items of a batch in one launch of `nblocks` workgroups on stream
+// s (capturable): worker w = block * batch_workers_per_block() + wave runs
+// items [start[w], start[w + 1]).  dk = 4 (R4) or 8 (R8).
+int batch_launch(int dk, const BatchItem *items, const int *start, int nblocks, hipStream_t s);
 
 }  // namespace tal

@@ -2,14 +2,16 @@
 //
 // A batch collects many talshTensorContract-style operations on GPU-resident
 // tensors and runs them as one persistent kernel launch captured in a
-// hipGraph.  The first execution plans the batch (index tables, tile shapes,
-// graph); later executions of an unchanged batch replay the graph.
+// hipGraph.  The first execution plans the batch (index groups, wave tiles,
+// a static schedule of the work items over the kernel's waves, graph); later
+// executions of an unchanged batch replay the graph.
 //
-// Tile shapes come from an autotuner: tasks are grouped into shape classes
-// by (log2 M, log2 N) and, for every class not tuned yet in this process,
-// each candidate tile shape is timed on a sample of the class's own tasks
-// (writing into scratch outputs so user data is untouched); the fastest
-// shape is cached per (data kind, class) for the rest of the process.
+// Tile shapes come from a two-stage autotuner.  Stage 1 groups contractions
+// into shape classes by (log2 M, log2 N) and, for every class not tuned yet
+// in this process, times each candidate wave tile on up to 32 of the class's
+// own contractions (writing into scratch outputs, so user data is untouched);
+// the fastest is cached per (data kind, class).  Stage 2 times whole-batch
+// tile policies on one launch of the entire batch and keeps the fastest.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -18,6 +20,7 @@
 #include <vector>
 
 #include "batch.hpp"
+#include "batch_plan.hpp"
 #include "plan.hpp"
 #include "runtime.hpp"
 #include "talsh_amd.h"
@@ -31,18 +34,15 @@ struct talshAmdBatch {
   };
   std::vector<Entry> entries;
   bool built = false;
-  int g = -1, dk = NO_TYPE, grid = 0, ntasks = 0;
-  long long total = 0;
+  int g = -1, dk = NO_TYPE, nblocks = 0, ntasks = 0;
+  long long total = 0;  // work items
   double flops = 0.0;
   std::vector<const void *> sig;
   std::vector<size_t> sig_bytes;  // body bytes of each sig entry
   void *dev = nullptr;
   int dev_handle = -1;
-  tal::BatchTask *d_tasks = nullptr;
-  long long *d_prefix = nullptr;
-  int *d_item_task = nullptr;
-  unsigned long long *d_counter = nullptr;
-  int *d_tabs = nullptr;
+  const tal::BatchItem *d_items = nullptr;
+  const int *d_start = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int tuned_classes = 0, replays = 0;
@@ -59,14 +59,10 @@ struct talshAmdBatch {
 namespace tal {
 namespace {
 
-struct TileShape {
-  int tm, tn;
-};
-const TileShape kCandidates[] = {{16, 16}, {16, 32}, {32, 16}, {32, 32}, {16, 64},
-                                 {64, 16}, {32, 64}, {64, 32}, {64, 64}};
+constexpr int kXcds = 8;  // workgroups are dispatched round-robin over the XCDs
 
-std::map<std::pair<int, int>, TileShape> &tuned_tiles() {
-  static std::map<std::pair<int, int>, TileShape> t;
+std::map<std::pair<int, int>, WaveTile> &tuned_tiles() {
+  static std::map<std::pair<int, int>, WaveTile> t;
   return t;
 }
 
@@ -75,49 +71,18 @@ int bucket(int x) {
   while ((1 << b) < x && b < 12) ++b;
   return b;
 }
-int shape_class(int dk, int M, int N) { return dk * 10000 + bucket(M) * 100 + bucket(N); }
+int shape_class(int M, int N) { return bucket(M) * 100 + bucket(N); }
 
 // Host description of one contraction after planning.
 struct HostTask {
-  BatchTask t;
-  std::vector<int> tab;  // lM lK rK rN dM dN
+  BatchTaskGeom geom;
+  uint64_t L = 0, R = 0, D = 0;
+  uint64_t lbytes = 0, rbytes = 0, dbytes = 0;
+  double alpha = 1.0;
+  bool acc = false;
   int cls = 0;
-  size_t dbytes = 0;
+  WaveTile tile{16, 16};
 };
-
-// Mixed-radix offsets: for every multi-index over `ext` (first fastest),
-// sum_k idx_k * stride_a[k] and sum_k idx_k * stride_b[k].
-void odometer(const std::vector<int> &ext, const std::vector<long long> &sa, const std::vector<long long> &sb,
-              std::vector<int> *oa, std::vector<int> *ob) {
-  long long n = 1;
-  for (int e : ext) n *= e;
-  std::vector<int> idx(ext.size(), 0);
-  long long a = 0, b = 0;
-  for (long long i = 0; i < n; ++i) {
-    oa->push_back((int)a);
-    if (ob) ob->push_back((int)b);
-    for (size_t k = 0; k < ext.size(); ++k) {
-      if (++idx[k] < ext[k]) {
-        a += sa[k];
-        b += sb[k];
-        break;
-      }
-      a -= sa[k] * (ext[k] - 1);
-      b -= sb[k] * (ext[k] - 1);
-      idx[k] = 0;
-    }
-  }
-}
-
-std::vector<long long> strides(const talsh_tens_t *t) {
-  std::vector<long long> s(t->shape_p->num_dim);
-  long long acc = 1;
-  for (int i = 0; i < t->shape_p->num_dim; ++i) {
-    s[i] = acc;
-    acc *= t->shape_p->dims[i];
-  }
-  return s;
-}
 
 void *gpu_body(talsh_tens_t *t, int g, int *dk) {
   const int i = image_find(t, flat_gpu(g), NO_TYPE, true);
@@ -126,142 +91,84 @@ void *gpu_body(talsh_tens_t *t, int g, int *dk) {
   return t->dev_rsc[i].gmem_p;
 }
 
-int plan_task(const talshAmdBatch::Entry &e, int g, HostTask *h) {
+int plan_task(const talshAmdBatch::Entry &e, int g, int *dk_out, HostTask *h) {
   DigitalPattern p;
   if (parse_pattern(e.pattern.c_str(), &p) != 0) return TALSH_INVALID_ARGS;
   const talsh_tens_t *d = e.d, *l = e.l, *r = e.r;
   if (p.drank != d->shape_p->num_dim || p.lrank != l->shape_p->num_dim || p.rrank != r->shape_p->num_dim)
     return TALSH_INVALID_ARGS;
-  if (shape_volume(d->shape_p) >= (1ull << 31) || shape_volume(l->shape_p) >= (1ull << 31) ||
-      shape_volume(r->shape_p) >= (1ull << 31))
-    return DEVICE_UNABLE;  // 32-bit offset tables: large contractions belong to the TTGT path
-  ContractPlan cp;  // validates the pattern against the shapes
-  const int pe = plan_contraction(p, d->shape_p->dims, l->shape_p->dims, r->shape_p->dims, e.acc, &cp);
-  if (pe) return TALSH_INVALID_ARGS;
-  if (cp.batch > 1) return DEVICE_UNABLE;  // hyper indices: the TTGT path batches them
-  const std::vector<long long> ls = strides(l), rs = strides(r), ds = strides(d);
-  std::vector<int> mext, next, kext;
-  std::vector<long long> ml, md, nr, nd, kl, kr;
-  for (int i = 0; i < p.lrank; ++i) {
-    const int c = p.dig[i];
-    if (c > 0) {
-      mext.push_back(l->shape_p->dims[i]);
-      ml.push_back(ls[i]);
-      md.push_back(ds[c - 1]);
-    } else {
-      kext.push_back(l->shape_p->dims[i]);
-      kl.push_back(ls[i]);
-      kr.push_back(rs[-c - 1]);
-    }
-  }
-  for (int j = 0; j < p.rrank; ++j) {
-    const int c = p.dig[p.lrank + j];
-    if (c > 0) {
-      next.push_back(r->shape_p->dims[j]);
-      nr.push_back(rs[j]);
-      nd.push_back(ds[c - 1]);
-    }
-  }
-  // Contracted indices run in L's storage order unless only R has one as its
-  // contiguous index: then in R's order, so the B gathers can run along k.
-  const bool l_kfast = p.lrank > 0 && p.dig[0] < 0, r_kfast = p.rrank > 0 && p.dig[p.lrank] < 0;
-  if (r_kfast && !l_kfast) {
-    std::vector<size_t> ord(kext.size());
-    for (size_t i = 0; i < ord.size(); ++i) ord[i] = i;
-    std::sort(ord.begin(), ord.end(), [&](size_t x, size_t y) { return kr[x] < kr[y]; });
-    std::vector<int> e2;
-    std::vector<long long> l2, r2;
-    for (size_t i : ord) {
-      e2.push_back(kext[i]);
-      l2.push_back(kl[i]);
-      r2.push_back(kr[i]);
-    }
-    kext.swap(e2);
-    kl.swap(l2);
-    kr.swap(r2);
-  }
-  std::vector<int> lM, dM, lK, rK, rN, dN;
-  odometer(mext, ml, md, &lM, &dM);
-  odometer(kext, kl, kr, &lK, &rK);
-  odometer(next, nr, nd, &rN, &dN);
-  // gather layouts: along k when k (not m / n) is unit-stride in the operand
-  auto unit = [](const std::vector<int> &v) { return v.size() >= 2 && v[1] - v[0] == 1; };
-  h->t.amode = (unit(lK) && !unit(lM)) ? 1 : 0;
-  h->t.bmode = (unit(rK) && !unit(rN)) ? 1 : 0;
   int dk[3];
-  h->t.D = gpu_body(e.d, g, &dk[0]);
-  h->t.L = gpu_body(e.l, g, &dk[1]);
-  h->t.R = gpu_body(e.r, g, &dk[2]);
-  if (!h->t.D || !h->t.L || !h->t.R) return TALSH_NOT_AVAILABLE;
+  void *D = gpu_body(e.d, g, &dk[0]), *L = gpu_body(e.l, g, &dk[1]), *R = gpu_body(e.r, g, &dk[2]);
+  if (!D || !L || !R) return TALSH_NOT_AVAILABLE;
   if (dk[0] != dk[1] || dk[0] != dk[2] || (dk[0] != R4 && dk[0] != R8)) return TALSH_NOT_IMPLEMENTED;
-  h->t.M = (int)lM.size();
-  h->t.K = (int)lK.size();
-  h->t.N = (int)rN.size();
-  h->t.alpha = e.ar;
-  h->t.beta_one = e.acc ? 1 : 0;
-  h->tab.reserve(2 * (h->t.M + h->t.N + h->t.K));
-  for (auto *v : {&lM, &lK, &rK, &rN, &dM, &dN}) h->tab.insert(h->tab.end(), v->begin(), v->end());
-  h->cls = shape_class(dk[0], h->t.M, h->t.N);
-  h->dbytes = shape_volume(d->shape_p) * (size_t)dk_size(dk[0]);
+  const int esz = dk_size(dk[0]);
+  h->dbytes = shape_volume(d->shape_p) * (size_t)esz;
+  h->lbytes = shape_volume(l->shape_p) * (size_t)esz;
+  h->rbytes = shape_volume(r->shape_p) * (size_t)esz;
+  // 32-bit buffer offsets: large contractions belong to the TTGT path
+  if (h->dbytes > kBatchMaxBytes || h->lbytes > kBatchMaxBytes || h->rbytes > kBatchMaxBytes) return DEVICE_UNABLE;
+  const int ge = batch_task_geometry(p, d->shape_p->dims, l->shape_p->dims, r->shape_p->dims, esz, &h->geom);
+  if (ge == 1) return TALSH_INVALID_ARGS;
+  if (ge != 0) return DEVICE_UNABLE;  // hyper index or more than three dims in a group
+  h->L = (uint64_t)L;
+  h->R = (uint64_t)R;
+  h->D = (uint64_t)D;
+  h->alpha = e.ar;
+  h->acc = e.acc != 0;
+  h->cls = shape_class(h->geom.M, h->geom.N);
+  *dk_out = dk[0];
   return TALSH_SUCCESS;
 }
 
-TileShape default_tile(int M, int N) {
-  auto fit = [](int x) { return x <= 16 ? 16 : x <= 32 ? 32 : 64; };
-  return TileShape{fit(M), fit(N)};
+// Smallest candidate that covers the tile in one item where possible.
+WaveTile fit_tile(int M, int N) {
+  if (M <= 16) return WaveTile{16, N <= 16 ? 16 : N <= 32 ? 32 : 64};
+  if (N <= 16) return WaveTile{M <= 32 ? 32 : 64, 16};
+  return WaveTile{32, 32};
 }
 
-void set_tile(HostTask &h, TileShape s) {
-  h.t.tm = s.tm;
-  h.t.tn = s.tn;
-  h.t.tiles_m = (h.t.M + s.tm - 1) / s.tm;
-  h.t.tiles_n = (h.t.N + s.tn - 1) / s.tn;
-}
-
-// Device image of a task list: [tasks][prefix][counter][item -> task][tables].
+// Device image of a schedule: [start (workers + 1 ints)][items, 256-byte aligned].
 struct DeviceBatch {
   void *base = nullptr;
   int handle = -1;
-  BatchTask *tasks = nullptr;
-  long long *prefix = nullptr;
-  unsigned long long *counter = nullptr;
-  int *item_task = nullptr;
-  int *tabs = nullptr;
+  const BatchItem *items = nullptr;
+  const int *start = nullptr;
   long long total = 0;
 };
 
-int upload(int g, std::vector<HostTask> &hts, DeviceBatch *db, hipStream_t s) {
-  const int n = (int)hts.size();
-  std::vector<BatchTask> tasks(n);
-  std::vector<long long> prefix(n + 1, 0);
-  std::vector<int> tabs;
-  for (int i = 0; i < n; ++i) {
-    hts[i].t.tab = (long long)tabs.size();
-    tabs.insert(tabs.end(), hts[i].tab.begin(), hts[i].tab.end());
-    tasks[i] = hts[i].t;
-    prefix[i + 1] = prefix[i] + (long long)tasks[i].tiles_m * tasks[i].tiles_n;
+// Workgroups of the persistent launch: as many as are resident at once.
+int persistent_blocks(int g, int dk) {
+  hipDeviceProp_t prop;
+  int per_cu = 0;
+  if (batch_blocks_per_cu(dk, &per_cu) != 0) per_cu = 2;
+  if (hipGetDeviceProperties(&prop, rt().gpus[g].hip_id) != hipSuccess) return 256 * per_cu;
+  return std::max(1, prop.multiProcessorCount) * per_cu;
+}
+
+// Items of `hts` (each with its tile; D redirected when `scratch_d` is set),
+// scheduled over `nblocks` workgroups and copied to the device.
+int upload(int g, int dk, const std::vector<HostTask> &hts, int nblocks, hipStream_t s, DeviceBatch *db) {
+  std::vector<BatchItem> items;
+  std::vector<int> task_of;
+  for (size_t t = 0; t < hts.size(); ++t) {
+    const HostTask &h = hts[t];
+    const size_t before = items.size();
+    batch_task_items(h.geom, h.L, h.R, h.D, h.lbytes, h.rbytes, h.dbytes, h.alpha, h.acc, h.tile, &items);
+    task_of.insert(task_of.end(), items.size() - before, (int)t);
   }
-  if (tabs.empty()) tabs.push_back(0);
-  std::vector<int> item_task((size_t)std::max<long long>(1, prefix[n]));
-  for (int i = 0; i < n; ++i)
-    for (long long it = prefix[i]; it < prefix[i + 1]; ++it) item_task[(size_t)it] = i;
-  const size_t bt = ((sizeof(BatchTask) * n + 255) / 256) * 256, bp = ((8 * (n + 1) + 255) / 256) * 256;
-  const size_t bi = ((4 * item_task.size() + 255) / 256) * 256;
-  const size_t bytes = bt + bp + 256 + bi + 4 * tabs.size();
-  int e = dev_alloc(g, bytes, &db->base, &db->handle);
-  if (e) return e;
+  std::vector<BatchItem> ordered;
+  std::vector<int> start;
+  batch_schedule(items, task_of, dk_size(dk), nblocks, batch_workers_per_block(), kXcds, &ordered, &start);
+  const size_t sb = ((4 * start.size() + 255) / 256) * 256;
+  const size_t bytes = sb + sizeof(BatchItem) * std::max<size_t>(1, ordered.size());
+  if (int e = dev_alloc(g, bytes, &db->base, &db->handle)) return e;
   char *b = (char *)db->base;
-  db->tasks = (BatchTask *)b;
-  db->prefix = (long long *)(b + bt);
-  db->counter = (unsigned long long *)(b + bt + bp);
-  db->item_task = (int *)(b + bt + bp + 256);
-  db->tabs = (int *)(b + bt + bp + 256 + bi);
-  db->total = prefix[n];
-  if (hipMemcpyAsync(db->tasks, tasks.data(), sizeof(BatchTask) * n, hipMemcpyHostToDevice, s) != hipSuccess ||
-      hipMemcpyAsync(db->prefix, prefix.data(), 8 * (n + 1), hipMemcpyHostToDevice, s) != hipSuccess ||
-      hipMemcpyAsync(db->item_task, item_task.data(), 4 * item_task.size(), hipMemcpyHostToDevice, s) !=
-          hipSuccess ||
-      hipMemcpyAsync(db->tabs, tabs.data(), 4 * tabs.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
+  db->start = (const int *)b;
+  db->items = (const BatchItem *)(b + sb);
+  db->total = (long long)ordered.size();
+  if (hipMemcpyAsync((void *)db->start, start.data(), 4 * start.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
+      (!ordered.empty() && hipMemcpyAsync((void *)db->items, ordered.data(), sizeof(BatchItem) * ordered.size(),
+                                          hipMemcpyHostToDevice, s) != hipSuccess) ||
       hipStreamSynchronize(s) != hipSuccess) {
     dev_free(g, db->base, db->handle);
     return TALSH_FAILURE;
@@ -269,21 +176,43 @@ int upload(int g, std::vector<HostTask> &hts, DeviceBatch *db, hipStream_t s) {
   return TALSH_SUCCESS;
 }
 
-// Workgroups of the persistent launch: as many as are resident on the CUs at
-// once (4 per CU: 4 waves per SIMD by registers, 35 KB of LDS each for R8).
-int persistent_grid(int g, int dk) {
-  hipDeviceProp_t prop;
-  const int per_cu = 4;
-  (void)dk;
-  if (hipGetDeviceProperties(&prop, rt().gpus[g].hip_id) != hipSuccess) return 256 * per_cu;
-  return std::max(1, prop.multiProcessorCount) * per_cu;
+// Redirects every output into one scratch allocation (never into user tensors)
+// and makes the contractions non-accumulative; returns the allocation.
+int scratch_outputs(int g, std::vector<HostTask> *hts, void **sp, int *sh) {
+  size_t scratch = 0;
+  for (const auto &h : *hts) scratch += (h.dbytes + 255) / 256 * 256;
+  if (int e = dev_alloc(g, std::max<size_t>(256, scratch), sp, sh)) return e;
+  size_t off = 0;
+  for (auto &h : *hts) {
+    h.D = (uint64_t)((char *)*sp + off);
+    h.acc = false;
+    off += (h.dbytes + 255) / 256 * 256;
+  }
+  return TALSH_SUCCESS;
 }
 
-// Times every candidate tile shape on up to 32 tasks of each untuned class.
-int autotune(int g, int dk, std::vector<HostTask> &hts, hipStream_t s, int *nclasses, double *ms_spent) {
+// Launch time (ms) of one schedule: the second of two launches (the first warms caches).
+float time_launch(int g, int dk, const std::vector<HostTask> &hts, int nblocks, hipStream_t s, hipEvent_t e0,
+                  hipEvent_t e1, int *err) {
+  DeviceBatch db;
+  if ((*err = upload(g, dk, hts, nblocks, s, &db)) != 0) return 1e30f;
+  float ms = 1e30f;
+  for (int rep = 0; rep < 2; ++rep) {
+    (void)hipEventRecord(e0, s);
+    if (batch_launch(dk, db.items, db.start, nblocks, s) != 0) *err = TALSH_FAILURE;
+    (void)hipEventRecord(e1, s);
+    (void)hipEventSynchronize(e1);
+    (void)hipEventElapsedTime(&ms, e0, e1);
+  }
+  dev_free(g, db.base, db.handle);
+  return ms;
+}
+
+// Stage 1: times every candidate wave tile on up to 32 contractions of each untuned class.
+int autotune(int g, int dk, const std::vector<HostTask> &hts, hipStream_t s, int *nclasses, double *ms_spent) {
   std::map<int, std::vector<int>> classes;
   for (int i = 0; i < (int)hts.size(); ++i) classes[hts[i].cls].push_back(i);
-  const int grid = persistent_grid(g, dk);
+  const int nblocks = persistent_blocks(g, dk);
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return TALSH_FAILURE;
   const double t0 = wall_time();
@@ -292,35 +221,16 @@ int autotune(int g, int dk, std::vector<HostTask> &hts, hipStream_t s, int *ncla
     const auto key = std::make_pair(dk, kv.first);
     if (tuned_tiles().count(key)) continue;
     std::vector<HostTask> probe;
-    size_t scratch = 0;
-    for (int i = 0; i < (int)kv.second.size() && i < 32; ++i) {
-      probe.push_back(hts[kv.second[i]]);
-      scratch += (probe.back().dbytes + 255) / 256 * 256;
-    }
+    for (int i = 0; i < (int)kv.second.size() && i < 32; ++i) probe.push_back(hts[(size_t)kv.second[(size_t)i]]);
     void *sp = nullptr;
     int sh = -1;
-    if ((err = dev_alloc(g, scratch, &sp, &sh)) != 0) break;
-    size_t off = 0;
-    for (auto &p : probe) {  // write into scratch outputs, never into user tensors
-      p.t.D = (char *)sp + off;
-      p.t.beta_one = 0;
-      off += (p.dbytes + 255) / 256 * 256;
-    }
+    if ((err = scratch_outputs(g, &probe, &sp, &sh)) != 0) break;
     float best = 1e30f;
-    TileShape pick = default_tile(probe[0].t.M, probe[0].t.N);
-    for (const TileShape &c : kCandidates) {
-      for (auto &p : probe) set_tile(p, c);
-      DeviceBatch db;
-      if ((err = upload(g, probe, &db, s)) != 0) break;
-      float ms = 0;
-      for (int rep = 0; rep < 2; ++rep) {  // first launch warms caches
-        (void)hipEventRecord(e0, s);
-        batch_launch(dk, db.tasks, db.prefix, db.item_task, db.tabs, db.counter, db.total, grid, s);
-        (void)hipEventRecord(e1, s);
-        (void)hipEventSynchronize(e1);
-        (void)hipEventElapsedTime(&ms, e0, e1);
-      }
-      dev_free(g, db.base, db.handle);
+    WaveTile pick = fit_tile(probe[0].geom.M, probe[0].geom.N);
+    for (const WaveTile &c : kWaveTiles) {
+      for (auto &p : probe) p.tile = c;
+      const float ms = time_launch(g, dk, probe, nblocks, s, e0, e1, &err);
+      if (err) break;
       if (ms < best) {
         best = ms;
         pick = c;
@@ -337,23 +247,23 @@ int autotune(int g, int dk, std::vector<HostTask> &hts, hipStream_t s, int *ncla
   return err;
 }
 
-// Second tuning stage, on the whole batch: a class timed alone favours small
-// tiles (they expose parallelism a 32-task probe lacks), while the full batch
-// has work items to spare and pays a fixed cost per tile.  Each tile policy
-// below is timed on one launch of the entire batch writing scratch outputs;
-// the fastest is applied.  Skipped (policy 0) when the scratch outputs would
-// not fit in a quarter of the free pool.
-enum TilePolicy { POLICY_CLASS_TUNED = 0, POLICY_FIT = 1, POLICY_FIT_MIN32 = 2, POLICY_MAX = 3, NPOLICY = 4 };
+// Stage 2, on the whole batch: a class timed alone favours small tiles (they
+// expose parallelism a 32-contraction probe lacks), while the full batch has
+// items to spare and pays a fixed cost per item.  Each policy below is timed
+// on one launch of the entire batch writing scratch outputs; the fastest is
+// applied.  Skipped (policy 0) when the scratch outputs would not fit in a
+// quarter of the free pool.
+enum TilePolicy { POLICY_CLASS_TUNED = 0, POLICY_FIT = 1, POLICY_SMALL = 2, POLICY_SQUARE = 3, NPOLICY = 4 };
 
-TileShape policy_tile(int policy, int dk, const HostTask &h) {
-  const TileShape fit = default_tile(h.t.M, h.t.N);
+WaveTile policy_tile(int policy, int dk, const HostTask &h) {
   switch (policy) {
-    case POLICY_FIT: return fit;
-    case POLICY_FIT_MIN32: return TileShape{std::max(32, fit.tm), std::max(32, fit.tn)};
-    case POLICY_MAX: return TileShape{64, 64};
+    case POLICY_FIT: return fit_tile(h.geom.M, h.geom.N);
+    case POLICY_SMALL: return WaveTile{16, 16};
+    case POLICY_SQUARE:
+      return h.geom.M <= 16 || h.geom.N <= 16 ? fit_tile(h.geom.M, h.geom.N) : WaveTile{32, 32};
     default: {
       auto it = tuned_tiles().find(std::make_pair(dk, h.cls));
-      return it != tuned_tiles().end() ? it->second : fit;
+      return it != tuned_tiles().end() ? it->second : fit_tile(h.geom.M, h.geom.N);
     }
   }
 }
@@ -365,36 +275,20 @@ int choose_policy(int g, int dk, std::vector<HostTask> hts, hipStream_t s, int *
   if (scratch == 0 || scratch > rt().gpus[g].arena.free_bytes() / 4) return TALSH_SUCCESS;
   void *sp = nullptr;
   int sh = -1;
-  if (dev_alloc(g, scratch, &sp, &sh) != 0) return TALSH_SUCCESS;
-  size_t off = 0;
-  for (auto &h : hts) {  // scratch outputs: user tensors are never written while tuning
-    h.t.D = (char *)sp + off;
-    h.t.beta_one = 0;
-    off += (h.dbytes + 255) / 256 * 256;
-  }
+  if (scratch_outputs(g, &hts, &sp, &sh) != 0) return TALSH_SUCCESS;
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
     dev_free(g, sp, sh);
     return TALSH_FAILURE;
   }
   const double t0 = wall_time();
-  const int grid = persistent_grid(g, dk);
+  const int nblocks = persistent_blocks(g, dk);
   float best = 1e30f;
   int err = TALSH_SUCCESS;
   for (int pol = 0; pol < NPOLICY && !err; ++pol) {
-    for (auto &h : hts) set_tile(h, policy_tile(pol, dk, h));
-    DeviceBatch db;
-    if ((err = upload(g, hts, &db, s)) != 0) break;
-    float ms = 0;
-    for (int rep = 0; rep < 2; ++rep) {  // first launch warms caches
-      (void)hipEventRecord(e0, s);
-      batch_launch(dk, db.tasks, db.prefix, db.item_task, db.tabs, db.counter, db.total, grid, s);
-      (void)hipEventRecord(e1, s);
-      (void)hipEventSynchronize(e1);
-      (void)hipEventElapsedTime(&ms, e0, e1);
-    }
-    dev_free(g, db.base, db.handle);
-    if (ms < best) {
+    for (auto &h : hts) h.tile = policy_tile(pol, dk, h);
+    const float ms = time_launch(g, dk, hts, nblocks, s, e0, e1, &err);
+    if (!err && ms < best) {
       best = ms;
       *policy = pol;
     }
@@ -430,20 +324,19 @@ std::vector<const void *> signature(const talshAmdBatch *b, int g) {
 int build(talshAmdBatch *b, int g) {
   release_build(b);
   std::vector<HostTask> hts(b->entries.size());
-  for (size_t i = 0; i < b->entries.size(); ++i)
-    if (int e = plan_task(b->entries[i], g, &hts[i])) return e;
-  const int dk = hts.empty() ? R8 : (hts[0].t.D ? hts[0].cls / 10000 : R8);
-  for (auto &h : hts)
-    if (h.cls / 10000 != dk) return TALSH_NOT_IMPLEMENTED;  // one data kind per batch
+  int dk = NO_TYPE;
+  for (size_t i = 0; i < b->entries.size(); ++i) {
+    int tdk = NO_TYPE;
+    if (int e = plan_task(b->entries[i], g, &tdk, &hts[i])) return e;
+    if (dk != NO_TYPE && tdk != dk) return TALSH_NOT_IMPLEMENTED;  // one data kind per batch
+    dk = tdk;
+  }
   // outputs must not overlap one another nor any input of the batch (byte ranges)
-  const size_t esz = (size_t)dk_size(dk);
   std::vector<std::pair<uintptr_t, uintptr_t>> outs, ins;
-  for (size_t i = 0; i < hts.size(); ++i) {
-    const auto &e = b->entries[i];
-    const uintptr_t d = (uintptr_t)hts[i].t.D, l = (uintptr_t)hts[i].t.L, r = (uintptr_t)hts[i].t.R;
-    outs.emplace_back(d, d + shape_volume(e.d->shape_p) * esz);
-    ins.emplace_back(l, l + shape_volume(e.l->shape_p) * esz);
-    ins.emplace_back(r, r + shape_volume(e.r->shape_p) * esz);
+  for (const auto &h : hts) {
+    outs.emplace_back((uintptr_t)h.D, (uintptr_t)(h.D + h.dbytes));
+    ins.emplace_back((uintptr_t)h.L, (uintptr_t)(h.L + h.lbytes));
+    ins.emplace_back((uintptr_t)h.R, (uintptr_t)(h.R + h.rbytes));
   }
   std::sort(outs.begin(), outs.end());
   for (size_t i = 1; i < outs.size(); ++i)
@@ -460,30 +353,27 @@ int build(talshAmdBatch *b, int g) {
   b->policy = policy;
   b->flops = 0;
   for (auto &h : hts) {
-    set_tile(h, policy_tile(policy, dk, h));
-    b->flops += 2.0 * h.t.M * (double)h.t.N * h.t.K;
+    h.tile = policy_tile(policy, dk, h);
+    b->flops += 2.0 * h.geom.M * (double)h.geom.N * h.geom.K;
   }
+  const int nblocks = persistent_blocks(g, dk);
   DeviceBatch db;
-  if (int e = upload(g, hts, &db, s)) return e;
+  if (int e = upload(g, dk, hts, nblocks, s, &db)) return e;
   b->dev = db.base;
   b->dev_handle = db.handle;
-  b->d_tasks = db.tasks;
-  b->d_prefix = db.prefix;
-  b->d_item_task = db.item_task;
-  b->d_counter = db.counter;
-  b->d_tabs = db.tabs;
+  b->d_items = db.items;
+  b->d_start = db.start;
   b->total = db.total;
   b->ntasks = (int)hts.size();
-  b->grid = persistent_grid(g, dk);
+  b->nblocks = nblocks;
   b->g = g;
   b->dk = dk;
-  // capture [clear counter, persistent kernel] once; executions replay it
+  // capture the persistent launch once; executions replay it
   hipStream_t cs;
   if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return TALSH_FAILURE;
   int err = TALSH_SUCCESS;
   if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) err = TALSH_FAILURE;
-  if (!err && batch_launch(dk, b->d_tasks, b->d_prefix, b->d_item_task, b->d_tabs, b->d_counter, b->total, b->grid, cs))
-    err = TALSH_FAILURE;
+  if (!err && batch_launch(dk, b->d_items, b->d_start, b->nblocks, cs)) err = TALSH_FAILURE;
   if (hipStreamEndCapture(cs, &b->graph) != hipSuccess) err = TALSH_FAILURE;
   if (!err && hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess) err = TALSH_FAILURE;
   (void)hipStreamDestroy(cs);
@@ -493,6 +383,7 @@ int build(talshAmdBatch *b, int g) {
   }
   b->sig = signature(b, g);
   b->sig_bytes.clear();
+  const size_t esz = (size_t)dk_size(dk);
   for (auto &e : b->entries)
     for (const talsh_tens_t *t : {e.d, e.l, e.r}) b->sig_bytes.push_back(shape_volume(t->shape_p) * esz);
   b->built = true;
