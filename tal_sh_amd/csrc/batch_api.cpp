@@ -42,7 +42,8 @@ struct talshAmdBatch {
   void *dev = nullptr;
   int dev_handle = -1;
   const tal::BatchItem *d_items = nullptr;
-  const int *d_start = nullptr;
+  const int *d_qstart = nullptr;
+  unsigned *d_counters = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int tuned_classes = 0, replays = 0;
@@ -120,19 +121,16 @@ int plan_task(const talshAmdBatch::Entry &e, int g, int *dk_out, HostTask *h) {
   return TALSH_SUCCESS;
 }
 
-// Smallest candidate that covers the tile in one item where possible.
-WaveTile fit_tile(int M, int N) {
-  if (M <= 16) return WaveTile{16, N <= 16 ? 16 : N <= 32 ? 32 : 64};
-  if (N <= 16) return WaveTile{M <= 32 ? 32 : 64, 16};
-  return WaveTile{32, 32};
-}
+// Smallest candidate that covers the contraction, else 32 x 32.
+WaveTile fit_tile(int M, int N) { return WaveTile{M <= 16 ? 16 : 32, N <= 16 ? 16 : 32}; }
 
-// Device image of a schedule: [start (workers + 1 ints)][items, 256-byte aligned].
+// Device image of a schedule: [queue counters][qstart (queues + 1 ints)][items, 256-byte aligned].
 struct DeviceBatch {
   void *base = nullptr;
   int handle = -1;
+  unsigned *counters = nullptr;
   const BatchItem *items = nullptr;
-  const int *start = nullptr;
+  const int *qstart = nullptr;
   long long total = 0;
 };
 
@@ -145,9 +143,8 @@ int persistent_blocks(int g, int dk) {
   return std::max(1, prop.multiProcessorCount) * per_cu;
 }
 
-// Items of `hts` (each with its tile; D redirected when `scratch_d` is set),
-// scheduled over `nblocks` workgroups and copied to the device.
-int upload(int g, int dk, const std::vector<HostTask> &hts, int nblocks, hipStream_t s, DeviceBatch *db) {
+// Items of `hts` (each with its tile), in kXcds work queues, copied to the device.
+int upload(int g, int dk, const std::vector<HostTask> &hts, hipStream_t s, DeviceBatch *db) {
   std::vector<BatchItem> items;
   std::vector<int> task_of;
   for (size_t t = 0; t < hts.size(); ++t) {
@@ -157,16 +154,18 @@ int upload(int g, int dk, const std::vector<HostTask> &hts, int nblocks, hipStre
     task_of.insert(task_of.end(), items.size() - before, (int)t);
   }
   std::vector<BatchItem> ordered;
-  std::vector<int> start;
-  batch_schedule(items, task_of, dk_size(dk), nblocks, batch_workers_per_block(), kXcds, &ordered, &start);
-  const size_t sb = ((4 * start.size() + 255) / 256) * 256;
-  const size_t bytes = sb + sizeof(BatchItem) * std::max<size_t>(1, ordered.size());
+  std::vector<int> qstart;
+  batch_queues(items, task_of, dk_size(dk), kXcds, &ordered, &qstart);
+  const size_t cb = ((batch_counter_bytes(kXcds) + 255) / 256) * 256;
+  const size_t sb = ((4 * qstart.size() + 255) / 256) * 256;
+  const size_t bytes = cb + sb + sizeof(BatchItem) * std::max<size_t>(1, ordered.size());
   if (int e = dev_alloc(g, bytes, &db->base, &db->handle)) return e;
   char *b = (char *)db->base;
-  db->start = (const int *)b;
-  db->items = (const BatchItem *)(b + sb);
+  db->counters = (unsigned *)b;
+  db->qstart = (const int *)(b + cb);
+  db->items = (const BatchItem *)(b + cb + sb);
   db->total = (long long)ordered.size();
-  if (hipMemcpyAsync((void *)db->start, start.data(), 4 * start.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
+  if (hipMemcpyAsync((void *)db->qstart, qstart.data(), 4 * qstart.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
       (!ordered.empty() && hipMemcpyAsync((void *)db->items, ordered.data(), sizeof(BatchItem) * ordered.size(),
                                           hipMemcpyHostToDevice, s) != hipSuccess) ||
       hipStreamSynchronize(s) != hipSuccess) {
@@ -195,11 +194,11 @@ int scratch_outputs(int g, std::vector<HostTask> *hts, void **sp, int *sh) {
 float time_launch(int g, int dk, const std::vector<HostTask> &hts, int nblocks, hipStream_t s, hipEvent_t e0,
                   hipEvent_t e1, int *err) {
   DeviceBatch db;
-  if ((*err = upload(g, dk, hts, nblocks, s, &db)) != 0) return 1e30f;
+  if ((*err = upload(g, dk, hts, s, &db)) != 0) return 1e30f;
   float ms = 1e30f;
   for (int rep = 0; rep < 2; ++rep) {
     (void)hipEventRecord(e0, s);
-    if (batch_launch(dk, db.items, db.start, nblocks, s) != 0) *err = TALSH_FAILURE;
+    if (batch_launch(dk, db.items, db.qstart, db.counters, kXcds, nblocks, s) != 0) *err = TALSH_FAILURE;
     (void)hipEventRecord(e1, s);
     (void)hipEventSynchronize(e1);
     (void)hipEventElapsedTime(&ms, e0, e1);
@@ -253,14 +252,12 @@ int autotune(int g, int dk, const std::vector<HostTask> &hts, hipStream_t s, int
 // on one launch of the entire batch writing scratch outputs; the fastest is
 // applied.  Skipped (policy 0) when the scratch outputs would not fit in a
 // quarter of the free pool.
-enum TilePolicy { POLICY_CLASS_TUNED = 0, POLICY_FIT = 1, POLICY_SMALL = 2, POLICY_SQUARE = 3, NPOLICY = 4 };
+enum TilePolicy { POLICY_CLASS_TUNED = 0, POLICY_FIT = 1, POLICY_SMALL = 2, NPOLICY = 3 };
 
 WaveTile policy_tile(int policy, int dk, const HostTask &h) {
   switch (policy) {
     case POLICY_FIT: return fit_tile(h.geom.M, h.geom.N);
     case POLICY_SMALL: return WaveTile{16, 16};
-    case POLICY_SQUARE:
-      return h.geom.M <= 16 || h.geom.N <= 16 ? fit_tile(h.geom.M, h.geom.N) : WaveTile{32, 32};
     default: {
       auto it = tuned_tiles().find(std::make_pair(dk, h.cls));
       return it != tuned_tiles().end() ? it->second : fit_tile(h.geom.M, h.geom.N);
@@ -358,22 +355,23 @@ int build(talshAmdBatch *b, int g) {
   }
   const int nblocks = persistent_blocks(g, dk);
   DeviceBatch db;
-  if (int e = upload(g, dk, hts, nblocks, s, &db)) return e;
+  if (int e = upload(g, dk, hts, s, &db)) return e;
   b->dev = db.base;
   b->dev_handle = db.handle;
   b->d_items = db.items;
-  b->d_start = db.start;
+  b->d_qstart = db.qstart;
+  b->d_counters = db.counters;
   b->total = db.total;
   b->ntasks = (int)hts.size();
   b->nblocks = nblocks;
   b->g = g;
   b->dk = dk;
-  // capture the persistent launch once; executions replay it
+  // capture [clear the queue counters, persistent launch] once; executions replay it
   hipStream_t cs;
   if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return TALSH_FAILURE;
   int err = TALSH_SUCCESS;
   if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) err = TALSH_FAILURE;
-  if (!err && batch_launch(dk, b->d_items, b->d_start, b->nblocks, cs)) err = TALSH_FAILURE;
+  if (!err && batch_launch(dk, b->d_items, b->d_qstart, b->d_counters, kXcds, b->nblocks, cs)) err = TALSH_FAILURE;
   if (hipStreamEndCapture(cs, &b->graph) != hipSuccess) err = TALSH_FAILURE;
   if (!err && hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess) err = TALSH_FAILURE;
   (void)hipStreamDestroy(cs);

@@ -5,15 +5,21 @@
 // kernel forms comes from the inline functions below.
 //
 // An item is one wave tile of one contraction D(m,n) (+)= alpha * sum_k
-// L(m,k) * R(k,n): BI x BJ blocks of 16 x 16 (at most four blocks), computed
-// by one wave with 16x16x4 MFMAs straight from the operands' own layouts.
-// Rows m, columns n and contracted indices k are mixed-radix numbers over up
-// to three (fused) tensor dimensions each; a row's byte offset in L and in D,
-// a column's in R and in D and a contracted index's in L and in R are sums of
-// digit * stride.  Offsets are 32-bit byte offsets into buffer resources
-// whose range is the operand's size: an index past the edge (m >= M, n >= N,
-// k >= K) gets the offset kOutside, so its loads return 0 and its stores are
-// dropped by the hardware range check, with no masks in the kernel.
+// L(m,k) * R(k,n): BI x BJ blocks of 16 x 16 (BI, BJ in {1, 2}), computed by
+// one wave with 16x16x4 MFMAs.  Rows m, columns n and contracted indices k
+// are mixed-radix numbers over up to three (fused) tensor dimensions each; a
+// row's byte offset in L and in D, a column's in R and in D and a contracted
+// index's in L and in R are sums of digit * stride.  Offsets are 32-bit byte
+// offsets into buffer resources whose range is the operand's size: an index
+// past the edge (m >= M, n >= N, k >= K) gets the offset kOutside, so its
+// loads return 0 and its stores are dropped by the hardware range check, with
+// no masks in the kernel.
+//
+// Operands are staged through a wave-private LDS tile of kBatchKC contracted
+// indices at a time, loaded along whichever index is the operand's
+// contiguous one (its rows, or its contracted indices), so that consecutive
+// lanes read consecutive addresses; the MFMAs then read their fragments from
+// LDS.
 #pragma once
 
 #include <cstdint>
@@ -31,9 +37,11 @@ namespace tal {
 constexpr uint32_t kOutside = 0x7FFFFF00u;
 // Largest operand (bytes) a batch accepts, so that every valid offset is below kOutside.
 constexpr uint64_t kBatchMaxBytes = kOutside;
-// K steps (of 4 contracted indices) whose loads a wave keeps in flight ahead of its MFMAs.
-constexpr int kBatchDepthF64 = 4;
-constexpr int kBatchDepthF32 = 6;
+// Contracted indices per LDS chunk (four MFMA steps of 4).
+constexpr int kBatchKC = 16;
+// Chunks whose loads a wave keeps in flight ahead of the chunk its MFMAs use.
+constexpr int kBatchDepthF64 = 1;
+constexpr int kBatchDepthF32 = 2;
 
 // Division of n < 2^31 by a fixed d >= 1 with a multiply-high (round-up
 // method of Granlund and Montgomery, PLDI 1994): with l = ceil(log2 d) and
@@ -61,8 +69,10 @@ struct BatchGroup {
   int32_t s[2][3];
 };
 
-// The wave tile: bits 0-1 BI - 1, bits 2-3 BJ - 1 (blocks of 16 rows /
-// columns), bit 4 set when D is accumulated (D += ...), else D = ....
+// `shape`: bits 0-1 BI - 1, bits 2-3 BJ - 1 (blocks of 16 rows / columns),
+// bit 4 set when D is accumulated (D += ...), else D = ..., bit 5 / bit 6 set
+// when L / R is staged along its contracted indices (else along its rows /
+// columns).
 struct BatchItem {
   uint64_t L, R, D;                 // device addresses of the operands
   uint32_t lbytes, rbytes, dbytes;  // their sizes: the buffer ranges
@@ -82,6 +92,8 @@ static_assert(sizeof(BatchItem) == 256, "BatchItem is four 64-byte lines");
 TAL_HD inline int item_bi(const BatchItem &it) { return (it.shape & 3) + 1; }
 TAL_HD inline int item_bj(const BatchItem &it) { return ((it.shape >> 2) & 3) + 1; }
 TAL_HD inline bool item_accumulates(const BatchItem &it) { return (it.shape >> 4) & 1; }
+TAL_HD inline bool item_a_along_k(const BatchItem &it) { return (it.shape >> 5) & 1; }
+TAL_HD inline bool item_b_along_k(const BatchItem &it) { return (it.shape >> 6) & 1; }
 
 TAL_HD inline void group_offsets(const BatchGroup &g, uint32_t x, uint32_t *o0, uint32_t *o1) {
   const uint32_t q0 = divide(x, g.mg0, g.sh0), d0 = x - q0 * g.e0;
@@ -90,43 +102,50 @@ TAL_HD inline void group_offsets(const BatchGroup &g, uint32_t x, uint32_t *o0, 
   *o1 = (uint32_t)((int32_t)d0 * g.s[1][0] + (int32_t)d1 * g.s[1][1] + (int32_t)d2 * g.s[1][2]);
 }
 
-// MFMA 16x16x4 operand layout: lane l holds A(row l % 16, k l / 16) and
-// B(k l / 16, column l % 16) of a 16 x 4 / 4 x 16 step.
-
-// Byte offset in L of the row lane `lane` loads for block row i.
-TAL_HD inline uint32_t a_row_offset(const BatchItem &it, int i, int lane) {
-  const int m = it.m0 + 16 * i + (lane & 15);
+// Byte offsets of an index past its group's extent `count`: kOutside.
+// Row m: offset in L (t = 0) or in D (t = 1).
+TAL_HD inline uint32_t row_offset(const BatchItem &it, int m, int t) {
   if (m >= it.M) return kOutside;
   uint32_t ol, od;
   group_offsets(it.gm, (uint32_t)m, &ol, &od);
-  return ol;
+  return t ? od : ol;
 }
-// Byte offsets in R and in D of the column lane `lane` handles in block column j.
-TAL_HD inline void b_col_offsets(const BatchItem &it, int j, int lane, uint32_t *orr, uint32_t *od) {
-  const int n = it.n0 + 16 * j + (lane & 15);
-  if (n >= it.N) {
-    *orr = *od = kOutside;
-    return;
-  }
-  group_offsets(it.gn, (uint32_t)n, orr, od);
+// Column n: offset in R (t = 0) or in D (t = 1).
+TAL_HD inline uint32_t col_offset(const BatchItem &it, int n, int t) {
+  if (n >= it.N) return kOutside;
+  uint32_t orr, od;
+  group_offsets(it.gn, (uint32_t)n, &orr, &od);
+  return t ? od : orr;
 }
-// Byte offsets in L and in R of the contracted index lane `lane` loads in K step `chunk` (4 indices per step).
-TAL_HD inline void k_offsets(const BatchItem &it, int chunk, int lane, uint32_t *ol, uint32_t *orr) {
-  const int k = 4 * chunk + (lane >> 4);
+// Contracted index k: offsets in L and in R.
+TAL_HD inline void k_offsets(const BatchItem &it, int k, uint32_t *ol, uint32_t *orr) {
   if (k >= it.K) {
     *ol = *orr = kOutside;
     return;
   }
   group_offsets(it.gk, (uint32_t)k, ol, orr);
 }
-// Byte offset in D of row `row` (0..15) of block row i.
-TAL_HD inline uint32_t d_row_offset(const BatchItem &it, int i, int row) {
-  const int m = it.m0 + 16 * i + row;
-  if (m >= it.M) return kOutside;
-  uint32_t ol, od;
-  group_offsets(it.gm, (uint32_t)m, &ol, &od);
-  return od;
+
+// Staging of an operand tile of `tr` rows (16 or 32) x kBatchKC contracted
+// indices: in its load j (0 <= j < tr * kBatchKC / 64) lane `lane` loads
+// tile row *r, chunk-local contracted index *c.  Along k, 16 lanes run over
+// the contracted indices of one row; along the rows, `tr` lanes run over the
+// rows of one contracted index.
+TAL_HD inline void stage_coords(bool along_k, int tr, int lane, int j, int *r, int *c) {
+  if (along_k) {
+    *c = lane % kBatchKC;
+    *r = lane / kBatchKC + j * (64 / kBatchKC);
+  } else {
+    *r = lane % tr;
+    *c = lane / tr + j * (64 / tr);
+  }
 }
+
+// MFMA 16x16x4 operand layout, read from the staged tiles: in step s of a
+// chunk, lane l takes A(row 16 i + l % 16, index 4 s + l / 16) of block row i
+// and B(index 4 s + l / 16, column 16 j + l % 16) of block column j.
+TAL_HD inline int frag_row(int blk, int lane) { return 16 * blk + (lane & 15); }
+TAL_HD inline int frag_k(int step, int lane) { return 4 * step + (lane >> 4); }
 
 // Accumulator layout of the 16x16x4 MFMAs: register r of lane l holds
 // C(row, l % 16) with row 4 * r + l / 16 (FP64) or 4 * (l / 16) + r (FP32).

@@ -5,12 +5,11 @@
 #include <cmath>
 #include <functional>
 #include <numeric>
-#include <queue>
 #include <utility>
 
 namespace tal {
 
-const WaveTile kWaveTiles[6] = {{16, 16}, {16, 32}, {32, 16}, {32, 32}, {16, 64}, {64, 16}};
+const WaveTile kWaveTiles[4] = {{16, 16}, {16, 32}, {32, 16}, {32, 32}};
 
 namespace {
 
@@ -92,6 +91,8 @@ int batch_task_geometry(const DigitalPattern &p, const int *ddims, const int *ld
   const int lf = fastest(p.lrank, ldims), rf = fastest(p.rrank, rdims);
   const bool l_kfast = lf >= 0 && p.dig[(size_t)lf] < 0, r_kfast = rf >= 0 && p.dig[(size_t)(p.lrank + rf)] < 0;
   const int kby = (r_kfast && !l_kfast) ? 1 : 0;
+  g->l_along_k = l_kfast;
+  g->r_along_k = r_kfast;
   if (!make_group(m, 0, esz, &g->gm, &g->M, &g->ndims_m) || !make_group(n, 0, esz, &g->gn, &g->N, &g->ndims_n) ||
       !make_group(k, kby, esz, &g->gk, &g->K, &g->ndims_k))
     return 2;
@@ -110,7 +111,8 @@ void batch_task_items(const BatchTaskGeom &g, uint64_t L, uint64_t R, uint64_t D
   it.M = g.M;
   it.N = g.N;
   it.K = g.K;
-  it.shape = (tile.tm / 16 - 1) | ((tile.tn / 16 - 1) << 2) | (accumulate ? 16 : 0);
+  it.shape = (tile.tm / 16 - 1) | ((tile.tn / 16 - 1) << 2) | (accumulate ? 16 : 0) | (g.l_along_k ? 32 : 0) |
+             (g.r_along_k ? 64 : 0);
   it.alpha = alpha;
   it.gm = g.gm;
   it.gn = g.gn;
@@ -125,18 +127,16 @@ void batch_task_items(const BatchTaskGeom &g, uint64_t L, uint64_t R, uint64_t D
 
 double batch_item_cost(const BatchItem &it, int esz) {
   // The kernel is bound by memory latency: an item costs about three round
-  // trips (descriptor, first loads, stores) plus one per kBatchDepth K steps
-  // in flight, plus its MFMA issue time (a 16x16x4 MFMA is about 1/20 of a
-  // round trip in FP64 and 1/40 in FP32 when three waves share the pipe).
-  const double bi = item_bi(it), bj = item_bj(it), steps = (it.K + 3) / 4;
+  // trips (descriptor, first loads, stores) plus one per kBatchDepth chunks in
+  // flight, plus the issue time of its loads and MFMAs.
+  const double bi = item_bi(it), bj = item_bj(it), chunks = (it.K + kBatchKC - 1) / kBatchKC;
   const double depth = esz == 8 ? kBatchDepthF64 : kBatchDepthF32;
-  return 3.0 + steps / depth + steps * bi * bj * (esz == 8 ? 0.05 : 0.025);
+  return 3.0 + chunks / depth + chunks * (0.1 * (bi + bj) + bi * bj * (esz == 8 ? 0.2 : 0.1));
 }
 
-void batch_schedule(const std::vector<BatchItem> &items, const std::vector<int> &task_of, int esz, int nblocks,
-                    int waves, int nxcd, std::vector<BatchItem> *ordered, std::vector<int> *start) {
-  const int nw = nblocks * waves;
-  nxcd = std::max(1, std::min(nxcd, nblocks));
+void batch_queues(const std::vector<BatchItem> &items, const std::vector<int> &task_of, int esz, int nq,
+                  std::vector<BatchItem> *ordered, std::vector<int> *qstart) {
+  nq = std::max(1, nq);
   std::vector<double> cost(items.size());
   double total = 0;
   int ntask = 0;
@@ -151,12 +151,11 @@ void batch_schedule(const std::vector<BatchItem> &items, const std::vector<int> 
     of_task[(size_t)task_of[i]].push_back((int)i);
     tcost[(size_t)task_of[i]] += cost[i];
   }
-  // 1. units: a contraction, or for a large one consecutive runs of its items
+  // units: a contraction, or for a large one runs of consecutive items
   // (neighbouring tiles share operand panels) of about `chunk` cost each
-  const double chunk = total / nxcd / 8;
+  const double chunk = total / nq / 8;
   struct Unit {
     double cost;
-    int task;
     std::vector<int> items;
   };
   std::vector<Unit> units;
@@ -165,57 +164,39 @@ void batch_schedule(const std::vector<BatchItem> &items, const std::vector<int> 
     if (its.empty()) continue;
     const int pieces = std::max(1, (int)std::floor(tcost[(size_t)t] / std::max(chunk, 1e-30)));
     const double per = tcost[(size_t)t] / pieces;
-    Unit u{0.0, t, {}};
+    Unit u{0.0, {}};
     int emitted = 0;
     for (int i : its) {
       u.items.push_back(i);
       u.cost += cost[(size_t)i];
       if (emitted + 1 < pieces && u.cost >= per) {  // the last piece takes the rest
         units.push_back(u);
-        u = Unit{0.0, t, {}};
+        u = Unit{0.0, {}};
         ++emitted;
       }
     }
     if (!u.items.empty()) units.push_back(u);
   }
-  // 2. units onto XCDs, largest first, each to the least loaded one
   std::vector<int> order(units.size());
   std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return units[(size_t)a].cost > units[(size_t)b].cost; });
-  std::vector<double> xload((size_t)nxcd, 0.0);
-  std::vector<std::vector<int>> xunits((size_t)nxcd);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return units[(size_t)a].cost > units[(size_t)b].cost; });
+  std::vector<double> qload((size_t)nq, 0.0);
+  std::vector<std::vector<int>> qunits((size_t)nq);
   for (int u : order) {
-    const int x = (int)(std::min_element(xload.begin(), xload.end()) - xload.begin());
-    xunits[(size_t)x].push_back(u);
-    xload[(size_t)x] += units[(size_t)u].cost;
-  }
-  // 3. within an XCD, units in that order and each unit's items to the least
-  // loaded waves: the items of a unit go to waves of about equal load, so
-  // they run at about the same time and share the unit's operand lines and
-  // D lines in the XCD's L2
-  std::vector<std::vector<int>> witems((size_t)nw);
-  for (int x = 0; x < nxcd; ++x) {
-    using Slot = std::pair<double, int>;
-    std::priority_queue<Slot, std::vector<Slot>, std::greater<Slot>> heap;
-    for (int b = x; b < nblocks; b += nxcd)
-      for (int v = 0; v < waves; ++v) heap.push(Slot{0.0, b * waves + v});
-    for (int u : xunits[(size_t)x])
-      for (int i : units[(size_t)u].items) {
-        Slot s = heap.top();
-        heap.pop();
-        witems[(size_t)s.second].push_back(i);
-        s.first += cost[(size_t)i];
-        heap.push(s);
-      }
+    const int q = (int)(std::min_element(qload.begin(), qload.end()) - qload.begin());
+    qunits[(size_t)q].push_back(u);
+    qload[(size_t)q] += units[(size_t)u].cost;
   }
   ordered->clear();
   ordered->reserve(items.size());
-  start->assign((size_t)nw + 1, 0);
-  for (int w = 0; w < nw; ++w) {
-    (*start)[(size_t)w] = (int)ordered->size();
-    for (int i : witems[(size_t)w]) ordered->push_back(items[(size_t)i]);
+  qstart->assign((size_t)nq + 1, 0);
+  for (int q = 0; q < nq; ++q) {
+    (*qstart)[(size_t)q] = (int)ordered->size();
+    for (int u : qunits[(size_t)q])
+      for (int i : units[(size_t)u].items) ordered->push_back(items[(size_t)i]);
   }
-  (*start)[(size_t)nw] = (int)ordered->size();
+  (*qstart)[(size_t)nq] = (int)ordered->size();
 }
 
 }  // namespace tal

@@ -16,6 +16,7 @@ struct BatchTaskGeom {
   BatchGroup gm, gn, gk;
   int M = 1, N = 1, K = 1;
   int ndims_m = 0, ndims_n = 0, ndims_k = 0;  // dimensions per group after fusion
+  bool l_along_k = false, r_along_k = false;  // the operand's contiguous index is a contracted one
 };
 
 // Builds the index groups of D (+)= L * R from its digital pattern and
@@ -31,11 +32,11 @@ struct BatchTaskGeom {
 int batch_task_geometry(const DigitalPattern &p, const int *ddims, const int *ldims, const int *rdims, int esz,
                         BatchTaskGeom *g);
 
-// Candidate wave tiles (rows x columns of 16-multiples, at most four 16 x 16 blocks).
+// Candidate wave tiles (rows x columns: 16 or 32 each).
 struct WaveTile {
   int tm, tn;
 };
-extern const WaveTile kWaveTiles[6];
+extern const WaveTile kWaveTiles[4];
 
 // Items of one contraction for wave tile (tm, tn), appended to `out`.
 void batch_task_items(const BatchTaskGeom &g, uint64_t L, uint64_t R, uint64_t D, uint64_t lbytes, uint64_t rbytes,
@@ -44,14 +45,17 @@ void batch_task_items(const BatchTaskGeom &g, uint64_t L, uint64_t R, uint64_t D
 // Estimated cost of an item (arbitrary units: issue slots of its loads and MFMAs plus a fixed part).
 double batch_item_cost(const BatchItem &it, int esz);
 
-// Static schedule of items over `nblocks` workgroups of `waves` waves each;
-// block b runs on XCD b % nxcd.  Contractions are kept on one XCD (their
-// items share operand panels in its L2) unless a contraction is a sizeable
-// part of the whole; items are then spread over the XCD's waves largest
-// first, each to the least loaded wave.  `task_of[i]` is the contraction of
-// item i.  Outputs: the items in worker order and, for worker w = b * waves
-// + wave, its items at [start[w], start[w + 1]).
-void batch_schedule(const std::vector<BatchItem> &items, const std::vector<int> &task_of, int esz, int nblocks,
-                    int waves, int nxcd, std::vector<BatchItem> *ordered, std::vector<int> *start);
+// Work queues of the persistent kernel, one per XCD (workgroup b takes its
+// items from queue b % nq while it has any, then from the others in turn).
+// Contractions are kept whole in one queue (their items share operand panels
+// in the XCD's L2) unless a contraction is a sizeable part of the whole; it
+// is then cut into runs of neighbouring tiles.  Units go to the least loaded
+// queue, largest first, and each queue holds its units largest first with a
+// unit's items side by side, so the waves that pull them concurrently share
+// the unit's lines and the queue ends with its smallest items.  `task_of[i]`
+// is the contraction of item i.  Outputs: the items in queue order and, for
+// queue q, its items at [qstart[q], qstart[q + 1]).
+void batch_queues(const std::vector<BatchItem> &items, const std::vector<int> &task_of, int esz, int nq,
+                  std::vector<BatchItem> *ordered, std::vector<int> *qstart);
 
 }  // namespace tal

@@ -40,46 +40,71 @@ void store(void *base, uint32_t bytes, uint32_t off, S v, long long *bad, int *c
   if (off < tal::kOutside) ++*bad;
 }
 
-// One wave computing one item, lane by lane.
+// One wave computing one item, lane by lane: every chunk of kBatchKC
+// contracted indices is loaded into the staged tiles with the kernel's lane
+// mapping (each tile entry written exactly once per chunk, checked), then
+// read back as MFMA fragments.
 template <typename S>
 void emulate_item(const tal::BatchItem &it, long long *bad, int *count) {
   using namespace tal;
-  const int BI = item_bi(it), BJ = item_bj(it);
+  const int BI = item_bi(it), BJ = item_bj(it), TA = 16 * BI, TB = 16 * BJ;
+  const bool ak = item_a_along_k(it), bk = item_b_along_k(it);
   const void *L = (const void *)it.L, *R = (const void *)it.R;
   void *D = (void *)it.D;
-  uint32_t arow[4][64], bcol[4][64], dcol[4][64];
-  for (int lane = 0; lane < 64; ++lane) {
-    for (int i = 0; i < BI; ++i) arow[i][lane] = a_row_offset(it, i, lane);
-    for (int j = 0; j < BJ; ++j) b_col_offsets(it, j, lane, &bcol[j][lane], &dcol[j][lane]);
-  }
   std::vector<S> C((size_t)BI * BJ * 256, S(0));  // block (i, j): C[((i * BJ + j) * 16 + row) * 16 + col]
-  const int nsteps = (it.K + 3) / 4;
-  for (int c = 0; c < nsteps; ++c) {
-    S a[4][64], b[4][64];
+  const int nch = (it.K + kBatchKC - 1) / kBatchKC;
+  for (int ch = 0; ch < nch; ++ch) {
+    const int k0 = ch * kBatchKC;
+    S a[32][kBatchKC], b[32][kBatchKC];
+    int wa[32][kBatchKC] = {}, wb[32][kBatchKC] = {};
     for (int lane = 0; lane < 64; ++lane) {
-      uint32_t kl, kr;
-      k_offsets(it, c, lane, &kl, &kr);
-      for (int i = 0; i < BI; ++i) a[i][lane] = load<S>(L, it.lbytes, arow[i][lane] + kl, bad);
-      for (int j = 0; j < BJ; ++j) b[j][lane] = load<S>(R, it.rbytes, kr + bcol[j][lane], bad);
+      for (int j = 0; j < TA * kBatchKC / 64; ++j) {
+        int r, c;
+        stage_coords(ak, TA, lane, j, &r, &c);
+        uint32_t kl, kr;
+        k_offsets(it, k0 + c, &kl, &kr);
+        a[r][c] = load<S>(L, it.lbytes, row_offset(it, it.m0 + r, 0) + kl, bad);
+        ++wa[r][c];
+      }
+      for (int j = 0; j < TB * kBatchKC / 64; ++j) {
+        int r, c;
+        stage_coords(bk, TB, lane, j, &r, &c);
+        uint32_t kl, kr;
+        k_offsets(it, k0 + c, &kl, &kr);
+        b[r][c] = load<S>(R, it.rbytes, kr + col_offset(it, it.n0 + r, 0), bad);
+        ++wb[r][c];
+      }
     }
-    // MFMA 16x16x4: A(row, k) in lane row + 16 k, B(k, col) in lane col + 16 k
-    for (int i = 0; i < BI; ++i)
-      for (int j = 0; j < BJ; ++j)
-        for (int row = 0; row < 16; ++row)
-          for (int col = 0; col < 16; ++col) {
-            S s = C[((size_t)(i * BJ + j) * 16 + row) * 16 + col];
-            for (int k = 0; k < 4; ++k) s += a[i][row + 16 * k] * b[j][col + 16 * k];
-            C[((size_t)(i * BJ + j) * 16 + row) * 16 + col] = s;
-          }
+    for (int r = 0; r < 32; ++r)
+      for (int c = 0; c < kBatchKC; ++c) {
+        if ((r < TA) != (wa[r][c] == 1) || (r >= TA && wa[r][c] != 0)) ++*bad;
+        if ((r < TB) != (wb[r][c] == 1) || (r >= TB && wb[r][c] != 0)) ++*bad;
+      }
+    // MFMA 16x16x4 steps: A(row, k) in lane row + 16 k, B(k, col) in lane col + 16 k
+    for (int st = 0; st < kBatchKC / 4; ++st) {
+      S fa[2][64], fb[2][64];
+      for (int lane = 0; lane < 64; ++lane) {
+        for (int i = 0; i < BI; ++i) fa[i][lane] = a[frag_row(i, lane)][frag_k(st, lane)];
+        for (int j = 0; j < BJ; ++j) fb[j][lane] = b[frag_row(j, lane)][frag_k(st, lane)];
+      }
+      for (int i = 0; i < BI; ++i)
+        for (int j = 0; j < BJ; ++j)
+          for (int row = 0; row < 16; ++row)
+            for (int col = 0; col < 16; ++col) {
+              S s = C[((size_t)(i * BJ + j) * 16 + row) * 16 + col];
+              for (int k = 0; k < 4; ++k) s += fa[i][row + 16 * k] * fb[j][col + 16 * k];
+              C[((size_t)(i * BJ + j) * 16 + row) * 16 + col] = s;
+            }
+    }
   }
   const S alpha = (S)it.alpha;
   for (int lane = 0; lane < 64; ++lane)
     for (int i = 0; i < BI; ++i)
       for (int r = 0; r < 4; ++r) {
         const int row = acc_row((int)sizeof(S), lane, r);
-        const uint32_t drow = d_row_offset(it, i, row);
+        const uint32_t drow = row_offset(it, it.m0 + 16 * i + row, 1);
         for (int j = 0; j < BJ; ++j) {
-          const uint32_t off = drow + dcol[j][lane];
+          const uint32_t off = drow + col_offset(it, it.n0 + frag_row(j, lane), 1);
           S v = alpha * C[((size_t)(i * BJ + j) * 16 + row) * 16 + (lane & 15)];
           if (item_accumulates(it)) v = load<S>(D, it.dbytes, off, bad) + v;
           store<S>(D, it.dbytes, off, v, bad, count);
@@ -145,13 +170,12 @@ int batch_emulate(const char *pattern, const int *ddims, const int *ldims, const
   return 0;
 }
 
-// Schedules `ntask` synthetic contractions (M[t] x N[t] x K[t], wave tile
-// tm[t] x tn[t]) over nblocks x waves workers on nxcd XCDs.  Outputs: for each
-// scheduled position its task and tile origin, the worker start offsets, the
-// estimated cost of each scheduled item and per worker.  Returns the number of items.
-long long batch_schedule_probe(int ntask, const int *M, const int *N, const int *K, const int *tm, const int *tn,
-                               int esz, int nblocks, int waves, int nxcd, int *out_task, int *out_m0, int *out_n0,
-                               int *start, double *icost, double *wcost) {
+// Queues `ntask` synthetic contractions (M[t] x N[t] x K[t], wave tile
+// tm[t] x tn[t]) over nq work queues.  Outputs: for each queued position its
+// task and tile origin and estimated cost, and the queue start offsets.
+// Returns the number of items.
+long long batch_queue_probe(int ntask, const int *M, const int *N, const int *K, const int *tm, const int *tn, int esz,
+                            int nq, int *out_task, int *out_m0, int *out_n0, double *icost, int *qstart) {
   std::vector<tal::BatchItem> items;
   std::vector<int> task_of;
   for (int t = 0; t < ntask; ++t) {
@@ -164,19 +188,15 @@ long long batch_schedule_probe(int ntask, const int *M, const int *N, const int 
     task_of.insert(task_of.end(), items.size() - before, t);
   }
   std::vector<tal::BatchItem> ordered;
-  std::vector<int> st;
-  tal::batch_schedule(items, task_of, esz, nblocks, waves, nxcd, &ordered, &st);
+  std::vector<int> qs;
+  tal::batch_queues(items, task_of, esz, nq, &ordered, &qs);
   for (size_t i = 0; i < ordered.size(); ++i) {
     out_task[i] = (int)(ordered[i].L >> 20) - 1;
     out_m0[i] = ordered[i].m0;
     out_n0[i] = ordered[i].n0;
     icost[i] = tal::batch_item_cost(ordered[i], esz);
   }
-  for (size_t w = 0; w < st.size(); ++w) start[w] = st[w];
-  for (size_t w = 0; w + 1 < st.size(); ++w) {
-    wcost[w] = 0;
-    for (int i = st[w]; i < st[w + 1]; ++i) wcost[w] += tal::batch_item_cost(ordered[(size_t)i], esz);
-  }
+  for (size_t q = 0; q < qs.size(); ++q) qstart[q] = qs[q];
   return (long long)ordered.size();
 }
 }

@@ -8,15 +8,16 @@ returns 0.  These tests build the real planner with g++ (tests/cxx/
 batch_shim.cpp) and check, without a GPU:
   - the multiply-high division against integer division, exhaustively for
     divisors up to 4096 (the digit decomposition rests on it);
-  - a lane-by-lane emulation of the kernel's waves (same address functions,
-    MFMA operand / accumulator layouts, out-of-range loads read 0 and stores
-    are dropped) reproduces the oracle's contraction for config-4-like
-    contractions and every wave tile, with no offset between an operand's
-    end and the out-of-range marker and every D element stored exactly once;
+  - a lane-by-lane emulation of the kernel's waves (same address functions
+    and staging lane maps, MFMA operand / accumulator layouts, out-of-range
+    loads read 0 and stores are dropped) reproduces the oracle's contraction
+    for config-4-like contractions and every wave tile, with no offset
+    between an operand's end and the out-of-range marker, every staged tile
+    entry written exactly once per chunk and every D element stored once;
   - groups that cannot be fused into three dimensions, and hyper indices, are
     refused (the single-contraction path runs them);
-  - the static schedule places every item exactly once, keeps small
-    contractions on one XCD and balances each XCD's waves to within one item
+  - the work queues hold every item exactly once, keep a small contraction's
+    items side by side in one queue and balance the queues to within one unit
     (by the planner's own cost estimate).
 """
 import ctypes
@@ -30,7 +31,7 @@ import pytest
 from tal_common import R4, R8, batch_contraction, tensor
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-WAVE_TILES = ((16, 16), (16, 32), (32, 16), (32, 32), (16, 64), (64, 16))
+WAVE_TILES = ((16, 16), (16, 32), (32, 16), (32, 32))
 
 
 @pytest.fixture(scope="module")
@@ -48,9 +49,8 @@ def shim(tmp_path_factory):
     lib.batch_emulate.restype = I
     lib.batch_emulate.argtypes = [ctypes.c_char_p, IP, IP, IP, I, P, P, P, D, I, I, I, ctypes.POINTER(LL), IP,
                                   ctypes.POINTER(LL), IP]
-    lib.batch_schedule_probe.restype = LL
-    lib.batch_schedule_probe.argtypes = [I, IP, IP, IP, IP, IP, I, I, I, I, IP, IP, IP, IP, ctypes.POINTER(D),
-                                         ctypes.POINTER(D)]
+    lib.batch_queue_probe.restype = LL
+    lib.batch_queue_probe.argtypes = [I, IP, IP, IP, IP, IP, I, I, IP, IP, IP, ctypes.POINTER(D), IP]
     return lib
 
 
@@ -142,7 +142,7 @@ def test_geometry_refusals(shim):
     assert rc == 1
 
 
-def test_schedule_places_every_item_once(shim):
+def test_queues_place_every_item_once(shim):
     rng = random.Random(5)
     ntask = 400
     M = [rng.choice([1, 3, 16, 40, 200, 1500]) for _ in range(ntask)]
@@ -154,34 +154,29 @@ def test_schedule_places_every_item_once(shim):
         for n0 in range(0, N[t], tiles[t][1]):
             for m0 in range(0, M[t], tiles[t][0]):
                 want.add((t, m0, n0))
-    nblocks, waves, nxcd = 64, 4, 8
+    nq = 8
     n = len(want)
     ot, om, on = (np.zeros(n, dtype=np.int32) for _ in range(3))
-    start = np.zeros(nblocks * waves + 1, dtype=np.int32)
-    icost, wcost = np.zeros(n), np.zeros(nblocks * waves)
+    qstart = np.zeros(nq + 1, dtype=np.int32)
+    icost = np.zeros(n)
     ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
-    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
-    got_n = shim.batch_schedule_probe(ntask, _ints(M), _ints(N), _ints(K), _ints([x[0] for x in tiles]),
-                                      _ints([x[1] for x in tiles]), 8, nblocks, waves, nxcd, ip(ot), ip(om), ip(on),
-                                      ip(start), dp(icost), dp(wcost))
+    got_n = shim.batch_queue_probe(ntask, _ints(M), _ints(N), _ints(K), _ints([x[0] for x in tiles]),
+                                   _ints([x[1] for x in tiles]), 8, nq, ip(ot), ip(om), ip(on),
+                                   icost.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ip(qstart))
     assert got_n == n
     got = list(zip(ot.tolist(), om.tolist(), on.tolist()))
     assert len(set(got)) == n and set(got) == want
-    assert start[0] == 0 and start[-1] == n and (np.diff(start) >= 0).all()
-    # worker w = block * waves + wave runs on XCD block % nxcd
-    worker_of_pos = np.zeros(n, dtype=np.int64)
-    for w in range(nblocks * waves):
-        worker_of_pos[start[w]:start[w + 1]] = w
-        assert abs(icost[start[w]:start[w + 1]].sum() - wcost[w]) <= 1e-9 * max(1.0, wcost[w])
-    xcd_of_pos = (worker_of_pos // waves) % nxcd
+    assert qstart[0] == 0 and qstart[-1] == n and (np.diff(qstart) >= 0).all()
+    queue_of = np.zeros(n, dtype=np.int64)
+    for q in range(nq):
+        queue_of[qstart[q]:qstart[q + 1]] = q
     tcost = np.bincount(ot, weights=icost, minlength=ntask)
-    chunk = icost.sum() / nxcd / 8
+    chunk = icost.sum() / nq / 8
     for t in range(ntask):
-        if tcost[t] < chunk:  # a small contraction is one unit: it stays on one XCD
-            assert len(set(xcd_of_pos[ot == t].tolist())) == 1, t
-    # within each XCD, wave loads differ by at most the largest item cost there
-    for x in range(nxcd):
-        ws = [w for w in range(nblocks * waves) if (w // waves) % nxcd == x]
-        c = wcost[ws]
-        biggest = icost[xcd_of_pos == x].max()
-        assert c.max() - c.min() <= biggest + 1e-9, (x, c.max(), c.min(), biggest)
+        pos = np.nonzero(ot == t)[0]
+        if tcost[t] < chunk:  # a small contraction is one unit: one queue, its items side by side
+            assert len(set(queue_of[pos].tolist())) == 1, t
+            assert pos.max() - pos.min() + 1 == len(pos), t
+    # queues are loaded to within the largest unit (at most about 2 * chunk)
+    qcost = np.array([icost[qstart[q]:qstart[q + 1]].sum() for q in range(nq)])
+    assert qcost.max() - qcost.min() <= 2 * chunk + 1e-9, (qcost.max(), qcost.min(), chunk)
