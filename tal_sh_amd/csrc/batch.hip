@@ -1,21 +1,21 @@
 // Persistent batched-contraction kernel (gfx950).
 //
 // Thousands of small contractions of uneven shape run in one launch.  Each
-// wave is an independent worker (no barrier): it pulls work items (wave tiles
-// of one contraction, self-contained 256-byte descriptors, batch_item.hpp)
-// from its XCD's queue with one atomic per item, issued as the previous item
-// starts, and turns to the other queues once its own is empty, so the load
-// balances itself whatever each item costs.  Operands are read straight from their
-// dlf storage: a row, column or contracted index becomes a byte offset by
+// workgroup pulls work items (workgroup tiles of one contraction,
+// self-contained 256-byte descriptors, batch_item.hpp) from its XCD's queue
+// with one atomic per item, issued as the previous item starts, and turns to
+// the other queues once its own is empty, so the load balances itself
+// whatever each item costs.  Operands are read straight from their dlf
+// storage: a row, column or contracted index becomes a byte offset by
 // mixed-radix digits (multiply-high division), and buffer loads with the
 // operand's size as their range return 0 for every index past an edge, so no
 // operand is transposed and no load is masked.  Each operand tile of
 // kBatchKC contracted indices is loaded along the operand's contiguous index
-// (coalesced), written to the wave's own LDS region and read back as MFMA
-// fragments; the loads of the next chunk(s) are in flight while the MFMAs of
-// the current one run.  The reference runs such tasks one TTGT schedule (up
-// to 4 transposes + a GEMM launch) at a time (tensor_algebra_gpu_nvidia.cu:
-// 5847-7188).
+// (coalesced) into a double-buffered LDS tile that the four waves read as
+// MFMA fragments, one barrier per chunk; the loads of the next chunks are in
+// flight while the MFMAs of the current one run.  The reference runs such
+// tasks one TTGT schedule (up to 4 transposes + a GEMM launch) at a time
+// (tensor_algebra_gpu_nvidia.cu:5847-7188).
 //
 // Each D element is produced by exactly one item, in a fixed order of the
 // contracted indices, so results are deterministic and independent of the
@@ -65,48 +65,50 @@ struct Wave<float> {
   __device__ static acc_t mfma(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
 };
 
-constexpr int kWaves = 4;  // waves per workgroup (independent workers)
-constexpr int kTile = 32;  // largest tile side (rows or columns)
+constexpr int kWaves = 4;  // waves per workgroup, 2 x 2 quadrants of the tile
+constexpr int kThreads = kWaves * 64;
+constexpr int kTile = 64;           // largest tile side (rows or columns)
 constexpr int kPad = kBatchKC + 1;  // LDS row length: conflict-free fragment reads
 
-// The wave's LDS region: the staged A (rows x contracted) and B (columns x
-// contracted) tiles of the current chunk.
+// Double-buffered staged tiles: A (rows x contracted), B (columns x contracted).
 template <typename S>
-struct WaveLds {
-  S a[kTile][kPad];
-  S b[kTile][kPad];
+struct TileLds {
+  S a[2][kTile][kPad];
+  S b[2][kTile][kPad];
 };
 
-// One item: tile of BI x BJ blocks; AK / BK: L / R staged along k (else along
-// its rows / columns).  `next` is lane 0's pending fetch of the wave's next
-// queue position (an atomic issued just before this item); once the first
-// chunk has arrived, so has that result: one scalar load per 64-byte line of
-// the descriptor at queue position base + next (if inside [.., end)) is then
-// issued and consumed after the item, so the next item starts from the
-// scalar cache instead of memory.
+// One item: tile of 32 BI x 32 BJ; AK / BK: L / R staged along k (else along
+// its rows / columns).  `next` is thread 0's pending fetch of the
+// workgroup's next queue position (an atomic issued just before this item).
+// Once the first chunk has arrived, so has that result, and wave 0 loads one
+// word of each 64-byte line of that descriptor (queue position base + next,
+// if inside [.., end)) into the scalar cache, so the next item's descriptor
+// does not come from memory.
 template <typename S, int BI, int BJ, bool AK, bool BK>
-__device__ __forceinline__ void item_body(const BatchItem &it, int lane, const BatchItem *__restrict__ items,
-                                          unsigned next, int base, int end, WaveLds<S> &lds) {
+__device__ __forceinline__ void item_body(const BatchItem &it, int tid, const BatchItem *__restrict__ items,
+                                          unsigned next, int base, int end, TileLds<S> &lds) {
   using W = Wave<S>;
   using acc_t = typename W::acc_t;
-  constexpr int TA = 16 * BI, TB = 16 * BJ;                       // tile rows, tile columns
-  constexpr int NA = TA * kBatchKC / 64, NB = TB * kBatchKC / 64;  // loads per chunk and lane
+  constexpr int TA = 32 * BI, TB = 32 * BJ;                             // tile rows, tile columns
+  constexpr int NA = TA * kBatchKC / kThreads, NB = TB * kBatchKC / kThreads;  // loads per chunk and thread
   constexpr int PD = W::kDepth;
+  const int lane = tid & 63, wave = tid >> 6;
   const auto rl = rsrc(it.L, it.lbytes), rr = rsrc(it.R, it.rbytes), rd = rsrc(it.D, it.dbytes);
-  // Along k each load has its own row (offsets fixed per item) and the lane
-  // one contracted index; along the rows the lane keeps one row and each load
-  // takes the contracted index of lane c (< kBatchKC) by a lane shuffle.
+  // Along k each load has its own row (offsets fixed per item) and the
+  // thread one contracted index; along the rows the thread keeps one row and
+  // each load takes the contracted index computed by lane c (< kBatchKC) of
+  // the same wave, by a lane shuffle.
   uint32_t aoff[AK ? NA : 1], boff[BK ? NB : 1];
 #pragma unroll
   for (int j = 0; j < (AK ? NA : 1); ++j) {
     int r, c;
-    stage_coords(AK, TA, lane, j, &r, &c);
+    stage_coords(AK, TA, kThreads, tid, j, &r, &c);
     aoff[j] = row_offset(it, it.m0 + r, 0);
   }
 #pragma unroll
   for (int j = 0; j < (BK ? NB : 1); ++j) {
     int r, c;
-    stage_coords(BK, TB, lane, j, &r, &c);
+    stage_coords(BK, TB, kThreads, tid, j, &r, &c);
     boff[j] = col_offset(it, it.n0 + r, 0);
   }
   S ra[PD][NA], rb[PD][NB];
@@ -116,30 +118,30 @@ __device__ __forceinline__ void item_body(const BatchItem &it, int lane, const B
 #pragma unroll
     for (int j = 0; j < NA; ++j) {
       int r, c;
-      stage_coords(AK, TA, lane, j, &r, &c);
+      stage_coords(AK, TA, kThreads, tid, j, &r, &c);
       const uint32_t o = AK ? aoff[j] + kl : aoff[0] + (uint32_t)__shfl((int)kl, c, 64);
       ra[set][j] = W::load(rl, o);
     }
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
       int r, c;
-      stage_coords(BK, TB, lane, j, &r, &c);
+      stage_coords(BK, TB, kThreads, tid, j, &r, &c);
       const uint32_t o = BK ? boff[j] + kr : boff[0] + (uint32_t)__shfl((int)kr, c, 64);
       rb[set][j] = W::load(rr, o);
     }
   };
-  auto stage = [&](int set) {
+  auto stage = [&](int buf, int set) {
 #pragma unroll
     for (int j = 0; j < NA; ++j) {
       int r, c;
-      stage_coords(AK, TA, lane, j, &r, &c);
-      lds.a[r][c] = ra[set][j];
+      stage_coords(AK, TA, kThreads, tid, j, &r, &c);
+      lds.a[buf][r][c] = ra[set][j];
     }
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
       int r, c;
-      stage_coords(BK, TB, lane, j, &r, &c);
-      lds.b[r][c] = rb[set][j];
+      stage_coords(BK, TB, kThreads, tid, j, &r, &c);
+      lds.b[buf][r][c] = rb[set][j];
     }
   };
   acc_t acc[BI][BJ];
@@ -147,14 +149,15 @@ __device__ __forceinline__ void item_body(const BatchItem &it, int lane, const B
   for (int i = 0; i < BI; ++i)
 #pragma unroll
     for (int j = 0; j < BJ; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
-  auto compute = [&]() {
+  const int qr = quad_row(BI, wave), qc = quad_col(BJ, wave);
+  auto compute = [&](int buf) {
 #pragma unroll
     for (int s = 0; s < kBatchKC / 4; ++s) {
       S fa[BI], fb[BJ];
 #pragma unroll
-      for (int i = 0; i < BI; ++i) fa[i] = lds.a[frag_row(i, lane)][frag_k(s, lane)];
+      for (int i = 0; i < BI; ++i) fa[i] = lds.a[buf][qr + frag_row(i, lane)][frag_k(s, lane)];
 #pragma unroll
-      for (int j = 0; j < BJ; ++j) fb[j] = lds.b[frag_row(j, lane)][frag_k(s, lane)];
+      for (int j = 0; j < BJ; ++j) fb[j] = lds.b[buf][qc + frag_row(j, lane)][frag_k(s, lane)];
 #pragma unroll
       for (int i = 0; i < BI; ++i)
 #pragma unroll
@@ -166,16 +169,18 @@ __device__ __forceinline__ void item_body(const BatchItem &it, int lane, const B
   for (int q = 0; q < PD; ++q)
     if (q < nch) load(q, q);
   int pf0 = 0, pf1 = 0, pf2 = 0, pf3 = 0;
-  // chunk c is staged from register set c % PD, whose loads then move on to
-  // chunk c + PD; LDS accesses of one wave complete in order, so staging chunk
-  // c cannot overtake the fragment reads of chunk c - 1
+  // Chunk c is staged from register set c % PD into LDS buffer c % 2 (whose
+  // previous chunk, c - 2, every wave finished reading before the barrier
+  // that ended chunk c - 1), the set then loads chunk c + PD, and after the
+  // barrier the waves multiply chunk c.
   for (int c0 = 0; c0 < nch; c0 += PD) {
 #pragma unroll
     for (int q = 0; q < PD; ++q) {
-      if (c0 + q < nch) {  // uniform over the wave
-        stage(q);
-        if (c0 + q + PD < nch) load(q, c0 + q + PD);
-        if (c0 + q == 0) {  // the next item's descriptor, into the scalar cache
+      const int c = c0 + q;
+      if (c < nch) {  // uniform over the workgroup
+        stage(c & 1, q);
+        if (c + PD < nch) load(q, c + PD);
+        if (c == 0 && wave == 0) {  // the next item's descriptor, into the scalar cache
           const int pn = base + __builtin_amdgcn_readfirstlane((int)next);
           const BatchItem &nx = items[pn < end ? pn : base];
           pf0 = nx.M;
@@ -183,7 +188,8 @@ __device__ __forceinline__ void item_body(const BatchItem &it, int lane, const B
           pf2 = nx.gn.s[0][0];
           pf3 = nx.gk.s[0][0];
         }
-        compute();
+        __syncthreads();
+        compute(c & 1);
       }
     }
   }
@@ -191,12 +197,13 @@ __device__ __forceinline__ void item_body(const BatchItem &it, int lane, const B
   const bool accumulate = item_accumulates(it);
   uint32_t dcol[BJ];
 #pragma unroll
-  for (int j = 0; j < BJ; ++j) dcol[j] = col_offset(it, it.n0 + frag_row(j, lane), 1);
+  for (int j = 0; j < BJ; ++j) dcol[j] = col_offset(it, it.n0 + qc + frag_row(j, lane), 1);
 #pragma unroll
   for (int i = 0; i < BI; ++i) {
     uint32_t drow[4];  // D offsets of this block row's accumulator rows
 #pragma unroll
-    for (int r = 0; r < 4; ++r) drow[r] = row_offset(it, it.m0 + 16 * i + acc_row((int)sizeof(S), lane, r), 1);
+    for (int r = 0; r < 4; ++r)
+      drow[r] = row_offset(it, it.m0 + qr + 16 * i + acc_row((int)sizeof(S), lane, r), 1);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
 #pragma unroll
@@ -214,8 +221,9 @@ __device__ __forceinline__ void item_body(const BatchItem &it, int lane, const B
 // Queue q holds items [qstart[q], qstart[q + 1]); counters[kCounterStride * q]
 // counts the positions taken from it.  Workgroup b starts on queue b % nq
 // and, once a fetch comes back past the end of a queue, moves on to the next
-// one; after a fetch past the end of all nq queues the wave exits, so every
-// wave exits and the grid drains.
+// one; after a fetch past the end of all nq queues it exits, so every
+// workgroup exits and the grid drains.  Thread 0 fetches; the position is
+// published through a two-slot LDS word behind a barrier.
 constexpr int kCounterStride = 64;  // one 256-byte line per counter
 
 __global__ void batch_reset_kernel(unsigned *counters, int nq) {
@@ -223,32 +231,35 @@ __global__ void batch_reset_kernel(unsigned *counters, int nq) {
 }
 
 template <typename S>
-__global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(4))) void batch_contract_kernel(
-    const BatchItem *__restrict__ items, const int *__restrict__ qstart, unsigned *counters, int nq) {
-  __shared__ WaveLds<S> lds[kWaves];
-  const int lane = threadIdx.x & 63;
-  const int v = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
-  int q = (int)blockIdx.x % nq, visited = 1;
-  unsigned got = 0;
-  if (lane == 0) got = atomicAdd(&counters[kCounterStride * q], 1u);
-  int cur = __builtin_amdgcn_readfirstlane((int)got);
+__global__ __launch_bounds__(kThreads) void batch_contract_kernel(const BatchItem *__restrict__ items,
+                                                                 const int *__restrict__ qstart,
+                                                                 unsigned *counters, int nq) {
+  __shared__ TileLds<S> lds;
+  __shared__ int s_pos[2];
+  const int tid = threadIdx.x;
+  int q = (int)blockIdx.x % nq, visited = 1, slot = 0;
+  if (tid == 0) s_pos[0] = (int)atomicAdd(&counters[kCounterStride * q], 1u);
+  __syncthreads();
+  int cur = s_pos[0];
   while (true) {
     const int base = qstart[q], end = qstart[q + 1];
+    slot ^= 1;  // the other slot was last read before the barrier that published `cur`
     if (base + cur >= end) {
       if (visited == nq) break;
       q = q + 1 == nq ? 0 : q + 1;
       ++visited;
-      if (lane == 0) got = atomicAdd(&counters[kCounterStride * q], 1u);
-      cur = __builtin_amdgcn_readfirstlane((int)got);
+      if (tid == 0) s_pos[slot] = (int)atomicAdd(&counters[kCounterStride * q], 1u);
+      __syncthreads();
+      cur = s_pos[slot];
       continue;
     }
     unsigned next = 0;
-    if (lane == 0) next = atomicAdd(&counters[kCounterStride * q], 1u);
+    if (tid == 0) next = atomicAdd(&counters[kCounterStride * q], 1u);
     const BatchItem &it = items[base + cur];
-    // (BI - 1) | (BJ - 1) << 2 | along-k bits of L and R, uniform over the wave
-#define TAL_ITEM(BI, BJ, AK, BK)                                        \
-  case (BI - 1) | (BJ - 1) << 2 | AK << 5 | BK << 6:                    \
-    item_body<S, BI, BJ, AK, BK>(it, lane, items, next, base, end, lds[v]); \
+    // (BI - 1) | (BJ - 1) << 2 | along-k bits of L and R, uniform over the workgroup
+#define TAL_ITEM(BI, BJ, AK, BK)                                         \
+  case (BI - 1) | (BJ - 1) << 2 | AK << 5 | BK << 6:                     \
+    item_body<S, BI, BJ, AK, BK>(it, tid, items, next, base, end, lds); \
     break;
 #define TAL_ITEMS(AK, BK) TAL_ITEM(1, 1, AK, BK) TAL_ITEM(2, 1, AK, BK) TAL_ITEM(1, 2, AK, BK) TAL_ITEM(2, 2, AK, BK)
     switch (it.shape & 0x6f) {
@@ -260,21 +271,22 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(4))
     }
 #undef TAL_ITEMS
 #undef TAL_ITEM
-    cur = __builtin_amdgcn_readfirstlane((int)next);
+    // every wave has finished the item's LDS reads before the next item stages into LDS
+    if (tid == 0) s_pos[slot] = (int)next;
+    __syncthreads();
+    cur = s_pos[slot];
   }
 }
 
 }  // namespace
 
-int batch_workers_per_block() { return kWaves; }
-
 int batch_blocks_per_cu(int dk, int *blocks) {
   int nb = 0;
   hipError_t e = hipErrorInvalidValue;
   if (dk == 8)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_contract_kernel<double>, kWaves * 64, 0);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_contract_kernel<double>, kThreads, 0);
   else if (dk == 4)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_contract_kernel<float>, kWaves * 64, 0);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_contract_kernel<float>, kThreads, 0);
   if (e != hipSuccess || nb <= 0) return -1;
   *blocks = nb;
   return 0;
@@ -282,11 +294,11 @@ int batch_blocks_per_cu(int dk, int *blocks) {
 
 int batch_launch(int dk, const BatchItem *items, const int *qstart, unsigned *counters, int nq, int nblocks,
                  hipStream_t s) {
-  if (!items || !qstart || !counters || nq <= 0 || nq > kWaves * 64 || nblocks <= 0) return -1;
-  batch_reset_kernel<<<1, kWaves * 64, 0, s>>>(counters, nq);
+  if (!items || !qstart || !counters || nq <= 0 || nq > kThreads || nblocks <= 0) return -1;
+  batch_reset_kernel<<<1, kThreads, 0, s>>>(counters, nq);
   switch (dk) {
-    case 8: batch_contract_kernel<double><<<nblocks, kWaves * 64, 0, s>>>(items, qstart, counters, nq); break;
-    case 4: batch_contract_kernel<float><<<nblocks, kWaves * 64, 0, s>>>(items, qstart, counters, nq); break;
+    case 8: batch_contract_kernel<double><<<nblocks, kThreads, 0, s>>>(items, qstart, counters, nq); break;
+    case 4: batch_contract_kernel<float><<<nblocks, kThreads, 0, s>>>(items, qstart, counters, nq); break;
     default: return -1;
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;

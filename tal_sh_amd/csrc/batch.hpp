@@ -7,9 +7,6 @@
 
 namespace tal {
 
-// Waves (independent workers) per workgroup of the persistent kernel.
-int batch_workers_per_block();
-
 // Workgroups of the persistent kernel resident on one CU at once (occupancy).
 int batch_blocks_per_cu(int dk, int *blocks);
 

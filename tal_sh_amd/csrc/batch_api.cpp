@@ -121,8 +121,8 @@ int plan_task(const talshAmdBatch::Entry &e, int g, int *dk_out, HostTask *h) {
   return TALSH_SUCCESS;
 }
 
-// Smallest candidate that covers the contraction, else 32 x 32.
-WaveTile fit_tile(int M, int N) { return WaveTile{M <= 16 ? 16 : 32, N <= 16 ? 16 : 32}; }
+// Smallest candidate that covers the contraction, else 64 x 64.
+WaveTile fit_tile(int M, int N) { return WaveTile{M <= 32 ? 32 : 64, N <= 32 ? 32 : 64}; }
 
 // Device image of a schedule: [queue counters][qstart (queues + 1 ints)][items, 256-byte aligned].
 struct DeviceBatch {
@@ -257,7 +257,7 @@ enum TilePolicy { POLICY_CLASS_TUNED = 0, POLICY_FIT = 1, POLICY_SMALL = 2, NPOL
 WaveTile policy_tile(int policy, int dk, const HostTask &h) {
   switch (policy) {
     case POLICY_FIT: return fit_tile(h.geom.M, h.geom.N);
-    case POLICY_SMALL: return WaveTile{16, 16};
+    case POLICY_SMALL: return WaveTile{32, 32};
     default: {
       auto it = tuned_tiles().find(std::make_pair(dk, h.cls));
       return it != tuned_tiles().end() ? it->second : fit_tile(h.geom.M, h.geom.N);

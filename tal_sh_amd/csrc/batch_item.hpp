@@ -4,10 +4,11 @@
 // and the CPU emulation of the kernel in the tests, so that every address the
 // kernel forms comes from the inline functions below.
 //
-// An item is one wave tile of one contraction D(m,n) (+)= alpha * sum_k
-// L(m,k) * R(k,n): BI x BJ blocks of 16 x 16 (BI, BJ in {1, 2}), computed by
-// one wave with 16x16x4 MFMAs.  Rows m, columns n and contracted indices k
-// are mixed-radix numbers over up to three (fused) tensor dimensions each; a
+// An item is one workgroup tile of one contraction D(m,n) (+)= alpha * sum_k
+// L(m,k) * R(k,n): 32 BI rows x 32 BJ columns (BI, BJ in {1, 2}) computed by
+// the workgroup's four waves, each a quadrant of BI x BJ blocks of 16 x 16
+// with 16x16x4 MFMAs.  Rows m, columns n and contracted indices k are
+// mixed-radix numbers over up to three (fused) tensor dimensions each; a
 // row's byte offset in L and in D, a column's in R and in D and a contracted
 // index's in L and in R are sums of digit * stride.  Offsets are 32-bit byte
 // offsets into buffer resources whose range is the operand's size: an index
@@ -15,11 +16,10 @@
 // loads return 0 and its stores are dropped by the hardware range check, with
 // no masks in the kernel.
 //
-// Operands are staged through a wave-private LDS tile of kBatchKC contracted
-// indices at a time, loaded along whichever index is the operand's
-// contiguous one (its rows, or its contracted indices), so that consecutive
-// lanes read consecutive addresses; the MFMAs then read their fragments from
-// LDS.
+// Operands are staged through LDS kBatchKC contracted indices at a time,
+// loaded along whichever index is the operand's contiguous one (its rows, or
+// its contracted indices), so that consecutive lanes read consecutive
+// addresses; the MFMAs then read their fragments from LDS.
 #pragma once
 
 #include <cstdint>
@@ -39,8 +39,8 @@ constexpr uint32_t kOutside = 0x7FFFFF00u;
 constexpr uint64_t kBatchMaxBytes = kOutside;
 // Contracted indices per LDS chunk (four MFMA steps of 4).
 constexpr int kBatchKC = 16;
-// Chunks whose loads a wave keeps in flight ahead of the chunk its MFMAs use.
-constexpr int kBatchDepthF64 = 1;
+// Chunks whose loads a workgroup keeps in flight ahead of the chunk its MFMAs use.
+constexpr int kBatchDepthF64 = 2;
 constexpr int kBatchDepthF32 = 2;
 
 // Division of n < 2^31 by a fixed d >= 1 with a multiply-high (round-up
@@ -69,7 +69,7 @@ struct BatchGroup {
   int32_t s[2][3];
 };
 
-// `shape`: bits 0-1 BI - 1, bits 2-3 BJ - 1 (blocks of 16 rows / columns),
+// `shape`: bits 0-1 BI - 1, bits 2-3 BJ - 1 (blocks of 16 rows / columns per wave),
 // bit 4 set when D is accumulated (D += ...), else D = ..., bit 5 / bit 6 set
 // when L / R is staged along its contracted indices (else along its rows /
 // columns).
@@ -126,24 +126,30 @@ TAL_HD inline void k_offsets(const BatchItem &it, int k, uint32_t *ol, uint32_t 
   group_offsets(it.gk, (uint32_t)k, ol, orr);
 }
 
-// Staging of an operand tile of `tr` rows (16 or 32) x kBatchKC contracted
-// indices: in its load j (0 <= j < tr * kBatchKC / 64) lane `lane` loads
-// tile row *r, chunk-local contracted index *c.  Along k, 16 lanes run over
-// the contracted indices of one row; along the rows, `tr` lanes run over the
-// rows of one contracted index.
-TAL_HD inline void stage_coords(bool along_k, int tr, int lane, int j, int *r, int *c) {
+// Staging of an operand tile of `tr` rows x kBatchKC contracted indices by
+// `nt` threads: in its load j (0 <= j < tr * kBatchKC / nt) thread t loads
+// tile row *r, chunk-local contracted index *c.  Along k, 16 consecutive
+// threads run over the contracted indices of one row; along the rows, `tr`
+// consecutive threads run over the rows of one contracted index.
+TAL_HD inline void stage_coords(bool along_k, int tr, int nt, int t, int j, int *r, int *c) {
   if (along_k) {
-    *c = lane % kBatchKC;
-    *r = lane / kBatchKC + j * (64 / kBatchKC);
+    *c = t % kBatchKC;
+    *r = t / kBatchKC + j * (nt / kBatchKC);
   } else {
-    *r = lane % tr;
-    *c = lane / tr + j * (64 / tr);
+    *r = t % tr;
+    *c = t / tr + j * (nt / tr);
   }
 }
 
+// Wave v of the workgroup computes the quadrant at tile row 16 BI (v & 1),
+// tile column 16 BJ (v >> 1).
+TAL_HD inline int quad_row(int bi, int wave) { return 16 * bi * (wave & 1); }
+TAL_HD inline int quad_col(int bj, int wave) { return 16 * bj * (wave >> 1); }
+
 // MFMA 16x16x4 operand layout, read from the staged tiles: in step s of a
 // chunk, lane l takes A(row 16 i + l % 16, index 4 s + l / 16) of block row i
-// and B(index 4 s + l / 16, column 16 j + l % 16) of block column j.
+// and B(index 4 s + l / 16, column 16 j + l % 16) of block column j (rows and
+// columns relative to the wave's quadrant).
 TAL_HD inline int frag_row(int blk, int lane) { return 16 * blk + (lane & 15); }
 TAL_HD inline int frag_k(int step, int lane) { return 4 * step + (lane >> 4); }
 

@@ -9,7 +9,7 @@
 
 namespace tal {
 
-const WaveTile kWaveTiles[4] = {{16, 16}, {16, 32}, {32, 16}, {32, 32}};
+const WaveTile kWaveTiles[4] = {{32, 32}, {32, 64}, {64, 32}, {64, 64}};
 
 namespace {
 
@@ -111,7 +111,7 @@ void batch_task_items(const BatchTaskGeom &g, uint64_t L, uint64_t R, uint64_t D
   it.M = g.M;
   it.N = g.N;
   it.K = g.K;
-  it.shape = (tile.tm / 16 - 1) | ((tile.tn / 16 - 1) << 2) | (accumulate ? 16 : 0) | (g.l_along_k ? 32 : 0) |
+  it.shape = (tile.tm / 32 - 1) | ((tile.tn / 32 - 1) << 2) | (accumulate ? 16 : 0) | (g.l_along_k ? 32 : 0) |
              (g.r_along_k ? 64 : 0);
   it.alpha = alpha;
   it.gm = g.gm;

@@ -32,13 +32,13 @@ struct BatchTaskGeom {
 int batch_task_geometry(const DigitalPattern &p, const int *ddims, const int *ldims, const int *rdims, int esz,
                         BatchTaskGeom *g);
 
-// Candidate wave tiles (rows x columns: 16 or 32 each).
+// Candidate workgroup tiles (rows x columns: 32 or 64 each).
 struct WaveTile {
   int tm, tn;
 };
 extern const WaveTile kWaveTiles[4];
 
-// Items of one contraction for wave tile (tm, tn), appended to `out`.
+// Items of one contraction for workgroup tile (tm, tn), appended to `out`.
 void batch_task_items(const BatchTaskGeom &g, uint64_t L, uint64_t R, uint64_t D, uint64_t lbytes, uint64_t rbytes,
                       uint64_t dbytes, double alpha, bool accumulate, WaveTile tile, std::vector<BatchItem> *out);
 

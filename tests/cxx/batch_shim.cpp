@@ -1,5 +1,5 @@
 // C entry points over the batched-contraction planner (tal_sh_amd/csrc/
-// batch_plan.cpp) and a CPU emulation of the batch kernel's waves, for the
+// batch_plan.cpp) and a CPU emulation of the batch kernel's workgroups, for the
 // CPU test tests/test_batch_plan.py, which builds this file together with
 // batch_plan.cpp and plan.cpp using g++.
 //
@@ -40,76 +40,87 @@ void store(void *base, uint32_t bytes, uint32_t off, S v, long long *bad, int *c
   if (off < tal::kOutside) ++*bad;
 }
 
-// One wave computing one item, lane by lane: every chunk of kBatchKC
-// contracted indices is loaded into the staged tiles with the kernel's lane
-// mapping (each tile entry written exactly once per chunk, checked), then
-// read back as MFMA fragments.
+// One workgroup computing one item, thread by thread: every chunk of
+// kBatchKC contracted indices is loaded into the staged tiles with the
+// kernel's thread mapping (each tile entry written exactly once per chunk,
+// checked), then each wave reads its quadrant's MFMA fragments.
 template <typename S>
 void emulate_item(const tal::BatchItem &it, long long *bad, int *count) {
   using namespace tal;
-  const int BI = item_bi(it), BJ = item_bj(it), TA = 16 * BI, TB = 16 * BJ;
+  constexpr int NT = 256;
+  const int BI = item_bi(it), BJ = item_bj(it), TA = 32 * BI, TB = 32 * BJ;
   const bool ak = item_a_along_k(it), bk = item_b_along_k(it);
   const void *L = (const void *)it.L, *R = (const void *)it.R;
   void *D = (void *)it.D;
-  std::vector<S> C((size_t)BI * BJ * 256, S(0));  // block (i, j): C[((i * BJ + j) * 16 + row) * 16 + col]
+  // C[wave][block i][block j][row][col]
+  std::vector<S> C((size_t)4 * BI * BJ * 256, S(0));
+  auto cidx = [&](int w, int i, int j, int row, int col) {
+    return ((((size_t)w * BI + i) * BJ + j) * 16 + row) * 16 + col;
+  };
   const int nch = (it.K + kBatchKC - 1) / kBatchKC;
   for (int ch = 0; ch < nch; ++ch) {
     const int k0 = ch * kBatchKC;
-    S a[32][kBatchKC], b[32][kBatchKC];
-    int wa[32][kBatchKC] = {}, wb[32][kBatchKC] = {};
-    for (int lane = 0; lane < 64; ++lane) {
-      for (int j = 0; j < TA * kBatchKC / 64; ++j) {
+    S a[64][kBatchKC], b[64][kBatchKC];
+    int wa[64][kBatchKC] = {}, wb[64][kBatchKC] = {};
+    for (int t = 0; t < NT; ++t) {
+      for (int j = 0; j < TA * kBatchKC / NT; ++j) {
         int r, c;
-        stage_coords(ak, TA, lane, j, &r, &c);
+        stage_coords(ak, TA, NT, t, j, &r, &c);
         uint32_t kl, kr;
         k_offsets(it, k0 + c, &kl, &kr);
         a[r][c] = load<S>(L, it.lbytes, row_offset(it, it.m0 + r, 0) + kl, bad);
         ++wa[r][c];
       }
-      for (int j = 0; j < TB * kBatchKC / 64; ++j) {
+      for (int j = 0; j < TB * kBatchKC / NT; ++j) {
         int r, c;
-        stage_coords(bk, TB, lane, j, &r, &c);
+        stage_coords(bk, TB, NT, t, j, &r, &c);
         uint32_t kl, kr;
         k_offsets(it, k0 + c, &kl, &kr);
         b[r][c] = load<S>(R, it.rbytes, kr + col_offset(it, it.n0 + r, 0), bad);
         ++wb[r][c];
       }
     }
-    for (int r = 0; r < 32; ++r)
+    for (int r = 0; r < 64; ++r)
       for (int c = 0; c < kBatchKC; ++c) {
-        if ((r < TA) != (wa[r][c] == 1) || (r >= TA && wa[r][c] != 0)) ++*bad;
-        if ((r < TB) != (wb[r][c] == 1) || (r >= TB && wb[r][c] != 0)) ++*bad;
+        if (wa[r][c] != (r < TA ? 1 : 0)) ++*bad;
+        if (wb[r][c] != (r < TB ? 1 : 0)) ++*bad;
       }
-    // MFMA 16x16x4 steps: A(row, k) in lane row + 16 k, B(k, col) in lane col + 16 k
-    for (int st = 0; st < kBatchKC / 4; ++st) {
-      S fa[2][64], fb[2][64];
-      for (int lane = 0; lane < 64; ++lane) {
-        for (int i = 0; i < BI; ++i) fa[i][lane] = a[frag_row(i, lane)][frag_k(st, lane)];
-        for (int j = 0; j < BJ; ++j) fb[j][lane] = b[frag_row(j, lane)][frag_k(st, lane)];
+    // MFMA 16x16x4 steps per wave: A(row, k) in lane row + 16 k, B(k, col) in lane col + 16 k
+    for (int w = 0; w < 4; ++w) {
+      const int qr = quad_row(BI, w), qc = quad_col(BJ, w);
+      for (int st = 0; st < kBatchKC / 4; ++st) {
+        S fa[2][64], fb[2][64];
+        for (int lane = 0; lane < 64; ++lane) {
+          for (int i = 0; i < BI; ++i) fa[i][lane] = a[qr + frag_row(i, lane)][frag_k(st, lane)];
+          for (int j = 0; j < BJ; ++j) fb[j][lane] = b[qc + frag_row(j, lane)][frag_k(st, lane)];
+        }
+        for (int i = 0; i < BI; ++i)
+          for (int j = 0; j < BJ; ++j)
+            for (int row = 0; row < 16; ++row)
+              for (int col = 0; col < 16; ++col) {
+                S s = C[cidx(w, i, j, row, col)];
+                for (int k = 0; k < 4; ++k) s += fa[i][row + 16 * k] * fb[j][col + 16 * k];
+                C[cidx(w, i, j, row, col)] = s;
+              }
       }
-      for (int i = 0; i < BI; ++i)
-        for (int j = 0; j < BJ; ++j)
-          for (int row = 0; row < 16; ++row)
-            for (int col = 0; col < 16; ++col) {
-              S s = C[((size_t)(i * BJ + j) * 16 + row) * 16 + col];
-              for (int k = 0; k < 4; ++k) s += fa[i][row + 16 * k] * fb[j][col + 16 * k];
-              C[((size_t)(i * BJ + j) * 16 + row) * 16 + col] = s;
-            }
     }
   }
   const S alpha = (S)it.alpha;
-  for (int lane = 0; lane < 64; ++lane)
-    for (int i = 0; i < BI; ++i)
-      for (int r = 0; r < 4; ++r) {
-        const int row = acc_row((int)sizeof(S), lane, r);
-        const uint32_t drow = row_offset(it, it.m0 + 16 * i + row, 1);
-        for (int j = 0; j < BJ; ++j) {
-          const uint32_t off = drow + col_offset(it, it.n0 + frag_row(j, lane), 1);
-          S v = alpha * C[((size_t)(i * BJ + j) * 16 + row) * 16 + (lane & 15)];
-          if (item_accumulates(it)) v = load<S>(D, it.dbytes, off, bad) + v;
-          store<S>(D, it.dbytes, off, v, bad, count);
+  for (int w = 0; w < 4; ++w) {
+    const int qr = quad_row(BI, w), qc = quad_col(BJ, w);
+    for (int lane = 0; lane < 64; ++lane)
+      for (int i = 0; i < BI; ++i)
+        for (int r = 0; r < 4; ++r) {
+          const int row = acc_row((int)sizeof(S), lane, r);
+          const uint32_t drow = row_offset(it, it.m0 + qr + 16 * i + row, 1);
+          for (int j = 0; j < BJ; ++j) {
+            const uint32_t off = drow + col_offset(it, it.n0 + qc + frag_row(j, lane), 1);
+            S v = alpha * C[cidx(w, i, j, row, lane & 15)];
+            if (item_accumulates(it)) v = load<S>(D, it.dbytes, off, bad) + v;
+            store<S>(D, it.dbytes, off, v, bad, count);
+          }
         }
-      }
+  }
 }
 
 }  // namespace
@@ -138,11 +149,12 @@ long long batch_divider_check(uint32_t dmax, uint32_t nmax) {
 }
 
 // Plans D (+)= alpha * L * R with wave tile tm x tn and runs every item on the
-// CPU emulation.  Returns batch_task_geometry's code (0 = planned and run);
+// CPU emulation.  Returns batch_task_geometry's code (0 = planned and run; 3 = not a kernel tile);
 // *bad = addressing errors, count[e] = stores to D element e, *items = item count.
 int batch_emulate(const char *pattern, const int *ddims, const int *ldims, const int *rdims, int esz, const void *L,
                   const void *R, void *D, double alpha, int accumulate, int tm, int tn, long long *bad, int *count,
                   long long *items, int *ndims) {
+  if ((tm != 32 && tm != 64) || (tn != 32 && tn != 64)) return 3;  // not a workgroup tile of the kernel
   tal::DigitalPattern p;
   if (tal::parse_pattern(pattern, &p) != 0) return 1;
   tal::BatchTaskGeom g;

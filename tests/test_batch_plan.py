@@ -8,10 +8,11 @@ returns 0.  These tests build the real planner with g++ (tests/cxx/
 batch_shim.cpp) and check, without a GPU:
   - the multiply-high division against integer division, exhaustively for
     divisors up to 4096 (the digit decomposition rests on it);
-  - a lane-by-lane emulation of the kernel's waves (same address functions
-    and staging lane maps, MFMA operand / accumulator layouts, out-of-range
-    loads read 0 and stores are dropped) reproduces the oracle's contraction
-    for config-4-like contractions and every wave tile, with no offset
+  - a thread-by-thread emulation of the kernel's workgroups (same address
+    functions and staging maps, MFMA operand / accumulator layouts of each
+    wave's quadrant, out-of-range loads read 0 and stores are dropped)
+    reproduces the oracle's contraction for config-4-like contractions and
+    every workgroup tile, with no offset
     between an operand's end and the out-of-range marker, every staged tile
     entry written exactly once per chunk and every D element stored once;
   - groups that cannot be fused into three dimensions, and hyper indices, are
@@ -31,7 +32,7 @@ import pytest
 from tal_common import R4, R8, batch_contraction, tensor
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-WAVE_TILES = ((16, 16), (16, 32), (32, 16), (32, 32))
+TILES = ((32, 32), (32, 64), (64, 32), (64, 64))
 
 
 @pytest.fixture(scope="module")
@@ -85,7 +86,7 @@ def test_emulated_kernel_matches_oracle(shim, oracle, dk):
         alpha = 0.75 if t % 2 else 1.0
         want = oracle.contract(dk, pat, dd, D0, ld, L, rd, R, (alpha, 0.0), 1 if acc else 0)
         den = np.linalg.norm(want.astype(np.float64)) or 1.0
-        for tile in WAVE_TILES:
+        for tile in TILES:
             rc, got, count, bad, items, nd = emulate(shim, pat, dd, ld, rd, dk, L, R, D0, alpha, 1 if acc else 0,
                                                     tile)
             assert rc == 0, (pat, dd, ld, rd, rc)
@@ -114,7 +115,7 @@ def test_emulated_edge_shapes(shim, oracle):
             L, R, D0 = tensor(dk, ld, 50 + t), tensor(dk, rd, 60 + t), tensor(dk, dd, 70 + t)
             want = oracle.contract(dk, pat, dd, D0, ld, L, rd, R, (1.0, 0.0), 1)
             den = np.linalg.norm(want.astype(np.float64)) or 1.0
-            for tile in WAVE_TILES:
+            for tile in TILES:
                 rc, got, count, bad, items, nd = emulate(shim, pat, dd, ld, rd, dk, L, R, D0, 1.0, 1, tile)
                 assert rc == 0 and bad == 0, (pat, tile, rc, bad)
                 assert (count == 1).all(), (pat, tile)
@@ -122,7 +123,7 @@ def test_emulated_edge_shapes(shim, oracle):
     # rows a, b adjacent in L and in D fuse into one dimension
     L, R, D0 = tensor(R8, [4, 5, 7], 1), tensor(R8, [6, 7, 3], 2), tensor(R8, [4, 5, 6, 3], 3)
     rc, _, _, _, _, nd = emulate(shim, "D(a,b,c,d)+=L(a,b,i)*R(c,i,d)", [4, 5, 6, 3], [4, 5, 7], [6, 7, 3], R8, L, R,
-                                 D0, 1.0, 1, (16, 16))
+                                 D0, 1.0, 1, (32, 32))
     assert rc == 0 and nd == [1, 2, 1], nd
 
 
@@ -130,15 +131,15 @@ def test_geometry_refusals(shim):
     # four row dimensions that chain in L but not in D: no three-dimension form
     dd, ld, rd = [3, 4, 2, 2], [2, 3, 2, 4, 5], [5]
     L, R, D = np.zeros(240), np.zeros(5), np.zeros(48)
-    rc, *_ = emulate(shim, "D(b,d,a,c)+=L(a,b,c,d,i)*R(i)", dd, ld, rd, R8, L, R, D, 1.0, 0, (16, 16))
+    rc, *_ = emulate(shim, "D(b,d,a,c)+=L(a,b,c,d,i)*R(i)", dd, ld, rd, R8, L, R, D, 1.0, 0, (32, 32))
     assert rc == 2
     # a hyper index (in D, L and R)
     rc, *_ = emulate(shim, "D(a,b,h)+=L(a,i,h)*R(i,b,h)", [3, 4, 2], [3, 5, 2], [5, 4, 2], R8, np.zeros(30),
-                     np.zeros(40), np.zeros(24), 1.0, 0, (16, 16))
+                     np.zeros(40), np.zeros(24), 1.0, 0, (32, 32))
     assert rc == 2
     # extents that do not match the pattern
     rc, *_ = emulate(shim, "D(a,b)+=L(a,i)*R(i,b)", [3, 4], [3, 5], [6, 4], R8, np.zeros(15), np.zeros(24),
-                     np.zeros(12), 1.0, 0, (16, 16))
+                     np.zeros(12), 1.0, 0, (32, 32))
     assert rc == 1
 
 
@@ -148,7 +149,7 @@ def test_queues_place_every_item_once(shim):
     M = [rng.choice([1, 3, 16, 40, 200, 1500]) for _ in range(ntask)]
     N = [rng.choice([1, 7, 16, 33, 300, 2000]) for _ in range(ntask)]
     K = [rng.randint(1, 300) for _ in range(ntask)]
-    tiles = [WAVE_TILES[rng.randrange(len(WAVE_TILES))] for _ in range(ntask)]
+    tiles = [TILES[rng.randrange(len(TILES))] for _ in range(ntask)]
     want = set()
     for t in range(ntask):
         for n0 in range(0, N[t], tiles[t][1]):
