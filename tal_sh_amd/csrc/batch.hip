@@ -86,7 +86,7 @@ struct TileLds {
 // does not come from memory.
 template <typename S, int BI, int BJ, bool AK, bool BK>
 __device__ __forceinline__ void item_body(const BatchItem &it, int tid, const BatchItem *__restrict__ items,
-                                          unsigned next, int base, int end, TileLds<S> &lds) {
+                                          unsigned next, int base, int end, TileLds<S> &lds, int *pos_slot) {
   using W = Wave<S>;
   using acc_t = typename W::acc_t;
   constexpr int TA = 32 * BI, TB = 32 * BJ;                             // tile rows, tile columns
@@ -150,9 +150,13 @@ __device__ __forceinline__ void item_body(const BatchItem &it, int tid, const Ba
 #pragma unroll
     for (int j = 0; j < BJ; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
   const int qr = quad_row(BI, wave), qc = quad_col(BJ, wave);
-  auto compute = [&](int buf) {
+  // a quadrant wholly past the tile's last row or column has nothing to compute
+  const bool busy = it.m0 + qr < it.M && it.n0 + qc < it.N;
+  auto compute = [&](int buf, int nsteps) {  // the chunk's first nsteps MFMA steps (the rest are past K)
+    if (!busy) return;
 #pragma unroll
     for (int s = 0; s < kBatchKC / 4; ++s) {
+      if (s >= nsteps) break;  // uniform over the wave
       S fa[BI], fb[BJ];
 #pragma unroll
       for (int i = 0; i < BI; ++i) fa[i] = lds.a[buf][qr + frag_row(i, lane)][frag_k(s, lane)];
@@ -189,10 +193,13 @@ __device__ __forceinline__ void item_body(const BatchItem &it, int tid, const Ba
           pf3 = nx.gk.s[0][0];
         }
         __syncthreads();
-        compute(c & 1);
+        compute(c & 1, (it.K - c * kBatchKC + 3) / 4);
       }
     }
   }
+  // Thread 0 publishes the next queue position now, while no store is
+  // outstanding: waiting for the fetch later would also wait for the stores.
+  if (tid == 0) *pos_slot = (int)next;
   const S alpha = (S)it.alpha;
   const bool accumulate = item_accumulates(it);
   uint32_t dcol[BJ];
@@ -204,16 +211,15 @@ __device__ __forceinline__ void item_body(const BatchItem &it, int tid, const Ba
 #pragma unroll
     for (int r = 0; r < 4; ++r)
       drow[r] = row_offset(it, it.m0 + qr + 16 * i + acc_row((int)sizeof(S), lane, r), 1);
+    S dv[4][BJ];  // an accumulating item reads all of the block row's D first
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < 4; ++r)
 #pragma unroll
-      for (int j = 0; j < BJ; ++j) {
-        const uint32_t off = drow[r] + dcol[j];
-        S v = alpha * acc[i][j][r];
-        if (accumulate) v = W::load(rd, off) + v;
-        W::store(rd, off, v);
-      }
-    }
+      for (int j = 0; j < BJ; ++j) dv[r][j] = accumulate ? W::load(rd, drow[r] + dcol[j]) : (S)0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int j = 0; j < BJ; ++j) W::store(rd, drow[r] + dcol[j], dv[r][j] + alpha * acc[i][j][r]);
   }
   asm volatile("" ::"s"(pf0), "s"(pf1), "s"(pf2), "s"(pf3));
 }
@@ -259,7 +265,7 @@ __global__ __launch_bounds__(kThreads) void batch_contract_kernel(const BatchIte
     // (BI - 1) | (BJ - 1) << 2 | along-k bits of L and R, uniform over the workgroup
 #define TAL_ITEM(BI, BJ, AK, BK)                                         \
   case (BI - 1) | (BJ - 1) << 2 | AK << 5 | BK << 6:                     \
-    item_body<S, BI, BJ, AK, BK>(it, tid, items, next, base, end, lds); \
+    item_body<S, BI, BJ, AK, BK>(it, tid, items, next, base, end, lds, &s_pos[slot]); \
     break;
 #define TAL_ITEMS(AK, BK) TAL_ITEM(1, 1, AK, BK) TAL_ITEM(2, 1, AK, BK) TAL_ITEM(1, 2, AK, BK) TAL_ITEM(2, 2, AK, BK)
     switch (it.shape & 0x6f) {
@@ -267,12 +273,14 @@ __global__ __launch_bounds__(kThreads) void batch_contract_kernel(const BatchIte
       TAL_ITEMS(1, 0)
       TAL_ITEMS(0, 1)
       TAL_ITEMS(1, 1)
-      default: break;  // not produced by the planner
+      default:  // not produced by the planner: skipped
+        if (tid == 0) s_pos[slot] = (int)next;
+        break;
     }
 #undef TAL_ITEMS
 #undef TAL_ITEM
-    // every wave has finished the item's LDS reads before the next item stages into LDS
-    if (tid == 0) s_pos[slot] = (int)next;
+    // every wave has finished the item's LDS reads before the next item stages
+    // into LDS, and the next position (published by the item) is visible
     __syncthreads();
     cur = s_pos[slot];
   }

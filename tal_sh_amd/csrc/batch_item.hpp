@@ -40,8 +40,8 @@ constexpr uint64_t kBatchMaxBytes = kOutside;
 // Contracted indices per LDS chunk (four MFMA steps of 4).
 constexpr int kBatchKC = 16;
 // Chunks whose loads a workgroup keeps in flight ahead of the chunk its MFMAs use.
-constexpr int kBatchDepthF64 = 2;
-constexpr int kBatchDepthF32 = 2;
+constexpr int kBatchDepthF64 = 1;
+constexpr int kBatchDepthF32 = 1;
 
 // Division of n < 2^31 by a fixed d >= 1 with a multiply-high (round-up
 // method of Granlund and Montgomery, PLDI 1994): with l = ceil(log2 d) and

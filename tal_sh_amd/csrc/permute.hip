@@ -580,8 +580,11 @@ T make_val(double vr, double vi) {
 // fully, the problem is a set of rows: a run of L elements contiguous on both
 // sides (L = 1 when the first remaining dimension is strided in the big
 // tensor), enumerated by the remaining row dimensions.  A block covers
-// `ry` rows x (rx * V) run elements; each thread decomposes its row index
-// once and moves V elements per access.
+// kSliceRows x `ry` rows x (rx * V) run elements: each thread moves V
+// elements of kSliceRows rows (all loads issued before the stores), and
+// decomposes each row index with 32-bit arithmetic when the row count allows
+// (the last row digit needs no division).
+constexpr int kSliceRows = 4;
 struct SliceArgs {
   int nrd;           // row dimensions
   int rx_log;        // log2 of threads along the run
@@ -591,25 +594,50 @@ struct SliceArgs {
   int64_t re[kMaxRank], rss[kMaxRank], rds[kMaxRank];  // row extents, source / destination strides
 };
 
-template <typename T, int V, int OP>
+template <typename R>
+__device__ inline void slice_row(const SliceArgs &p, R r, int64_t *so, int64_t *dof) {
+  for (int k = 0; k + 1 < p.nrd; ++k) {
+    const R q = r / (R)p.re[k];
+    const R i = r - q * (R)p.re[k];
+    r = q;
+    *so += (int64_t)i * p.rss[k];
+    *dof += (int64_t)i * p.rds[k];
+  }
+  if (p.nrd > 0) {
+    *so += (int64_t)r * p.rss[p.nrd - 1];
+    *dof += (int64_t)r * p.rds[p.nrd - 1];
+  }
+}
+
+template <typename T, int V, int OP, bool NARROW>
 __global__ __launch_bounds__(kThreads) void slice_kernel(const T *__restrict__ src, T *__restrict__ dst, SliceArgs p) {
   const int rx = 1 << p.rx_log, ry = kThreads >> p.rx_log;
   const int tx = threadIdx.x & (rx - 1), ty = threadIdx.x >> p.rx_log;
-  int64_t blk = blockIdx.x;
-  const int64_t chunk = blk % p.nchunk;
-  blk /= p.nchunk;
-  int64_t r = blk * ry + ty;
+  const int64_t chunk = (int64_t)blockIdx.x % p.nchunk, blk = (int64_t)blockIdx.x / p.nchunk;
   const int64_t e = (chunk * rx + tx) * V;
-  if (r >= p.nrows || e >= p.L) return;
-  int64_t so = p.src0 + e, dof = p.dst0 + e;
-  for (int k = 0; k < p.nrd; ++k) {
-    const int64_t q = r / p.re[k];
-    const int64_t i = r - q * p.re[k];
-    r = q;
-    so += i * p.rss[k];
-    dof += i * p.rds[k];
+  if (e >= p.L) return;
+  int64_t so[kSliceRows], dof[kSliceRows];
+  bool ok[kSliceRows];
+#pragma unroll
+  for (int i = 0; i < kSliceRows; ++i) {
+    const int64_t r = (blk * kSliceRows + i) * ry + ty;
+    ok[i] = r < p.nrows;
+    so[i] = p.src0 + e;
+    dof[i] = p.dst0 + e;
+    if (ok[i]) {
+      if (NARROW)
+        slice_row<uint32_t>(p, (uint32_t)r, &so[i], &dof[i]);
+      else
+        slice_row<int64_t>(p, r, &so[i], &dof[i]);
+    }
   }
-  emit<T, V, OP>(dst + dof, ldv<T, V>(src + so), (typename Real<T>::type)1, (typename Real<T>::type)0, 0);
+  Pack<T, V> v[kSliceRows];
+#pragma unroll
+  for (int i = 0; i < kSliceRows; ++i)
+    if (ok[i]) v[i] = ldv<T, V>(src + so[i]);
+#pragma unroll
+  for (int i = 0; i < kSliceRows; ++i)
+    if (ok[i]) emit<T, V, OP>(dst + dof[i], v[i], (typename Real<T>::type)1, (typename Real<T>::type)0, 0);
 }
 
 template <typename T>
@@ -667,15 +695,24 @@ int launch_slice_typed(int mode, const T *src, T *dst, int rank, const int *sdim
   p.rx_log = rx_log;
   const int64_t rx = (int64_t)1 << rx_log, ry = kThreads / rx;
   p.nchunk = (nvec + rx - 1) / rx;
-  const int64_t nblk = p.nchunk * ((p.nrows + ry - 1) / ry);
+  const int64_t nblk = p.nchunk * ((p.nrows + ry * kSliceRows - 1) / (ry * kSliceRows));
   if (nblk > 0x7fffffffLL) return -4;
-#define TAL_SLICE_LAUNCH(VV)                                                                                   \
-  (accumulative ? launch_clocked(slice_kernel<T, VV, PERM_ADD>, dim3((unsigned)nblk), dim3(kThreads), s, src, dst, p)   \
-                : launch_clocked(slice_kernel<T, VV, PERM_COPY>, dim3((unsigned)nblk), dim3(kThreads), s, src, dst, p))
-  if (V > 1)
-    TAL_SLICE_LAUNCH(VW);
-  else
-    TAL_SLICE_LAUNCH(1);
+  const bool narrow = p.nrows <= 0xffffffffLL;  // row indices and extents fit 32 bits
+#define TAL_SLICE_LAUNCH(VV, NW)                                                                                   \
+  (accumulative                                                                                                  \
+       ? launch_clocked(slice_kernel<T, VV, PERM_ADD, NW>, dim3((unsigned)nblk), dim3(kThreads), s, src, dst, p) \
+       : launch_clocked(slice_kernel<T, VV, PERM_COPY, NW>, dim3((unsigned)nblk), dim3(kThreads), s, src, dst, p))
+  if (V > 1) {
+    if (narrow)
+      TAL_SLICE_LAUNCH(VW, true);
+    else
+      TAL_SLICE_LAUNCH(VW, false);
+  } else {
+    if (narrow)
+      TAL_SLICE_LAUNCH(1, true);
+    else
+      TAL_SLICE_LAUNCH(1, false);
+  }
 #undef TAL_SLICE_LAUNCH
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
