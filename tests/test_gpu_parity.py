@@ -177,6 +177,32 @@ def test_contract_config1_r8(talsh_gpu, oracle):
     assert e <= 1e-12
 
 
+@pytest.mark.parametrize("dk,n", ((R8, 96), (R4, 96), (C8, 48)))
+def test_contract_config3_large(talsh_gpu, oracle, dk, n):
+    """Config 3's pattern D(a,b,c,d)+=L(a,i,c,j)*R(j,b,i,d) at its full extents 96 (R8, R4: GEMM M = N = K = 9216)
+    and at 48 for C8, alpha 1, accumulative; 256 sampled elements of D, each an n^2-term dot product the oracle
+    computes in long double, checked normwise: <= 1e-12 (R8, C8), <= 1e-5 (R4, FP32 MFMA)."""
+    ts = talsh_gpu
+    pat = "D(a,b,c,d)+=L(a,i,c,j)*R(j,b,i,d)"
+    dims = [n] * 4
+    say("config3 contraction", dk, n)
+    L, R, D = tensor(dk, dims, 21), tensor(dk, dims, 22), tensor(dk, dims, 23)
+    lt, rt, dt = on_gpu(ts, dk, dims, L), on_gpu(ts, dk, dims, R), on_gpu(ts, dk, dims, D)
+    ts.check(ts.contract(pat, dt, lt, rt, alpha=1.0), "contract")
+    got = dt.host_array()
+    idx = np.random.default_rng(7).choice(got.size, size=256, replace=False)
+    num = den = 0.0
+    for off in idx:
+        want = complex(D[off]) + oracle.contract_element(dk, pat, dims, int(off), dims, L, dims, R)
+        num += abs(complex(got[off]) - want) ** 2
+        den += abs(want) ** 2
+    e = (num / den) ** 0.5
+    print("config3 %d^4 kind %d sampled normwise relative error %.3e" % (n, dk, e))
+    assert e <= (1e-12 if dk in (R8, C8) else 1e-5), e
+    for t in (lt, rt, dt):
+        t.destruct()
+
+
 def test_contract_random_all_kinds(talsh_gpu, oracle):
     ts = talsh_gpu
     rng = random.Random(99)
