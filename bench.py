@@ -85,14 +85,16 @@ def timed_contractions(pat, d, l, r, steps, warmup, alpha=1.0):
     return wall, mm, tot
 
 
-def pipelined_contractions(pat, d, l, r, steps, alpha=1.0):
-    """Wall time of `steps` contractions submitted back to back as tasks and waited at the end: the host's
-    launch work overlaps the GPU's, and the runtime orders the steps through D (each reads and writes it)."""
+def pipelined_contractions(pat, dk, dims, l, r, steps, alpha=1.0):
+    """Wall time of `steps` contractions submitted back to back as tasks and waited at the end, so that the
+    host's launch work overlaps the GPU's.  Each step writes its own D (an output image is unavailable while a
+    task writes it); L and R are shared."""
     lib = ts.load()
+    outs = [gpu_tensor(dk, dims, 1000 + i)[0] for i in range(steps)]
     lib.talshAmdDeviceSynchronize(0)
     tasks = []
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for d in outs:
         tk = ts.Task()
         ts.check(ts.contract(pat, d, l, r, alpha=alpha, task=tk), "contract")
         tasks.append(tk)
@@ -102,6 +104,8 @@ def pipelined_contractions(pat, d, l, r, steps, alpha=1.0):
     wall = time.perf_counter() - t0
     for tk in tasks:
         tk.destruct()
+    for d in outs:
+        d.destruct()
     return wall
 
 
@@ -134,11 +138,10 @@ def bench_contraction(oracle, dk, pat, n, steps, warmup, check=True, seed=100, p
            "gemm_ms": 1e3 * float(np.mean(mm)), "gemm_tflops": flop / float(np.mean(mm)) / 1e12,
            "task_ms": 1e3 * float(np.mean(tot)), "flop_per_step": flop}
     reps = warmup + steps
-    if pipelined:  # the same steps submitted back to back (host launch work overlapped)
-        pwall = pipelined_contractions(pat, d, l, r, steps)
+    if pipelined:  # the same contraction submitted back to back into separate outputs
+        pwall = pipelined_contractions(pat, dk, dims, l, r, steps)
         res["pipelined_tflops"] = flop * steps / pwall / 1e12
         res["pipelined_ms_per_step"] = 1e3 * pwall / steps
-        reps += steps
     if check:
         # D now holds D0 + reps * L*R: check against the oracle on samples
         out = d.host_array()

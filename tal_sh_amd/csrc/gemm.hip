@@ -105,7 +105,7 @@ __device__ inline void tile_of(int64_t gx, int64_t gy, int64_t t0, int64_t count
 
 // layout codes: 0 = N, 1 = T; CJA/CJB conjugate the complex operand; W is the
 // vector width in scalars along each operand's contiguous axis.
-template <typename S, int NC, int OPA, int OPB, int CJA, int CJB, int BM, int BN, int W, int NW>
+template <typename S, int NC, int OPA, int OPB, int CJA, int CJB, int BM, int BN, int W, int NW, int KB>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2))) void gemm_kernel(
                                                         int64_t M, int64_t N, int64_t K, const S *__restrict__ A,
                                                         int64_t lda, const S *__restrict__ B, int64_t ldb,
@@ -129,11 +129,11 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2)
   constexpr int NT = 64 * NW, WBN = BN / (NW / 2);
   constexpr int SM = BM / 32, SN = WBN / 16;  // 16x16 tiles per wave along M, N
   constexpr int PAD = W;                     // keeps rows 16-byte aligned
-  constexpr int RS = BK + PAD;               // LDS row stride (scalars)
-  // A tile: BM x BK elements (BM*BK*NC scalars) moved in vectors of W scalars
+  constexpr int RS = KB + PAD;               // LDS row stride (scalars)
+  // A tile: BM x KB elements (BM*KB*NC scalars) moved in vectors of W scalars
   // along the contiguous axis (m for 'N', k for 'T'; re/im interleaved).
-  constexpr int ACONT = (OPA == 0 ? BM : BK) * NC, BCONT = (OPB == 0 ? BK : BN) * NC;
-  constexpr int AVEC = BM * BK * NC / W / NT, BVEC = BN * BK * NC / W / NT;
+  constexpr int ACONT = (OPA == 0 ? BM : KB) * NC, BCONT = (OPB == 0 ? KB : BN) * NC;
+  constexpr int AVEC = BM * KB * NC / W / NT, BVEC = BN * KB * NC / W / NT;
   static_assert(AVEC >= 1 && BVEC >= 1, "tile too small");
   constexpr int ATPL = ACONT / W, BTPL = BCONT / W;  // threads per contiguous line
   static_assert(NT % ATPL == 0 && NT % BTPL == 0, "line split");
@@ -227,18 +227,18 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2)
 
   const S sa = CJA ? (S)-1 : (S)1;  // sign of A's imaginary part
   const S sb = CJB ? (S)-1 : (S)1;
-  const int64_t nk = (kend - kbeg + BK - 1) / BK;
+  const int64_t nk = (kend - kbeg + KB - 1) / KB;
 
   load_tiles(kbeg);
   store_tiles(0);
   __syncthreads();
   for (int64_t kt = 0; kt < nk; ++kt) {
     const int buf = (int)(kt & 1);
-    if (kt + 1 < nk) load_tiles(kbeg + (kt + 1) * BK);
+    if (kt + 1 < nk) load_tiles(kbeg + (kt + 1) * KB);
     // interleave the LDS fragment reads with the MFMAs (+2 % on FP64, FP32 and complex)
     __builtin_amdgcn_iglp_opt(1);
 #pragma unroll
-    for (int ks = 0; ks < BK / 4; ++ks) {
+    for (int ks = 0; ks < KB / 4; ++ks) {
       const int kk = ks * 4 + (lane >> 4);
       S fa[NC][SM], fb[NC][SN];
 #pragma unroll
@@ -1012,7 +1012,7 @@ int launch_bf16x6(int64_t M, int64_t N, int64_t K, const void *A, int64_t lda, c
 
 // Workgroups of gemm_kernel<S, NC, ..., BM, BN> resident at once on the
 // current device (CUs x blocks per CU), cached per device.
-template <typename S, int NC, int BM, int BN, int NWS>
+template <typename S, int NC, int BM, int BN, int NWS, int KB>
 int64_t gemm_slots() {
   static int64_t cache[64] = {0};
   int dev = 0;
@@ -1021,7 +1021,7 @@ int64_t gemm_slots() {
     int cus = 0, per = 0;
     constexpr int WV = (int)(16 / sizeof(S));
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, gemm_kernel<S, NC, 1, 0, 0, 0, BM, BN, WV, NWS>,
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, gemm_kernel<S, NC, 1, 0, 0, 0, BM, BN, WV, NWS, KB>,
                                                      64 * NWS, 0) !=
             hipSuccess)
       return 0;
@@ -1038,15 +1038,15 @@ struct TailSplit {
   int64_t rem = 0;
   int ksplit = 1;
 };
-template <typename S, int NC, int BM, int BN, int NWS>
+template <typename S, int NC, int BM, int BN, int NWS, int KB>
 TailSplit tail_split(int64_t M, int64_t N, int64_t K, int64_t batch) {
   TailSplit t;
   if (batch != 1 || M <= 0 || N <= 0) return t;
-  const int64_t T = ((M + BM - 1) / BM) * ((N + BN - 1) / BN), slots = gemm_slots<S, NC, BM, BN, NWS>();
+  const int64_t T = ((M + BM - 1) / BM) * ((N + BN - 1) / BN), slots = gemm_slots<S, NC, BM, BN, NWS, KB>();
   if (slots <= 0) return t;
   const int64_t rem = T % slots;
   if (rem == 0 || rem * 4 > slots) return t;
-  const int64_t ks = std::min<int64_t>(std::min<int64_t>(16, slots / rem), ((K + BK - 1) / BK) / 8);
+  const int64_t ks = std::min<int64_t>(std::min<int64_t>(16, slots / rem), ((K + KB - 1) / KB) / 8);
   if (ks < 2) return t;
   t.rem = rem;
   t.ksplit = (int)ks;
@@ -1057,7 +1057,7 @@ size_t tail_bytes(const TailSplit &t) {
   return t.ksplit > 1 ? (size_t)t.rem * t.ksplit * BM * BN * NC * sizeof(S) : 0;
 }
 
-template <typename S, int NC, int OPA, int OPB, int CJA, int CJB, int BM, int BN, int NWS>
+template <typename S, int NC, int OPA, int OPB, int CJA, int CJB, int BM, int BN, int NWS, int KB>
 int launch_typed(int64_t M, int64_t N, int64_t K, const void *A, int64_t lda, const void *B, int64_t ldb,
                  void *C, int64_t ldc, double ar, double ai, int beta_one, int64_t batch, int64_t sa, int64_t sb,
                  int64_t sc, void *ws, size_t ws_bytes, const int64_t *cmap, hipStream_t s) {
@@ -1070,10 +1070,10 @@ int launch_typed(int64_t M, int64_t N, int64_t K, const void *A, int64_t lda, co
   const bool vec = ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0) && acont % WV == 0 &&
                    bcont % WV == 0 && (lda * NC) % WV == 0 && (ldb * NC) % WV == 0 &&
                    (batch == 1 || ((sa * NC) % WV == 0 && (sb * NC) % WV == 0));
-  TailSplit ts = tail_split<S, NC, BM, BN, NWS>(M, N, K, batch);
+  TailSplit ts = tail_split<S, NC, BM, BN, NWS, KB>(M, N, K, batch);
   if (!ws || tail_bytes<S, NC, BM, BN>(ts) > ws_bytes) ts = TailSplit{};
   const int64_t full = nblk - ts.rem;  // tiles computed whole, in the first launch
-  const int64_t kchunk = ts.ksplit > 1 ? (((K + BK - 1) / BK + ts.ksplit - 1) / ts.ksplit) * BK : 0;
+  const int64_t kchunk = ts.ksplit > 1 ? (((K + KB - 1) / KB + ts.ksplit - 1) / ts.ksplit) * KB : 0;
   // batch entries go to grid.y, at most 65535 per launch
   for (int64_t b0 = 0; b0 < batch; b0 += 65535) {
     const unsigned nb = (unsigned)std::min<int64_t>(65535, batch - b0);
@@ -1083,19 +1083,19 @@ int launch_typed(int64_t M, int64_t N, int64_t K, const void *A, int64_t lda, co
       const dim3 grid((unsigned)full, nb);
       const int64_t bsa = sa * NC, bsb = sb * NC, bsc = sc * NC;
       if (batch > 1 && vec)
-        gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, WV, kWavesBatched><<<grid, dim3(64 * kWavesBatched), 0, s>>>(
+        gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, WV, kWavesBatched, KB><<<grid, dim3(64 * kWavesBatched), 0, s>>>(
             M, N, K, a, lda, b, ldb, c, ldc, (S)ar, (S)ai, beta_one, gx, gy, bsa, bsb, bsc, 0, full, 1, 0, nullptr,
             cmap);
       else if (batch > 1)
-        gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, 1, kWavesBatched><<<grid, dim3(64 * kWavesBatched), 0, s>>>(
+        gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, 1, kWavesBatched, KB><<<grid, dim3(64 * kWavesBatched), 0, s>>>(
             M, N, K, a, lda, b, ldb, c, ldc, (S)ar, (S)ai, beta_one, gx, gy, bsa, bsb, bsc, 0, full, 1, 0, nullptr,
             cmap);
       else if (vec)
-        gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, WV, NWS><<<grid, dim3(64 * NWS), 0, s>>>(
+        gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, WV, NWS, KB><<<grid, dim3(64 * NWS), 0, s>>>(
             M, N, K, a, lda, b, ldb, c, ldc, (S)ar, (S)ai, beta_one, gx, gy, bsa, bsb, bsc, 0, full, 1, 0, nullptr,
             cmap);
       else
-        gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, 1, NWS><<<grid, dim3(64 * NWS), 0, s>>>(
+        gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, 1, NWS, KB><<<grid, dim3(64 * NWS), 0, s>>>(
             M, N, K, a, lda, b, ldb, c, ldc, (S)ar, (S)ai, beta_one, gx, gy, bsa, bsb, bsc, 0, full, 1, 0, nullptr,
             cmap);
     }
@@ -1103,11 +1103,11 @@ int launch_typed(int64_t M, int64_t N, int64_t K, const void *A, int64_t lda, co
   if (ts.ksplit > 1) {
     const unsigned nw = (unsigned)(ts.rem * ts.ksplit);
     if (vec)
-      gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, WV, NWS><<<dim3(nw), dim3(64 * NWS), 0, s>>>(
+      gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, WV, NWS, KB><<<dim3(nw), dim3(64 * NWS), 0, s>>>(
           M, N, K, (const S *)A, lda, (const S *)B, ldb, (S *)C, ldc, (S)ar, (S)ai, beta_one, gx, gy, 0, 0, 0, full,
           ts.rem, ts.ksplit, kchunk, (S *)ws, cmap);
     else
-      gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, 1, NWS><<<dim3(nw), dim3(64 * NWS), 0, s>>>(
+      gemm_kernel<S, NC, OPA, OPB, CJA, CJB, BM, BN, 1, NWS, KB><<<dim3(nw), dim3(64 * NWS), 0, s>>>(
           M, N, K, (const S *)A, lda, (const S *)B, ldb, (S *)C, ldc, (S)ar, (S)ai, beta_one, gx, gy, 0, 0, 0, full,
           ts.rem, ts.ksplit, kchunk, (S *)ws, cmap);
     const int64_t nel = ts.rem * BM * BN;
@@ -1117,15 +1117,15 @@ int launch_typed(int64_t M, int64_t N, int64_t K, const void *A, int64_t lda, co
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-template <typename S, int NC, int BM, int BN, int NWS>
+template <typename S, int NC, int BM, int BN, int NWS, int KB>
 int dispatch_ops(int opa, int opb, int cja, int cjb, int64_t M, int64_t N, int64_t K, const void *A, int64_t lda,
                  const void *B, int64_t ldb, void *C, int64_t ldc, double ar, double ai, int beta_one,
                  int64_t batch, int64_t sa, int64_t sb, int64_t sc, void *ws, size_t ws_bytes,
                  const int64_t *cmap, hipStream_t s) {
 #define TAL_GEMM_CASE(a, b, ca, cb)                                                                      \
   if (opa == a && opb == b && cja == ca && cjb == cb)                                                    \
-    return launch_typed<S, NC, a, b, ca, cb, BM, BN, NWS>(M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, \
-                                                     sa, sb, sc, ws, ws_bytes, cmap, s);
+    return launch_typed<S, NC, a, b, ca, cb, BM, BN, NWS, KB>(M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, \
+                                                              batch, sa, sb, sc, ws, ws_bytes, cmap, s);
 #define TAL_GEMM_LAYOUTS(ca, cb) \
   TAL_GEMM_CASE(0, 0, ca, cb)    \
   TAL_GEMM_CASE(0, 1, ca, cb)    \
@@ -1291,21 +1291,21 @@ template <typename S>
 bool small_real(int64_t M, int64_t N, int64_t batch) {
   if (batch != 1) return false;
   const int64_t t128 = ((M + 127) / 128) * ((N + 127) / 128);
-  return t128 < gemm_slots<S, 1, 128, 128, kWavesSingle>();
+  return t128 < gemm_slots<S, 1, 128, 128, kWavesSingle, BK>();
 }
 
 size_t gemm_tail_workspace(int dk, int64_t M, int64_t N, int64_t K, int64_t batch) {
   switch (dk) {
     case 8:
       return small_real<double>(M, N, batch)
-                 ? tail_bytes<double, 1, 64, 64>(tail_split<double, 1, 64, 64, 4>(M, N, K, batch))
-                 : tail_bytes<double, 1, 128, 128>(tail_split<double, 1, 128, 128, kWavesSingle>(M, N, K, batch));
+                 ? tail_bytes<double, 1, 64, 64>(tail_split<double, 1, 64, 64, kWavesSingle, 2 * BK>(M, N, K, batch))
+                 : tail_bytes<double, 1, 128, 128>(tail_split<double, 1, 128, 128, kWavesSingle, BK>(M, N, K, batch));
     case 4:
       return small_real<float>(M, N, batch)
-                 ? tail_bytes<float, 1, 64, 64>(tail_split<float, 1, 64, 64, 4>(M, N, K, batch))
-                 : tail_bytes<float, 1, 128, 128>(tail_split<float, 1, 128, 128, kWavesSingle>(M, N, K, batch));
-    case 18: return tail_bytes<double, 2, 64, 64>(tail_split<double, 2, 64, 64, kWavesSingle>(M, N, K, batch));
-    case 14: return tail_bytes<float, 2, 64, 64>(tail_split<float, 2, 64, 64, kWavesSingle>(M, N, K, batch));
+                 ? tail_bytes<float, 1, 64, 64>(tail_split<float, 1, 64, 64, 4, BK>(M, N, K, batch))
+                 : tail_bytes<float, 1, 128, 128>(tail_split<float, 1, 128, 128, kWavesSingle, BK>(M, N, K, batch));
+    case 18: return tail_bytes<double, 2, 64, 64>(tail_split<double, 2, 64, 64, kWavesSingle, BK>(M, N, K, batch));
+    case 14: return tail_bytes<float, 2, 64, 64>(tail_split<float, 2, 64, 64, kWavesSingle, BK>(M, N, K, batch));
     default: return 0;
   }
 }
@@ -1337,18 +1337,18 @@ int launch_gemm(int dk, int r4mode, char ta, char tb, int conja, int conjb, int6
     }
     return 0;
   }
-#define TAL_GEMM_KIND(S, NC, BM, BN, NWS)                                                                        \
-  dispatch_ops<S, NC, BM, BN, NWS>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, sa, \
-                                   sb, sc, ws, ws_bytes, cmap, s)
+#define TAL_GEMM_KIND(S, NC, BM, BN, NWS, KB)                                                                    \
+  dispatch_ops<S, NC, BM, BN, NWS, KB>(opa, opb, cja, cjb, M, N, K, A, lda, B, ldb, C, ldc, ar, ai, beta_one, batch, \
+                                       sa, sb, sc, ws, ws_bytes, cmap, s)
   switch (dk) {
     case 8:
-      return small_real<double>(M, N, batch) ? TAL_GEMM_KIND(double, 1, 64, 64, 4)
-                                             : TAL_GEMM_KIND(double, 1, 128, 128, kWavesSingle);
+      return small_real<double>(M, N, batch) ? TAL_GEMM_KIND(double, 1, 64, 64, kWavesSingle, 2 * BK)
+                                             : TAL_GEMM_KIND(double, 1, 128, 128, kWavesSingle, BK);
     case 4:
-      return small_real<float>(M, N, batch) ? TAL_GEMM_KIND(float, 1, 64, 64, 4)
-                                            : TAL_GEMM_KIND(float, 1, 128, 128, kWavesSingle);
-    case 18: return TAL_GEMM_KIND(double, 2, 64, 64, kWavesSingle);
-    case 14: return TAL_GEMM_KIND(float, 2, 64, 64, kWavesSingle);
+      return small_real<float>(M, N, batch) ? TAL_GEMM_KIND(float, 1, 64, 64, 4, BK)
+                                            : TAL_GEMM_KIND(float, 1, 128, 128, kWavesSingle, BK);
+    case 18: return TAL_GEMM_KIND(double, 2, 64, 64, kWavesSingle, BK);
+    case 14: return TAL_GEMM_KIND(float, 2, 64, 64, kWavesSingle, BK);
     default: return -1;
   }
 #undef TAL_GEMM_KIND

@@ -24,6 +24,8 @@
 #include <algorithm>
 #include <cctype>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 
 namespace tal {
@@ -160,10 +162,55 @@ double vol_of(const int *dims, int n) {
   return v;
 }
 
+int plan_uncached(const DigitalPattern &p, const int *ddims, const int *ldims, const int *rdims, int accumulative,
+                  ContractPlan *plan);
+
+// Plans already made, keyed by ranks, digits, extents and the accumulate
+// flag (the plan depends on nothing else).  The search over matricizations
+// costs ~16 us per call on the host, as much as a small contraction's GPU
+// work; repeated shapes (iterative solvers, benchmarks, serving templates)
+// then plan in well under a microsecond.  Cleared when full.
+constexpr size_t kPlanCacheMax = 4096;
+std::mutex g_plan_mu;
+std::map<std::vector<int>, ContractPlan> g_plans;
+
 }  // namespace
 
 int plan_contraction(const DigitalPattern &p, const int *ddims, const int *ldims, const int *rdims,
                      int accumulative, ContractPlan *plan) {
+  const int nd = p.drank, nl = p.lrank, nr = p.rrank;
+  if (nd < 0 || nl < 0 || nr < 0 || (int)p.dig.size() != nl + nr || !plan) return 1;
+  std::vector<int> key;
+  key.reserve(4 + 2 * (size_t)(nl + nr) + nd);
+  key.push_back(nd);
+  key.push_back(nl);
+  key.push_back(nr);
+  key.push_back(accumulative ? 1 : 0);
+  key.insert(key.end(), p.dig.begin(), p.dig.end());
+  key.insert(key.end(), ddims, ddims + nd);
+  key.insert(key.end(), ldims, ldims + nl);
+  key.insert(key.end(), rdims, rdims + nr);
+  {
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    auto it = g_plans.find(key);
+    if (it != g_plans.end()) {
+      *plan = it->second;
+      return 0;
+    }
+  }
+  const int e = plan_uncached(p, ddims, ldims, rdims, accumulative, plan);
+  if (e == 0) {
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    if (g_plans.size() >= kPlanCacheMax) g_plans.clear();
+    g_plans.emplace(std::move(key), *plan);
+  }
+  return e;
+}
+
+namespace {
+
+int plan_uncached(const DigitalPattern &p, const int *ddims, const int *ldims, const int *rdims, int accumulative,
+                  ContractPlan *plan) {
   const int nd = p.drank, nl = p.lrank, nr = p.rrank;
   if (nd < 0 || nl < 0 || nr < 0 || (int)p.dig.size() != nl + nr) return 1;
   std::vector<int> sl(nl), sr(nr), sd(nd);
@@ -300,5 +347,7 @@ int plan_contraction(const DigitalPattern &p, const int *ddims, const int *ldims
   *plan = best;
   return 0;
 }
+
+}  // namespace
 
 }  // namespace tal

@@ -354,7 +354,11 @@ void task_register(Task *t, const std::vector<talsh_tens_t *> &tens) {
 
 int task_finalize(Task *t, bool block) {
   if (t->status == TALSH_TASK_COMPLETED || t->status == TALSH_TASK_ERROR) return t->status;
-  hipError_t q = block ? hipEventSynchronize(t->ev_done->ev) : hipEventQuery(t->ev_done->ev);
+  // a blocking wait polls the completion event, as talshTaskWait polls
+  // talshTaskComplete (talshc.cpp:2481): the runtime's own blocking wait
+  // wakes the thread some 10-20 us after a short task ends
+  hipError_t q = hipEventQuery(t->ev_done->ev);
+  while (block && q == hipErrorNotReady) q = hipEventQuery(t->ev_done->ev);
   if (q == hipErrorNotReady) return t->status;
   Runtime &r = rt();
   Device &d = r.gpus[t->dev];
