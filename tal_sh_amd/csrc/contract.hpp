@@ -35,6 +35,10 @@ int enqueue_ttgt(const DigitalPattern &p, const ContractPlan &plan, int dk, bool
                  void *D, const void *L, const void *R, const int *ddims, const int *ldims, const int *rdims,
                  const TtgtBuffers &tmp, hipStream_t s, hipEvent_t gemm_start, hipEvent_t gemm_end);
 
+// Frees the device tables enqueue_ttgt keeps between calls (at shutdown,
+// after the devices are idle).
+void ttgt_cache_clear();
+
 // Copies the box [off, off + sd) of a dlf tensor with extents bd (element
 // size esz) between a host body and a packed device buffer (h2d: host to
 // device) with strided DMA copies on stream s.  Pageable or pinned host memory.

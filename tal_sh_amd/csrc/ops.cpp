@@ -14,7 +14,11 @@
 // DOC/TALSH_manual.txt:719-750 (D discard, M move, T temporary, K keep).
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "contract.hpp"
 #include "plan.hpp"
@@ -407,7 +411,70 @@ bool d_in_place(const ContractPlan &plan, int *nm, int *nn) {
   return pm == plan.M && pn == plan.N;
 }
 
+// C maps of in-place GEMMs, kept on each device for the shapes already seen:
+// a map depends only on D's extents and the plan, so a repeated contraction
+// reads the stored table instead of launching offset_map_kernel (a launch
+// and a few microseconds of a short contraction's critical path).  Written
+// once with a blocking copy, before any kernel can read it; bounded in
+// total size (further shapes take the per-call kernel); freed at shutdown.
+constexpr size_t kCmapCacheMaxBytes = (size_t)64 << 20;
+struct CmapCache {
+  std::mutex mu;
+  std::map<std::pair<int, std::vector<int64_t>>, int64_t *> maps;
+  size_t bytes = 0;
+};
+CmapCache &cmap_cache() {
+  static CmapCache c;
+  return c;
+}
+
+const int64_t *cached_cmap(const OffsetMap &map) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  const int nt = map.nm + map.nn;
+  std::vector<int64_t> key;
+  key.reserve(2 + 2 * (size_t)nt);
+  key.push_back(map.nm);
+  key.push_back(map.nn);
+  key.insert(key.end(), map.ext, map.ext + nt);
+  key.insert(key.end(), map.str, map.str + nt);
+  CmapCache &c = cmap_cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  auto k = std::make_pair(dev, std::move(key));
+  auto it = c.maps.find(k);
+  if (it != c.maps.end()) return it->second;
+  const size_t n = (size_t)(map.M + map.N), bytes = n * sizeof(int64_t);
+  if (c.bytes + bytes > kCmapCacheMaxBytes) return nullptr;
+  std::vector<int64_t> h(n);
+  for (size_t e = 0; e < n; ++e) {  // offset_map_kernel's sums, on the host
+    const bool row = e < (size_t)map.M;
+    int64_t x = row ? (int64_t)e : (int64_t)e - map.M, off = 0;
+    for (int t = row ? 0 : map.nm; t < (row ? map.nm : nt); ++t) {
+      off += (x % map.ext[t]) * map.str[t];
+      x /= map.ext[t];
+    }
+    h[e] = off;
+  }
+  int64_t *d = nullptr;
+  if (hipMalloc(&d, bytes) != hipSuccess) return nullptr;
+  if (hipMemcpy(d, h.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(d);
+    return nullptr;
+  }
+  c.maps.emplace(std::move(k), d);
+  c.bytes += bytes;
+  return d;
+}
+
 }  // namespace
+
+void ttgt_cache_clear() {
+  CmapCache &c = cmap_cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  for (auto &m : c.maps) (void)hipFree(m.second);
+  c.maps.clear();
+  c.bytes = 0;
+}
 
 TtgtTemps ttgt_temp_bytes(const ContractPlan &plan, int dk, size_t lbytes, size_t rbytes, size_t dbytes) {
   TtgtTemps t;
@@ -464,8 +531,11 @@ int enqueue_ttgt(const DigitalPattern &p, const ContractPlan &plan, int dk, bool
       }
       stride *= ddims[i];
     }
-    if (launch_offset_map(map, (int64_t *)tmp.d, s)) return TALSH_FAILURE;
-    cmap = (const int64_t *)tmp.d;
+    cmap = cached_cmap(map);
+    if (!cmap) {
+      if (launch_offset_map(map, (int64_t *)tmp.d, s)) return TALSH_FAILURE;
+      cmap = (const int64_t *)tmp.d;
+    }
   } else if (plan.d_transpose) {
     if (!tmp.d) return TALSH_FAILURE;
     C = tmp.d;

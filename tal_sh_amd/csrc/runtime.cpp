@@ -15,6 +15,7 @@
 // (LastTask, tensor_algebra_gpu_nvidia.cu:6287-6295), so independent tasks
 // overlap while dependent ones stay ordered.
 #include "runtime.hpp"
+#include "contract.hpp"
 
 #include <chrono>
 #include <cstdio>
@@ -496,6 +497,7 @@ int talshShutdown(void) {
   for (auto &m : r.hazards) m.clear();
   r.tens_tasks.clear();
   r.hazard_epoch++;
+  ttgt_cache_clear();
   for (auto &d : r.gpus) {
     if (!d.on) continue;
     d.backlog.clear();
