@@ -131,7 +131,7 @@ int talshAmdServerDestroy(talshAmdServer *server);
  * = sum of |x|^2 over the `volume` elements of kind data_kind (R4, R8, C4,
  * C8) at device address `body` on GPU `gpu`, enqueued on `stream` (a
  * hipStream_t of that GPU; NULL = its null stream).  The reduction's partial
- * sums live in a fixed per-GPU device array of 8 KiB (1024 doubles, set up by
+ * sums live in a fixed per-GPU device array of 4 KiB (512 doubles, set up by
  * talshInit; enough for any volume); no pool buffer, task, event or device
  * allocation is made.  The caller orders the call against other work on
  * `body` and `result` (no hazard tracking).  Calls that overlap on one GPU

@@ -40,6 +40,12 @@ enum PermOp { PERM_COPY = 0, PERM_ADD = 1 };
 // O2N permutation `o2n`.  Complex kinds may conjugate src.  Bit-exact.
 int launch_permute(int dk, const void *src, void *dst, int rank, const int *dims, const int *o2n, int op,
                    double ar, double ai, int conj, hipStream_t s);
+// Two independent permutation copies (dst_j = perm_j(src_j), arguments as
+// launch_permute's with op PERM_COPY; neither destination may overlap the
+// other copy's source or destination) in one launch when both take the same
+// kernel, else in two, on stream s.
+int launch_permute_pair(int dk, const void *src0, void *dst0, int rank0, const int *dims0, const int *o2n0,
+                        const void *src1, void *dst1, int rank1, const int *dims1, const int *o2n1, hipStream_t s);
 
 // Slice (mode 0): small = big[offsets + i] (small += ... when accumulative);
 // insert (mode 1): big[offsets + i] = small (+=).  small has extents sdims,

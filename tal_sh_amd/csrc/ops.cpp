@@ -501,12 +501,17 @@ int enqueue_ttgt(const DigitalPattern &p, const ContractPlan &plan, int dk, bool
   const int xr = plan.swap ? p.rrank : p.lrank, yr = plan.swap ? p.lrank : p.rrank;
   const void *A = X, *B = Y;
   void *C = D;
-  if (plan.a.transpose) {
+  if (plan.a.transpose && plan.b.transpose) {  // both operand transposes in one launch where they can
+    if (!tmp.a || !tmp.b ||
+        launch_permute_pair(dk, X, tmp.a, xr, xd, plan.a.o2n.data(), Y, tmp.b, yr, yd, plan.b.o2n.data(), s))
+      return TALSH_FAILURE;
+    A = tmp.a;
+    B = tmp.b;
+  } else if (plan.a.transpose) {
     if (!tmp.a || launch_permute(dk, X, tmp.a, xr, xd, plan.a.o2n.data(), PERM_COPY, 1, 0, 0, s))
       return TALSH_FAILURE;
     A = tmp.a;
-  }
-  if (plan.b.transpose) {
+  } else if (plan.b.transpose) {
     if (!tmp.b || launch_permute(dk, Y, tmp.b, yr, yd, plan.b.o2n.data(), PERM_COPY, 1, 0, 0, s))
       return TALSH_FAILURE;
     B = tmp.b;

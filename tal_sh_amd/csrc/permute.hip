@@ -104,7 +104,10 @@ struct Real<double2> {
 // mode: a is fastest on both sides and b is the dim with the next smallest
 // output stride.  c is an outer dim carried through the tile so that one
 // block moves kElems elements with all of its loads issued before any store.
-struct PermArgs {
+// NO: capacity for outer dims (a launch of two permutations packs both
+// argument sets into the kernel arguments, so it uses a short form).
+template <int NO>
+struct PermArgsT {
   int nouter;
   int64_t ea, eb, ec;          // tile-dim extents
   int64_t isb, isc;            // input strides of b and c (a has stride 1)
@@ -112,12 +115,15 @@ struct PermArgs {
   int64_t ntile_a, ntile_b, ntile_c;
   int tc;                      // tile size along c (<= TCMAX of the kernel)
   int tb_log;                  // row mode: log2 of the tile size along b
-  int64_t oe[kMaxRank];        // outer extents
-  int64_t ois[kMaxRank];       // outer input strides
-  int64_t oos[kMaxRank];       // outer output strides
+  int64_t oe[NO];              // outer extents
+  int64_t ois[NO];             // outer input strides
+  int64_t oos[NO];             // outer output strides
 };
+using PermArgs = PermArgsT<kMaxRank>;
+constexpr int kPairOuter = 6;
 
-__device__ inline void tile_origin(const PermArgs &p, int64_t blk, int TA, int TB, int64_t &a0, int64_t &b0,
+template <class PA>
+__device__ inline void tile_origin(const PA &p, int64_t blk, int TA, int TB, int64_t &a0, int64_t &b0,
                                    int64_t &c0, int64_t &ib, int64_t &ob) {
   int64_t t = blk % p.ntile_a;
   blk /= p.ntile_a;
@@ -206,14 +212,13 @@ __device__ inline void emit(T *o, Pack<T, V> x, typename Real<T>::type ar, typen
 // 16-byte aligned bases when V > 1).  The tile size along b is a power of two
 // chosen at launch (p.tb_log) to fit the extent of b, so a small b does not
 // leave most of the block idle; c takes the rest of the block.
-template <typename T, int TA, int V, int OP>
-__global__ __launch_bounds__(kThreads) void perm_rows_kernel(const T *__restrict__ src, T *__restrict__ dst,
-                                                             PermArgs p, typename Real<T>::type ar,
-                                                             typename Real<T>::type ai, int conj) {
+template <typename T, int TA, int V, int OP, class PA>
+__device__ inline void perm_rows_body(const T *__restrict__ src, T *__restrict__ dst, const PA &p, int64_t blk,
+                                      typename Real<T>::type ar, typename Real<T>::type ai, int conj) {
   constexpr int EPB = elems_per_block<T>(), N = EPB / kThreads / V, AV = TA / V;
   const int tbl = p.tb_log, tb = 1 << tbl;
   int64_t a0, b0, c0, ib, ob;
-  tile_origin(p, blockIdx.x, TA, tb, a0, b0, c0, ib, ob);
+  tile_origin(p, blk, TA, tb, a0, b0, c0, ib, ob);
   const int na = (int)min<int64_t>(TA, p.ea - a0), nb = (int)min<int64_t>(tb, p.eb - b0);
   const int nc = (int)min<int64_t>(p.tc, p.ec - c0);
   Pack<T, V> v[N];
@@ -230,19 +235,24 @@ __global__ __launch_bounds__(kThreads) void perm_rows_kernel(const T *__restrict
     if (ia < na && jb < nb && kc < nc) emit<T, V, OP>(dst + ob + ia + jb * p.osb + kc * p.osc, v[q], ar, ai, conj);
   }
 }
+template <typename T, int TA, int V, int OP>
+__global__ __launch_bounds__(kThreads) void perm_rows_kernel(const T *__restrict__ src, T *__restrict__ dst,
+                                                             PermArgs p, typename Real<T>::type ar,
+                                                             typename Real<T>::type ai, int conj) {
+  perm_rows_body<T, TA, V, OP>(src, dst, p, blockIdx.x, ar, ai, conj);
+}
 
 // Tile mode: stage a (a x b x c) tile through LDS (+1 pad on the a axis) so
 // the reads run along a and the writes along b, V elements per access on
 // both sides (the host guarantees V | extents of a and b when V > 1).
-template <typename T, int TA, int TB, int V, int OP>
-__global__ __launch_bounds__(kThreads) void perm_tile_kernel(const T *__restrict__ src, T *__restrict__ dst,
-                                                             PermArgs p, typename Real<T>::type ar,
-                                                             typename Real<T>::type ai, int conj) {
+template <typename T, int TA, int TB, int V, int OP, class PA>
+__device__ inline void perm_tile_body(const T *__restrict__ src, T *__restrict__ dst, const PA &p, int64_t blk,
+                                      typename Real<T>::type ar, typename Real<T>::type ai, int conj) {
   constexpr int EPB = elems_per_block<T>(), N = EPB / kThreads / V, TCMAX = EPB / (TA * TB);
   constexpr int AV = TA / V, BV = TB / V;
   __shared__ T tile[TCMAX][TB][TA + 1];
   int64_t a0, b0, c0, ib, ob;
-  tile_origin(p, blockIdx.x, TA, TB, a0, b0, c0, ib, ob);
+  tile_origin(p, blk, TA, TB, a0, b0, c0, ib, ob);
   const int na = (int)min<int64_t>(TA, p.ea - a0), nb = (int)min<int64_t>(TB, p.eb - b0);
   const int nc = (int)min<int64_t>(p.tc, p.ec - c0);
   Pack<T, V> v[N];
@@ -271,6 +281,36 @@ __global__ __launch_bounds__(kThreads) void perm_tile_kernel(const T *__restrict
       for (int t = 0; t < V; ++t) x.v[t] = tile[kc][jb + t][ia];
       emit<T, V, OP>(dst + ob + jb + ia * p.osa + kc * p.osc, x, ar, ai, conj);
     }
+  }
+}
+template <typename T, int TA, int TB, int V, int OP>
+__global__ __launch_bounds__(kThreads) void perm_tile_kernel(const T *__restrict__ src, T *__restrict__ dst,
+                                                             PermArgs p, typename Real<T>::type ar,
+                                                             typename Real<T>::type ai, int conj) {
+  perm_tile_body<T, TA, TB, V, OP>(src, dst, p, blockIdx.x, ar, ai, conj);
+}
+
+// Two permutation copies of one kind and tile shape in one launch (the A and
+// B operand transposes of a contraction): blocks [0, nblk0) move the first,
+// the rest the second.  One launch instead of two on a short contraction's
+// critical path.
+template <typename T>
+struct PermPair {
+  const T *src[2];
+  T *dst[2];
+  PermArgsT<kPairOuter> p[2];
+  int64_t nblk0;
+};
+template <typename T, int TA, int TB, int V, bool ROWS>
+__global__ __launch_bounds__(kThreads) void perm_pair_kernel(PermPair<T> q) {
+  using R = typename Real<T>::type;
+  const int64_t b = blockIdx.x;
+  if constexpr (ROWS) {
+    if (b < q.nblk0) perm_rows_body<T, TA, V, PERM_COPY>(q.src[0], q.dst[0], q.p[0], b, (R)1, (R)0, 0);
+    else perm_rows_body<T, TA, V, PERM_COPY>(q.src[1], q.dst[1], q.p[1], b - q.nblk0, (R)1, (R)0, 0);
+  } else {
+    if (b < q.nblk0) perm_tile_body<T, TA, TB, V, PERM_COPY>(q.src[0], q.dst[0], q.p[0], b, (R)1, (R)0, 0);
+    else perm_tile_body<T, TA, TB, V, PERM_COPY>(q.src[1], q.dst[1], q.p[1], b - q.nblk0, (R)1, (R)0, 0);
   }
 }
 
@@ -312,7 +352,9 @@ __global__ __launch_bounds__(kThreads) void scale_kernel(T *__restrict__ d, int6
 }
 
 // ------------------------------------------------------------------ norm2
-constexpr int kNormBlocks = 1024;
+// Stage-1 blocks (2 per CU): 512 read a 134 MB tensor in 23.6 us against
+// 24.9 us with 1,024, 25.0 with 2,048 and 25.4 with 8 loads in flight.
+constexpr int kNormBlocks = 512;
 
 __device__ inline double wave_sum(double v) {
 #pragma unroll
@@ -413,14 +455,22 @@ constexpr int vec_width() {
   return sizeof(T) >= 16 ? 1 : 16 / (int)sizeof(T);
 }
 
-// Fuses the permutation and launches the matching kernel.
+// A permutation after fusion: mode 0 = nothing to move, 1 = flat copy
+// (identity), 2 = row problem, 3 = tile problem (tile ta x tb).
+struct PermPlan {
+  int mode = 0;
+  int ta = 0, tb = 0;
+  bool vec = false;  // 16-byte accesses
+  int64_t nblk = 0, vol = 0;
+  PermArgs p;
+};
+
+// Fuses the permutation and picks its kernel and launch shape.
 template <typename T>
-int launch_perm_typed(const T *src, T *dst, int rank, const int *dims, const int *o2n, int op, double ar_,
-                      double ai_, int conj, hipStream_t s) {
-  using R = typename Real<T>::type;
-  const R ar = (R)ar_, ai = (R)ai_;
+int plan_perm(const T *src, T *dst, int rank, const int *dims, const int *o2n, PermPlan *pl) {
   int64_t vol = 1;
   for (int k = 0; k < rank; ++k) vol *= dims[k];
+  pl->mode = 0;
   if (vol == 0) return 0;
   int64_t oext[kMaxRank], ostride_of_out[kMaxRank];
   for (int k = 0; k < rank; ++k) oext[o2n[k]] = dims[k];
@@ -468,19 +518,10 @@ int launch_perm_typed(const T *src, T *dst, int rank, const int *dims, const int
   constexpr int VW = vec_width<T>();
   const bool aligned = ((uintptr_t)src % 16 == 0) && ((uintptr_t)dst % 16 == 0);
   if (r <= 1) {
-    const int g = grid_for(vol / VW);
-    if (aligned) {
-      if (op == PERM_COPY)
-        launch_clocked(perm_flat_kernel<T, VW, PERM_COPY>, dim3(g), dim3(kThreads), s, src, dst, vol, ar, ai, conj);
-      else
-        launch_clocked(perm_flat_kernel<T, VW, PERM_ADD>, dim3(g), dim3(kThreads), s, src, dst, vol, ar, ai, conj);
-    } else {
-      if (op == PERM_COPY)
-        launch_clocked(perm_flat_kernel<T, 1, PERM_COPY>, dim3(g), dim3(kThreads), s, src, dst, vol, ar, ai, conj);
-      else
-        launch_clocked(perm_flat_kernel<T, 1, PERM_ADD>, dim3(g), dim3(kThreads), s, src, dst, vol, ar, ai, conj);
-    }
-    return hipGetLastError() == hipSuccess ? 0 : -2;
+    pl->mode = 1;
+    pl->vol = vol;
+    pl->vec = aligned;
+    return 0;
   }
   int b = -1;
   for (int k = 0; k < r; ++k)
@@ -497,7 +538,7 @@ int launch_perm_typed(const T *src, T *dst, int rank, const int *dims, const int
   int c = -1;
   for (int k = 1; k < r; ++k)
     if (k != b && (c < 0 || fis[k] < fis[c])) c = k;
-  PermArgs p;
+  PermArgs &p = pl->p;
   std::memset(&p, 0, sizeof(p));
   p.ea = fe[0];
   p.osa = fos[0];
@@ -540,7 +581,41 @@ int launch_perm_typed(const T *src, T *dst, int rank, const int *dims, const int
   // vector accesses need V | extent on each contiguous side (so every other
   // stride is a multiple of V) and 16-byte aligned bodies
   const bool vec = aligned && (p.ea % VW == 0) && (rows || p.eb % VW == 0) && VW > 1;
-  if (rows) {
+  pl->mode = rows ? 2 : 3;
+  pl->ta = ta;
+  pl->tb = tb;
+  pl->vec = vec;
+  pl->nblk = nblk;
+  pl->vol = vol;
+  return 0;
+}
+
+template <typename T>
+int launch_plan(const T *src, T *dst, const PermPlan &pl, int op, double ar_, double ai_, int conj, hipStream_t s) {
+  using R = typename Real<T>::type;
+  const R ar = (R)ar_, ai = (R)ai_;
+  constexpr int VW = vec_width<T>();
+  const PermArgs &p = pl.p;
+  const bool vec = pl.vec;
+  const int64_t nblk = pl.nblk, vol = pl.vol;
+  if (pl.mode == 0) return 0;
+  if (pl.mode == 1) {
+    const int g = grid_for(vol / VW);
+    if (vec) {
+      if (op == PERM_COPY)
+        launch_clocked(perm_flat_kernel<T, VW, PERM_COPY>, dim3(g), dim3(kThreads), s, src, dst, vol, ar, ai, conj);
+      else
+        launch_clocked(perm_flat_kernel<T, VW, PERM_ADD>, dim3(g), dim3(kThreads), s, src, dst, vol, ar, ai, conj);
+    } else {
+      if (op == PERM_COPY)
+        launch_clocked(perm_flat_kernel<T, 1, PERM_COPY>, dim3(g), dim3(kThreads), s, src, dst, vol, ar, ai, conj);
+      else
+        launch_clocked(perm_flat_kernel<T, 1, PERM_ADD>, dim3(g), dim3(kThreads), s, src, dst, vol, ar, ai, conj);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+  }
+  const int ta = pl.ta;
+  if (pl.mode == 2) {
 #define TAL_ROWS(A)                                                            \
   (vec ? launch_rows<T, A, VW>(src, dst, p, op, ar, ai, conj, nblk, s)         \
        : launch_rows<T, A, 1>(src, dst, p, op, ar, ai, conj, nblk, s))
@@ -557,6 +632,79 @@ int launch_perm_typed(const T *src, T *dst, int rank, const int *dims, const int
     else if (ta == 16) TAL_TILE(16, 16);
     else TAL_TILE(8, 8);
 #undef TAL_TILE
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+template <typename T>
+int launch_perm_typed(const T *src, T *dst, int rank, const int *dims, const int *o2n, int op, double ar,
+                      double ai, int conj, hipStream_t s) {
+  PermPlan pl;
+  if (int e = plan_perm(src, dst, rank, dims, o2n, &pl)) return e;
+  return launch_plan(src, dst, pl, op, ar, ai, conj, s);
+}
+
+template <typename T, int TA, int TB, int V, bool ROWS>
+void launch_pair(const PermPair<T> &q, int64_t nblk, hipStream_t s) {
+  launch_clocked(perm_pair_kernel<T, TA, TB, V, ROWS>, dim3((unsigned)nblk), dim3(kThreads), s, q);
+}
+
+// Two permutation copies: one launch when both take the same kernel (row or
+// tile problems of one tile shape and access width with few outer dims),
+// else one launch each.
+template <typename T>
+int launch_perm_pair_typed(const T *src0, T *dst0, int rank0, const int *dims0, const int *o2n0, const T *src1,
+                           T *dst1, int rank1, const int *dims1, const int *o2n1, hipStream_t s) {
+  PermPlan a, b;
+  if (int e = plan_perm(src0, dst0, rank0, dims0, o2n0, &a)) return e;
+  if (int e = plan_perm(src1, dst1, rank1, dims1, o2n1, &b)) return e;
+  const bool pair = a.mode >= 2 && a.mode == b.mode && a.ta == b.ta && a.tb == b.tb && a.vec == b.vec &&
+                    a.p.nouter <= kPairOuter && b.p.nouter <= kPairOuter && a.nblk + b.nblk <= 0x7fffffffLL;
+  if (!pair) {
+    if (int e = launch_plan(src0, dst0, a, PERM_COPY, 1.0, 0.0, 0, s)) return e;
+    return launch_plan(src1, dst1, b, PERM_COPY, 1.0, 0.0, 0, s);
+  }
+  PermPair<T> q;
+  std::memset(&q, 0, sizeof(q));
+  q.src[0] = src0;
+  q.dst[0] = dst0;
+  q.src[1] = src1;
+  q.dst[1] = dst1;
+  q.nblk0 = a.nblk;
+  const PermPlan *pl[2] = {&a, &b};
+  for (int j = 0; j < 2; ++j) {
+    const PermArgs &x = pl[j]->p;
+    auto &y = q.p[j];
+    y.nouter = x.nouter;
+    y.ea = x.ea, y.eb = x.eb, y.ec = x.ec;
+    y.isb = x.isb, y.isc = x.isc;
+    y.osa = x.osa, y.osb = x.osb, y.osc = x.osc;
+    y.ntile_a = x.ntile_a, y.ntile_b = x.ntile_b, y.ntile_c = x.ntile_c;
+    y.tc = x.tc;
+    y.tb_log = x.tb_log;
+    for (int k = 0; k < x.nouter; ++k) {
+      y.oe[k] = x.oe[k];
+      y.ois[k] = x.ois[k];
+      y.oos[k] = x.oos[k];
+    }
+  }
+  constexpr int VW = vec_width<T>();
+  const int64_t n = a.nblk + b.nblk;
+  if (a.mode == 2) {
+#define TAL_PAIR_ROWS(A) \
+  (a.vec ? launch_pair<T, A, 1, VW, true>(q, n, s) : launch_pair<T, A, 1, 1, true>(q, n, s))
+    if (a.ta == 64) TAL_PAIR_ROWS(64);
+    else if (a.ta == 32) TAL_PAIR_ROWS(32);
+    else if (a.ta == 16) TAL_PAIR_ROWS(16);
+    else TAL_PAIR_ROWS(8);
+#undef TAL_PAIR_ROWS
+  } else {
+#define TAL_PAIR_TILE(A) \
+  (a.vec ? launch_pair<T, A, A, VW, false>(q, n, s) : launch_pair<T, A, A, 1, false>(q, n, s))
+    if (a.ta == 32) TAL_PAIR_TILE(32);
+    else if (a.ta == 16) TAL_PAIR_TILE(16);
+    else TAL_PAIR_TILE(8);
+#undef TAL_PAIR_TILE
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
@@ -737,6 +885,32 @@ int launch_permute(int dk, const void *src, void *dst, int rank, const int *dims
       return launch_perm_typed<double2>((const double2 *)src, (double2 *)dst, rank, dims, o2n, op, ar, ai, conj, s);
     default: return -1;
   }
+}
+
+static bool valid_perm(int rank, const int *dims, const int *o2n) {
+  if (rank < 0 || rank > kMaxRank || (rank > 0 && (!dims || !o2n))) return false;
+  bool seen[kMaxRank] = {false};
+  for (int k = 0; k < rank; ++k) {
+    if (dims[k] < 0 || o2n[k] < 0 || o2n[k] >= rank || seen[o2n[k]]) return false;
+    seen[o2n[k]] = true;
+  }
+  return true;
+}
+
+int launch_permute_pair(int dk, const void *src0, void *dst0, int rank0, const int *dims0, const int *o2n0,
+                        const void *src1, void *dst1, int rank1, const int *dims1, const int *o2n1, hipStream_t s) {
+  if (!src0 || !dst0 || !src1 || !dst1 || !valid_perm(rank0, dims0, o2n0) || !valid_perm(rank1, dims1, o2n1))
+    return -1;
+#define TAL_PAIR_ARGS(T)                                                                                    \
+  (const T *)src0, (T *)dst0, rank0, dims0, o2n0, (const T *)src1, (T *)dst1, rank1, dims1, o2n1, s
+  switch (dk) {
+    case 4: return launch_perm_pair_typed<float>(TAL_PAIR_ARGS(float));
+    case 8: return launch_perm_pair_typed<double>(TAL_PAIR_ARGS(double));
+    case 14: return launch_perm_pair_typed<float2>(TAL_PAIR_ARGS(float2));
+    case 18: return launch_perm_pair_typed<double2>(TAL_PAIR_ARGS(double2));
+    default: return -1;
+  }
+#undef TAL_PAIR_ARGS
 }
 
 int launch_slice(int dk, int mode, const void *src, void *dst, int rank, const int *sdims, const int *bdims,
