@@ -411,59 +411,75 @@ bool d_in_place(const ContractPlan &plan, int *nm, int *nn) {
   return pm == plan.M && pn == plan.N;
 }
 
-// C maps of in-place GEMMs, kept on each device for the shapes already seen:
-// a map depends only on D's extents and the plan, so a repeated contraction
-// reads the stored table instead of launching offset_map_kernel (a launch
-// and a few microseconds of a short contraction's critical path).  Written
-// once with a blocking copy, before any kernel can read it; bounded in
-// total size (further shapes take the per-call kernel); freed at shutdown.
-constexpr size_t kCmapCacheMaxBytes = (size_t)64 << 20;
+// C maps of in-place GEMMs, kept on each device for shapes that recur: a map
+// depends only on D's extents and the plan, so a repeated contraction reads
+// the stored table instead of launching offset_map_kernel (a launch and a
+// few microseconds on a short contraction's critical path).  A shape's
+// second use writes its table with that kernel into a per-device chunk, on
+// the task's stream, and records an event that later users on other streams
+// wait for; shapes seen once (most of a mixed serving stream) cost nothing.
+// The chunk is bounded (further shapes take the per-call kernel); freed at
+// shutdown.
+constexpr size_t kCmapChunkBytes = (size_t)64 << 20;
+using CmapKey = std::pair<int, std::vector<int64_t>>;
+struct CmapEntry {
+  int64_t *d = nullptr;
+  hipEvent_t ready = nullptr;
+};
+struct CmapChunk {
+  char *base = nullptr;
+  size_t used = 0;
+  bool failed = false;
+};
 struct CmapCache {
   std::mutex mu;
-  std::map<std::pair<int, std::vector<int64_t>>, int64_t *> maps;
-  size_t bytes = 0;
+  std::map<CmapKey, int> seen;  // uses of shapes not stored yet
+  std::map<CmapKey, CmapEntry> maps;
+  std::map<int, CmapChunk> chunks;  // per HIP device
 };
 CmapCache &cmap_cache() {
   static CmapCache c;
   return c;
 }
 
-const int64_t *cached_cmap(const OffsetMap &map) {
+const int64_t *cached_cmap(const OffsetMap &map, hipStream_t s) {
   int dev = -1;
   if (hipGetDevice(&dev) != hipSuccess) return nullptr;
   const int nt = map.nm + map.nn;
-  std::vector<int64_t> key;
-  key.reserve(2 + 2 * (size_t)nt);
-  key.push_back(map.nm);
-  key.push_back(map.nn);
-  key.insert(key.end(), map.ext, map.ext + nt);
-  key.insert(key.end(), map.str, map.str + nt);
+  CmapKey k;
+  k.first = dev;
+  k.second.reserve(2 + 2 * (size_t)nt);
+  k.second.push_back(map.nm);
+  k.second.push_back(map.nn);
+  k.second.insert(k.second.end(), map.ext, map.ext + nt);
+  k.second.insert(k.second.end(), map.str, map.str + nt);
   CmapCache &c = cmap_cache();
   std::lock_guard<std::mutex> lk(c.mu);
-  auto k = std::make_pair(dev, std::move(key));
   auto it = c.maps.find(k);
-  if (it != c.maps.end()) return it->second;
-  const size_t n = (size_t)(map.M + map.N), bytes = n * sizeof(int64_t);
-  if (c.bytes + bytes > kCmapCacheMaxBytes) return nullptr;
-  std::vector<int64_t> h(n);
-  for (size_t e = 0; e < n; ++e) {  // offset_map_kernel's sums, on the host
-    const bool row = e < (size_t)map.M;
-    int64_t x = row ? (int64_t)e : (int64_t)e - map.M, off = 0;
-    for (int t = row ? 0 : map.nm; t < (row ? map.nm : nt); ++t) {
-      off += (x % map.ext[t]) * map.str[t];
-      x /= map.ext[t];
-    }
-    h[e] = off;
+  if (it != c.maps.end()) {
+    if (hipEventQuery(it->second.ready) == hipErrorNotReady) (void)hipStreamWaitEvent(s, it->second.ready, 0);
+    return it->second.d;
   }
-  int64_t *d = nullptr;
-  if (hipMalloc(&d, bytes) != hipSuccess) return nullptr;
-  if (hipMemcpy(d, h.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
-    (void)hipFree(d);
+  if (c.seen.size() >= 4096) c.seen.clear();
+  if (++c.seen[k] < 2) return nullptr;
+  CmapChunk &ch = c.chunks[dev];
+  const size_t bytes = ((size_t)(map.M + map.N) * sizeof(int64_t) + 255) / 256 * 256;
+  if (!ch.base && !ch.failed && hipMalloc((void **)&ch.base, kCmapChunkBytes) != hipSuccess) {
+    ch.base = nullptr;
+    ch.failed = true;
+  }
+  if (!ch.base || ch.used + bytes > kCmapChunkBytes) return nullptr;
+  CmapEntry en;
+  en.d = (int64_t *)(ch.base + ch.used);
+  if (hipEventCreateWithFlags(&en.ready, hipEventDisableTiming) != hipSuccess) return nullptr;
+  if (launch_offset_map(map, en.d, s) || hipEventRecord(en.ready, s) != hipSuccess) {
+    (void)hipEventDestroy(en.ready);
     return nullptr;
   }
-  c.maps.emplace(std::move(k), d);
-  c.bytes += bytes;
-  return d;
+  ch.used += bytes;
+  c.seen.erase(k);
+  c.maps.emplace(std::move(k), en);
+  return en.d;
 }
 
 }  // namespace
@@ -471,9 +487,12 @@ const int64_t *cached_cmap(const OffsetMap &map) {
 void ttgt_cache_clear() {
   CmapCache &c = cmap_cache();
   std::lock_guard<std::mutex> lk(c.mu);
-  for (auto &m : c.maps) (void)hipFree(m.second);
+  for (auto &m : c.maps) (void)hipEventDestroy(m.second.ready);
+  for (auto &ch : c.chunks)
+    if (ch.second.base) (void)hipFree(ch.second.base);
   c.maps.clear();
-  c.bytes = 0;
+  c.seen.clear();
+  c.chunks.clear();
 }
 
 TtgtTemps ttgt_temp_bytes(const ContractPlan &plan, int dk, size_t lbytes, size_t rbytes, size_t dbytes) {
@@ -536,7 +555,7 @@ int enqueue_ttgt(const DigitalPattern &p, const ContractPlan &plan, int dk, bool
       }
       stride *= ddims[i];
     }
-    cmap = cached_cmap(map);
+    cmap = cached_cmap(map, s);
     if (!cmap) {
       if (launch_offset_map(map, (int64_t *)tmp.d, s)) return TALSH_FAILURE;
       cmap = (const int64_t *)tmp.d;
