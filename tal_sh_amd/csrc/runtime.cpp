@@ -262,9 +262,12 @@ void image_release(talsh_dev_rsc_t *r, size_t bytes) {
       host_free(r->gmem_p, r->buf_entry, bytes);
     else if (g >= 0)
       dev_free(g, r->gmem_p, r->buf_entry);
-  } else if (r->gmem_p) {
-    hazard_drain(r->gmem_p, bytes);
   }
+  // Attached (caller-owned) memory is not drained: the tasks that name this
+  // tensor were finalized before its images go, and other tensors' tasks on
+  // overlapping memory keep their hazard entries, which still order later
+  // views of it.  (Draining them made a destruct wait for unrelated tasks
+  // that read the same caller buffer.)
   r->dev_id = DEV_NULL;
   r->gmem_p = nullptr;
   r->buf_entry = -1;
