@@ -398,3 +398,40 @@ def test_rank0_operands_and_outer_products(talsh_gpu, oracle, dk):
                 assert np.all(np.abs(got - want) <= 1e-6 * bound), (pat, acc)
             for x in (lt, rt, dt):
                 x.destruct()
+
+
+@pytest.mark.parametrize("dk", (R8, R4, C8))
+def test_repeated_shapes_in_flight(talsh_gpu, oracle, dk):
+    """The same contraction shapes submitted several times without waiting, each into its own D: the plan
+    is reused, both operand transposes share one launch where they take the same kernel, and the D offset
+    table of an in-place GEMM is kept on the device from a shape's second use on, read by the later tasks
+    (on whichever stream each lands) once the task that wrote it has.  Shapes with unusual extents so the
+    first use here is the first anywhere in the process.  Every result against the oracle."""
+    ts = talsh_gpu
+    shapes = [("D(a,b,c,d)+=L(c,i,b,j)*R(d,j,a,i)", [17, 19, 18, 21], [18, 5, 19, 7], [21, 7, 17, 5]),
+              ("D(a,b,c,d)+=L(a,i,c,j)*R(j,b,i,d)", [19, 17, 21, 18], [19, 6, 21, 9], [9, 17, 6, 18]),
+              ("D(a,b,c)+=L(c,i,a)*R(i,b)", [23, 17, 19], [19, 33, 23], [33, 17]),
+              ("D(a,b,c,d)+=L(d,b,i)*R(a,i,c)", [17, 18, 19, 20], [20, 18, 11], [17, 11, 19])]
+    copies = 4
+    tol = 1e-12 if dk in (R8, C8) else 1e-5
+    for s, (pat, dd, ld, rd) in enumerate(shapes):
+        say("repeated shape", dk, pat, dd, ld, rd, "x", copies)
+        L, R = tensor(dk, ld, 700 + s), tensor(dk, rd, 710 + s)
+        lt, rt = on_gpu(ts, dk, ld, L), on_gpu(ts, dk, rd, R)
+        ds, d0s, tasks = [], [], []
+        for c in range(copies):
+            D0 = tensor(dk, dd, 720 + 10 * s + c)
+            ds.append(on_gpu(ts, dk, dd, D0))
+            d0s.append(D0)
+        alpha = complex(0.5, -0.25) if dk == C8 else 0.5
+        for c in range(copies):
+            tk = ts.Task()
+            ts.check(ts.contract(pat, ds[c], lt, rt, alpha=alpha, task=tk), "contract %d" % c)
+            tasks.append(tk)
+        finish(*tasks)
+        for c in range(copies):
+            want = oracle.contract(dk, pat, dd, d0s[c], ld, L, rd, R, (alpha.real, alpha.imag) if dk == C8
+                                   else (alpha, 0.0), 1)
+            assert relerr(snapshot(ts, ds[c]), want) <= tol, (pat, c)
+        for t in ds + [lt, rt]:
+            t.destruct()
