@@ -4,7 +4,8 @@ Headline (`value`): R8 contraction TFLOP/s of the large dense contraction
     D(a,b,c,d) += L(a,i,c,j) * R(j,b,i,d), all extents 96
 (84.9 M elements per tensor, TTGT GEMM M=N=K=9216, 1.5655e12 FLOP per step),
 through talshTensorContract on GPU-resident tensors; one step = one
-contraction (input transposes + FP64 MFMA GEMM + output transpose).
+contraction (both input transposes in one launch + the FP64 MFMA GEMM
+stage, which reads and writes D in place through row/column offsets).
 Secondary figures on the same line: the same contraction in R4 and C8, the
 32^4 contraction, and the rank-6 16^6 transpose / permuted-add / norm2 sweep
 (34 permutations) as GB/s.  `roofline` is computed for the FP64 GEMM kernel
