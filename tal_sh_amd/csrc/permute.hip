@@ -867,15 +867,20 @@ int launch_slice_typed(int mode, const T *src, T *dst, int rank, const int *sdim
 
 }  // namespace
 
-int launch_permute(int dk, const void *src, void *dst, int rank, const int *dims, const int *o2n, int op,
-                   double ar, double ai, int conj, hipStream_t s) {
-  if (rank < 0 || rank > kMaxRank || (rank > 0 && (!dims || !o2n)) || !src || !dst) return -1;
-  if (op != PERM_COPY && op != PERM_ADD) return -1;
+static bool valid_perm(int rank, const int *dims, const int *o2n) {
+  if (rank < 0 || rank > kMaxRank || (rank > 0 && (!dims || !o2n))) return false;
   bool seen[kMaxRank] = {false};
   for (int k = 0; k < rank; ++k) {
-    if (dims[k] < 0 || o2n[k] < 0 || o2n[k] >= rank || seen[o2n[k]]) return -1;
+    if (dims[k] < 0 || o2n[k] < 0 || o2n[k] >= rank || seen[o2n[k]]) return false;
     seen[o2n[k]] = true;
   }
+  return true;
+}
+
+int launch_permute(int dk, const void *src, void *dst, int rank, const int *dims, const int *o2n, int op,
+                   double ar, double ai, int conj, hipStream_t s) {
+  if (!src || !dst || !valid_perm(rank, dims, o2n)) return -1;
+  if (op != PERM_COPY && op != PERM_ADD) return -1;
   switch (dk) {
     case 4: return launch_perm_typed<float>((const float *)src, (float *)dst, rank, dims, o2n, op, ar, ai, 0, s);
     case 8: return launch_perm_typed<double>((const double *)src, (double *)dst, rank, dims, o2n, op, ar, ai, 0, s);
@@ -887,15 +892,6 @@ int launch_permute(int dk, const void *src, void *dst, int rank, const int *dims
   }
 }
 
-static bool valid_perm(int rank, const int *dims, const int *o2n) {
-  if (rank < 0 || rank > kMaxRank || (rank > 0 && (!dims || !o2n))) return false;
-  bool seen[kMaxRank] = {false};
-  for (int k = 0; k < rank; ++k) {
-    if (dims[k] < 0 || o2n[k] < 0 || o2n[k] >= rank || seen[o2n[k]]) return false;
-    seen[o2n[k]] = true;
-  }
-  return true;
-}
 
 int launch_permute_pair(int dk, const void *src0, void *dst0, int rank0, const int *dims0, const int *o2n0,
                         const void *src1, void *dst1, int rank1, const int *dims1, const int *o2n1, hipStream_t s) {
