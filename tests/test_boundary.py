@@ -196,6 +196,33 @@ def test_planner_avoids_transposes_when_layout_allows():
     assert err == 0 and mnk == [1024, 1024, 1024] and flags[1:] == [1, 1, 1]
 
 
+
+def test_plan_cache_returns_each_shape_its_own_plan():
+    """Plans are memoised by (ranks, digits, extents, accumulate): interleaved repeats of different shapes,
+    patterns and accumulate flags return exactly what their first planning returned, and shapes that differ
+    only in one extent or in the accumulate flag are told apart."""
+    import random
+    from tal_common import random_contraction
+    rng = random.Random(31337)
+    cases = []
+    for t in range(40):
+        pat, dd, ld, rd = random_contraction(rng, max_rank=5, ext_lo=1, ext_hi=7)
+        cases.append((pat, dd, ld, rd, t % 2))
+    # near-duplicates: one extent changed, or the other accumulate flag
+    pat, dd, ld, rd, acc = cases[0]
+    cases.append((pat, dd, ld, rd, 1 - acc))
+    cases.append(("D(a,b,c,d)+=L(c,i,b,j)*R(d,j,a,i)", [32] * 4, [32] * 4, [32] * 4, 1))
+    cases.append(("D(a,b,c,d)+=L(c,i,b,j)*R(d,j,a,i)", [32, 32, 32, 16], [32, 32, 32, 32], [16, 32, 32, 32], 1))
+    first = [plan(p_, d_, l_, r_, a_) for (p_, d_, l_, r_, a_) in cases]
+    assert all(f[0] == 0 for f in first)
+    assert sorted(first[-1][1][:2]) == [512, 1024] and first[-1][1][2] == 1024  # (A and B may be swapped)
+    assert first[-2][1] == [1024, 1024, 1024]
+    order = list(range(len(cases))) * 3
+    rng.shuffle(order)
+    for i in order:
+        p_, d_, l_, r_, a_ = cases[i]
+        assert plan(p_, d_, l_, r_, a_) == first[i], cases[i]
+
 def test_host_side_errors_without_gpu():
     lib = ts.load()
     hb, ham = ctypes.c_size_t(0), ctypes.c_int(0)
