@@ -67,22 +67,28 @@ def gpu_tensor(dk, dims, seed):
 
 
 def timed_contractions(pat, d, l, r, steps, warmup, alpha=1.0):
+    """Wall time of `steps` contractions, each submitted and waited for before the next (task handles and
+    arguments prepared outside the timed loop; the tasks' event times are read after it)."""
     lib = ts.load()
     for _ in range(warmup):
         ts.check(ts.contract(pat, d, l, r, alpha=alpha), "contract")
     lib.talshAmdDeviceSynchronize(0)
-    mm, tot = [], []
+    tasks = [ts.Task() for _ in range(steps)]
+    cpat, a, cc = pat.encode(), complex(alpha), ts.coh("M", "T", "T")
+    dp, lp, rp = d.ptr, l.ptr, r.ptr
     t0 = time.perf_counter()
-    for _ in range(steps):
-        tk = ts.Task()
-        ts.check(ts.contract(pat, d, l, r, alpha=alpha, task=tk), "contract")
+    for tk in tasks:
+        ts.check(lib.talshTensorContract(cpat, dp, lp, rp, a.real, a.imag, ts.DEV_DEFAULT, ts.DEV_DEFAULT, cc,
+                                         ts.YEP, tk.ptr), "contract")
         tk.wait()
+    lib.talshAmdDeviceSynchronize(0)
+    wall = time.perf_counter() - t0
+    mm, tot = [], []
+    for tk in tasks:
         tm = tk.times()
         mm.append(tm["mmul"])
         tot.append(tm["total"])
         tk.destruct()
-    lib.talshAmdDeviceSynchronize(0)
-    wall = time.perf_counter() - t0
     return wall, mm, tot
 
 
