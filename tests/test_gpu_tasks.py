@@ -435,3 +435,28 @@ def test_repeated_shapes_in_flight(talsh_gpu, oracle, dk):
             assert relerr(snapshot(ts, ds[c]), want) <= tol, (pat, c)
         for t in ds + [lt, rt]:
             t.destruct()
+
+
+@pytest.mark.parametrize("dk", (R8, R4, C8))
+def test_norm2_of_views_at_every_alignment(talsh_gpu, oracle, dk):
+    """norm2 of views that start 0..3 elements into a GPU body (so 16-byte vector loads are and are not
+    possible) and end on odd lengths, against the oracle on the same elements."""
+    ts = talsh_gpu
+    n = 100003
+    data = tensor(dk, [n], 77)
+    base = on_gpu(ts, dk, [n], data)
+    addr = gpu_address(ts, base)
+    esz = {R8: 8, R4: 4, C8: 16}[dk]
+    tol = 1e-12 if dk in (R8, C8) else 1e-6
+    for start in range(4):
+        for length in (1, 5, 4099, n - start):
+            say("norm2 view", dk, start, length)
+            v = view(ts, dk, [length], addr + start * esz)
+            d = ts.Tensor(dk, [], ts.gpu_flat(0))
+            ts.check(ts.norm2(d, v), "norm2")
+            got = d.scalar().real
+            want = oracle.norm2(dk, data[start:start + length])
+            assert abs(got - want) <= tol * want, (start, length, got, want)
+            d.destruct()
+            v.destruct()
+    base.destruct()
