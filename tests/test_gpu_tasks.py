@@ -460,3 +460,45 @@ def test_norm2_of_views_at_every_alignment(talsh_gpu, oracle, dk):
             d.destruct()
             v.destruct()
     base.destruct()
+
+
+@pytest.mark.parametrize("dk", (R8, R4, C4))
+def test_permute_misaligned_views(talsh_gpu, oracle, dk):
+    """Permuted copies and adds between views that start 1 and 3 elements into GPU bodies (so the 16-byte
+    vector paths are not taken), random rank-4 permutations with odd and even extents: bit-exact against the
+    oracle."""
+    import random
+    ts = talsh_gpu
+    rng = random.Random(606 + dk)
+    esz = {R8: 8, R4: 4, C4: 8}[dk]
+    cap = 9 ** 4 + 8
+    lbase = on_gpu(ts, dk, [cap], tensor(dk, [cap], 61))
+    dbase = on_gpu(ts, dk, [cap], tensor(dk, [cap], 62))
+    la, da = gpu_address(ts, lbase), gpu_address(ts, dbase)
+    labels_ = list("abcd")
+    for t in range(12):
+        ld = [rng.randint(2, 9) for _ in range(4)]
+        perm = labels_[:]
+        rng.shuffle(perm)
+        o2n = [labels_.index(x) + 1 for x in perm]  # L dim k (label perm[k]) goes to D position o2n[k]
+        dd = [0] * 4
+        for k in range(4):
+            dd[o2n[k] - 1] = ld[k]
+        pat_copy = "D(a,b,c,d)=L(%s)" % ",".join(perm)
+        pat_add = "D(a,b,c,d)+=L(%s)" % ",".join(perm)
+        vol = int(np.prod(ld))
+        L = snapshot(ts, lbase)[1:1 + vol]
+        lv = view(ts, dk, ld, la + 1 * esz)
+        dv = view(ts, dk, dd, da + 3 * esz)
+        say("misaligned", dk, pat_copy, ld)
+        ts.check(ts.copy(pat_copy, dv, lv), "copy")
+        got = snapshot(ts, dv)
+        assert got.tobytes() == oracle.copy(dk, ld, o2n, L).tobytes(), (pat_copy, ld)
+        alpha = complex(0.5, -0.25) if dk == C4 else 0.5
+        ts.check(ts.add(pat_add, dv, lv, alpha=alpha), "add")
+        want = oracle.add(dk, ld, o2n, L, got, (alpha.real, alpha.imag) if dk == C4 else (alpha, 0.0))
+        assert snapshot(ts, dv).tobytes() == want.tobytes(), (pat_add, ld)
+        lv.destruct()
+        dv.destruct()
+    lbase.destruct()
+    dbase.destruct()
