@@ -502,3 +502,39 @@ def test_permute_misaligned_views(talsh_gpu, oracle, dk):
         dv.destruct()
     lbase.destruct()
     dbase.destruct()
+
+
+@pytest.mark.parametrize("dk", (R8, R4, C8))
+def test_contract_misaligned_views(talsh_gpu, oracle, dk):
+    """Contractions whose L, R and D are views 1, 2 and 3 elements into GPU bodies (no 16-byte vector
+    accesses anywhere), the configs' interleaved patterns and random ones, against the oracle."""
+    import random
+    from tal_common import random_contraction
+    ts = talsh_gpu
+    rng = random.Random(707 + dk)
+    esz = {R8: 8, R4: 4, C8: 16}[dk]
+    cap = 40 ** 4 + 8
+    bases = [on_gpu(ts, dk, [cap], tensor(dk, [cap], 81 + i)) for i in range(3)]
+    addrs = [gpu_address(ts, b) for b in bases]
+    cases = [("D(a,b,c,d)+=L(a,i,c,j)*R(j,b,i,d)", [20, 18, 22, 17], [20, 9, 22, 7], [7, 18, 9, 17]),
+             ("D(a,b,c,d)+=L(c,i,b,j)*R(d,j,a,i)", [17, 19, 18, 21], [18, 5, 19, 7], [21, 7, 17, 5])]
+    for _ in range(6):
+        cases.append(random_contraction(rng, max_rank=4, ext_lo=2, ext_hi=17))
+    tol = 1e-12 if dk in (R8, C8) else 1e-5
+    for pat, dd, ld, rd in cases:
+        say("misaligned contract", dk, pat, dd, ld, rd)
+        host = [snapshot(ts, b) for b in bases]
+        L = host[0][1:1 + int(np.prod(ld))]
+        R = host[1][2:2 + int(np.prod(rd))]
+        D0 = host[2][3:3 + int(np.prod(dd))]
+        lv, rv = view(ts, dk, ld, addrs[0] + esz), view(ts, dk, rd, addrs[1] + 2 * esz)
+        dv = view(ts, dk, dd, addrs[2] + 3 * esz)
+        alpha = complex(0.75, 0.5) if dk == C8 else 0.75
+        ts.check(ts.contract(pat, dv, lv, rv, alpha=alpha), "contract")
+        want = oracle.contract(dk, pat, dd, D0, ld, L, rd, R, (alpha.real, alpha.imag) if dk == C8 else
+                               (alpha, 0.0), 1)
+        assert relerr(snapshot(ts, dv), want) <= tol, (pat, dd)
+        for v in (lv, rv, dv):
+            v.destruct()
+    for b in bases:
+        b.destruct()
