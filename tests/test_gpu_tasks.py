@@ -538,3 +538,33 @@ def test_contract_misaligned_views(talsh_gpu, oracle, dk):
             v.destruct()
     for b in bases:
         b.destruct()
+
+
+@pytest.mark.parametrize("dk", (R8, C4))
+def test_contract_high_rank(talsh_gpu, oracle, dk):
+    """Contractions of rank-6 to rank-9 operands with extents 2-3 (after fusion up to eight outer dimensions
+    per transpose, past what the paired transpose launch carries, so both the paired and the separate launches
+    run), against the oracle."""
+    import random
+    ts = talsh_gpu
+    rng = random.Random(808 + dk)
+    tol = 1e-12 if dk == R8 else 1e-5
+    for t in range(10):
+        nl, nr, nc = rng.randint(2, 5), rng.randint(2, 5), rng.randint(2, 4)
+        labels_ = [chr(ord("a") + i) for i in range(nl + nr + nc)]
+        ext = {x: rng.randint(2, 3) for x in labels_}
+        lu, ru, cc = labels_[:nl], labels_[nl:nl + nr], labels_[nl + nr:]
+        d, l, r = lu + ru, lu + cc, ru + cc
+        rng.shuffle(d)
+        rng.shuffle(l)
+        rng.shuffle(r)
+        pat = "D(%s)+=L(%s)*R(%s)" % (",".join(d), ",".join(l), ",".join(r))
+        dd, ld, rd = [ext[x] for x in d], [ext[x] for x in l], [ext[x] for x in r]
+        say("high rank", dk, pat)
+        L, R, D = tensor(dk, ld, 900 + t), tensor(dk, rd, 920 + t), tensor(dk, dd, 940 + t)
+        lt, rt, dt = on_gpu(ts, dk, ld, L), on_gpu(ts, dk, rd, R), on_gpu(ts, dk, dd, D)
+        ts.check(ts.contract(pat, dt, lt, rt, alpha=0.5), "contract")
+        want = oracle.contract(dk, pat, dd, D, ld, L, rd, R, (0.5, 0.0), 1)
+        assert relerr(snapshot(ts, dt), want) <= tol, pat
+        for x in (lt, rt, dt):
+            x.destruct()
