@@ -17,8 +17,11 @@ or at an overlapping address range.
 
 Also here: a test_talsh_qc-style pipeline (test.cpp:656-892) that keeps
 retrying TRY_LATER while earlier tasks complete, tensors destructed before
-their task is waited on, and the rank-0 / outer-product contractions that the
-reference runs with dedicated kernels (tensor_algebra_gpu_nvidia.cu:6732-6849).
+their task is waited on, the rank-0 / outer-product contractions that the
+reference runs with dedicated kernels (tensor_algebra_gpu_nvidia.cu:6732-6849),
+repeated shapes in flight (plans, paired transposes and D offset tables
+reused across streams), operands that are views at every alignment (the
+scalar kernel paths), and high-rank contractions.
 
 Bars: copies and initialisations bit-exact; R8/C8 contractions normwise
 relative error <= 1e-12 against the oracle; R4/C4 contractions elementwise
