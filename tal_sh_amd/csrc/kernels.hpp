@@ -40,6 +40,9 @@ enum PermOp { PERM_COPY = 0, PERM_ADD = 1 };
 // O2N permutation `o2n`.  Complex kinds may conjugate src.  Bit-exact.
 int launch_permute(int dk, const void *src, void *dst, int rank, const int *dims, const int *o2n, int op,
                    double ar, double ai, int conj, hipStream_t s);
+// Forgets launch_permute's per-shape kernel-variant choices and frees their
+// timing events (at shutdown).
+void perm_tune_clear();
 // Two independent permutation copies (dst_j = perm_j(src_j), arguments as
 // launch_permute's with op PERM_COPY; neither destination may overlap the
 // other copy's source or destination) in one launch when both take the same

@@ -501,6 +501,7 @@ int talshShutdown(void) {
   r.tens_tasks.clear();
   r.hazard_epoch++;
   ttgt_cache_clear();
+  perm_tune_clear();
   for (auto &d : r.gpus) {
     if (!d.on) continue;
     d.backlog.clear();

@@ -20,8 +20,9 @@ retrying TRY_LATER while earlier tasks complete, tensors destructed before
 their task is waited on, the rank-0 / outer-product contractions that the
 reference runs with dedicated kernels (tensor_algebra_gpu_nvidia.cu:6732-6849),
 repeated shapes in flight (plans, paired transposes and D offset tables
-reused across streams), operands that are views at every alignment (the
-scalar kernel paths), and high-rank contractions.
+reused across streams), permutation shapes repeated while their kernel
+variants are timed and one is chosen, operands that are views at every
+alignment (the scalar kernel paths), and high-rank contractions.
 
 Bars: copies and initialisations bit-exact; R8/C8 contractions normwise
 relative error <= 1e-12 against the oracle; R4/C4 contractions elementwise
@@ -505,6 +506,45 @@ def test_permute_misaligned_views(talsh_gpu, oracle, dk):
         dv.destruct()
     lbase.destruct()
     dbase.destruct()
+
+
+@pytest.mark.parametrize("dk,n", ((R8, 12), (R4, 12), (C8, 10)))
+def test_permute_repeated_shapes(talsh_gpu, oracle, dk, n):
+    """The first launches of a permutation shape of 8 MB or more time its kernel variants (tile dimensions,
+    block order) and later ones run the fastest: five copies of each rank-6 permutation (12-16 MB) into a
+    destination refilled with a sentinel before each, then four accumulating adds, every result bit-exact
+    against the oracle."""
+    import random
+    ts = talsh_gpu
+    rng = random.Random(707 + dk)
+    labels_ = list("abcdef")
+    ld = [n] * 6
+    L = tensor(dk, ld, 71)
+    lt = on_gpu(ts, dk, ld, L)
+    dt = on_gpu(ts, dk, ld, tensor(dk, ld, 72))
+    perms = [list("fcbead"), list("bafedc"), list("dbfcae")]
+    for _ in range(2):
+        p = labels_[:]
+        rng.shuffle(p)
+        perms.append(p)
+    for perm in perms:
+        o2n = [labels_.index(x) + 1 for x in perm]
+        pat_copy = "D(a,b,c,d,e,f)=L(%s)" % ",".join(perm)
+        want = oracle.copy(dk, ld, o2n, L)
+        for call in range(5):
+            ts.check(ts.init_value(dt, -7.0), "init")
+            say("repeated", dk, pat_copy, ld, "call", call)
+            ts.check(ts.copy(pat_copy, dt, lt), "copy")
+            assert snapshot(ts, dt).tobytes() == want.tobytes(), (pat_copy, call)
+        got = want
+        pat_add = "D(a,b,c,d,e,f)+=L(%s)" % ",".join(perm)
+        alpha = complex(0.5, -0.25) if dk == C8 else 0.75
+        for call in range(4):
+            ts.check(ts.add(pat_add, dt, lt, alpha=alpha), "add")
+            got = oracle.add(dk, ld, o2n, L, got, (alpha.real, alpha.imag) if dk == C8 else (alpha, 0.0))
+            assert snapshot(ts, dt).tobytes() == got.tobytes(), (pat_add, call)
+    lt.destruct()
+    dt.destruct()
 
 
 @pytest.mark.parametrize("dk", (R8, R4, C8))
