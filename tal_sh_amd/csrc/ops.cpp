@@ -392,14 +392,13 @@ std::vector<int> dims_of(const talsh_tens_t *t) {
 namespace {
 
 // Whether the GEMM writes a permuted D in place through a C map instead of
-// transposing D into a temporary and back: the leading dims of D' must
-// multiply to exactly M and the next ones to N (no batch), and D's stride-1
-// dim must lead D' with an extent of at least 16, so that the GEMM's and the
-// K-split summing pass's stores (runs along m) stay contiguous.  Sets the
-// number of D' dims in each group.
+// transposing D into a temporary and back (the planner's d_in_place: D's
+// stride-1 dim leads D' with an extent of at least 16, so that the GEMM's and
+// the K-split summing pass's stores, runs along m, stay contiguous; no batch),
+// and the leading dims of D' multiply to exactly M and the next ones to N.
+// Sets the number of D' dims in each group.
 bool d_in_place(const ContractPlan &plan, int *nm, int *nn) {
-  if (!plan.d_transpose || plan.batch != 1) return false;
-  if (plan.d_o2n.empty() || plan.d_o2n[0] != 0 || plan.dprime_dims[0] < 16) return false;
+  if (!plan.d_in_place || plan.d_o2n.empty() || plan.dprime_dims.empty()) return false;
   const int r = (int)plan.dprime_dims.size();
   if (r > kMaxMapDims) return false;
   int t = 0;

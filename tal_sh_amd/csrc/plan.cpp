@@ -318,14 +318,16 @@ int plan_uncached(const DigitalPattern &p, const int *ddims, const int *ldims, c
           c.batch = batch;
           c.a.ld = std::max<int64_t>(1, c.a.op == 'T' ? c.K : c.M);
           c.b.ld = std::max<int64_t>(1, c.b.op == 'N' ? c.K : c.N);
-          c.moved_elems = (c.a.transpose ? vx : 0) + (c.b.transpose ? vy : 0) +
-                          (c.d_transpose ? vd * (accumulative ? 2 : 1) : 0);
           if (c.d_transpose) {
             c.d_o2n = o2n_of(sd, dseq);
             c.d_back_o2n = o2n_of(dseq, sd);
             c.dprime_dims.resize(nd);
             for (int i = 0; i < nd; ++i) c.dprime_dims[i] = (int)ext_of[dseq[i]];
+            c.d_in_place = batch == 1 && c.d_o2n[0] == 0 && c.dprime_dims[0] >= 16;
           }
+          // a D transpose done in place by the GEMM moves nothing extra
+          c.moved_elems = (c.a.transpose ? vx : 0) + (c.b.transpose ? vy : 0) +
+                          (c.d_transpose && !c.d_in_place ? vd * (accumulative ? 2 : 1) : 0);
           const int ntrans = (int)c.a.transpose + (int)c.b.transpose + (int)c.d_transpose;
           // among equal plans, one whose D' starts with D's first (stride-1) index lets the GEMM
           // write D in place with m-contiguous stores

@@ -35,6 +35,10 @@ struct ContractPlan {
   int64_t batch = 1;        // product of the hyper extents: GEMMs of M x N x K, strided by M*K, K*N, M*N
   OperandPlan a, b;
   bool d_transpose = false;
+  // D' is written in place by the GEMM through per-row / per-column offsets
+  // instead of through a transposed temporary: D' starts with D's stride-1
+  // index, of extent >= 16 (stores run along m), and has no batch index
+  bool d_in_place = false;
   std::vector<int> d_o2n;        // D -> D' (0-based O2N)
   std::vector<int> d_back_o2n;   // D' -> D
   std::vector<int> dprime_dims;  // extents of D' in its own order
