@@ -286,6 +286,56 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2)
         }
     return;
   }
+  // Short-K GEMMs (and GEMMs into plain column-major C) are store-bound:
+  // there each 16 x 16 block of the wave's accumulators is turned round
+  // through a wave-private LDS patch (the operand tiles' space, free after
+  // the last barrier of the K loop) so that every store covers 16
+  // consecutive rows (m, C's and D's contiguous axis) of 4 columns instead of
+  // 4 rows, or 4 rows 4 apart, of 16 columns: runs of 128 bytes (FP64) rather
+  // than 32.  Rank-6 contractions with K = 24 writing D in place ran 10-12 ->
+  // 14-18 TFLOP/s, a 5136^3 GEMM 47.3 -> 50.0; long-K GEMMs writing D in
+  // place lost 0.6-1 % with it and keep the direct stores.  LDS accesses of
+  // one wave complete in order, so the patch needs no barrier between its
+  // writes and reads.
+  if (!cmap || K <= 2048) {
+    constexpr int TP = 17;  // patch column stride (scalars)
+    static_assert(sizeof(As) >= sizeof(S) * NW * NC * 16 * TP, "patches fit the A tiles' space");
+    S *patch = &As[0][0][0][0] + wave * NC * 16 * TP;
+    const int tl = lane & 15, tg = lane >> 4;
+#pragma unroll
+    for (int i = 0; i < SM; ++i)
+#pragma unroll
+      for (int j = 0; j < SN; ++j) {
+        S v[NC][4];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) patch[(c * 16 + (lane & 15)) * TP + MF::row(lane, r)] = acc[c][i][j][r];
+          __builtin_amdgcn_wave_barrier();
+#pragma unroll
+          for (int r = 0; r < 4; ++r) v[c][r] = patch[(c * 16 + 4 * r + tg) * TP + tl];
+          __builtin_amdgcn_wave_barrier();
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t m = m0 + wm * (BM / 2) + i * 16 + tl;
+          const int64_t n = n0 + wn * WBN + j * 16 + 4 * r + tg;
+          if (m < M && n < N) {
+            S *c = C + (cmap ? cmap[m] + cmap[M + n] : m + n * ldc) * NC;
+            if (NC == 1) {
+              const S x = alpha_r * v[0][r];
+              c[0] = beta_one ? c[0] + x : x;
+            } else {
+              const S xr = v[0][r], xi = v[NC - 1][r];
+              const S vr = alpha_r * xr - alpha_i * xi, vi = alpha_r * xi + alpha_i * xr;
+              c[0] = beta_one ? c[0] + vr : vr;
+              c[NC - 1] = beta_one ? c[NC - 1] + vi : vi;
+            }
+          }
+        }
+      }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < SM; ++i)
 #pragma unroll
