@@ -22,7 +22,8 @@ reference runs with dedicated kernels (tensor_algebra_gpu_nvidia.cu:6732-6849),
 repeated shapes in flight (plans, paired transposes and D offset tables
 reused across streams), permutation shapes repeated while their kernel
 variants are timed and one is chosen, operands that are views at every
-alignment (the scalar kernel paths), and high-rank contractions.
+alignment (the scalar kernel paths), GEMMs of at most 32 columns (the
+narrow-tile path) and high-rank contractions.
 
 Bars: copies and initialisations bit-exact; R8/C8 contractions normwise
 relative error <= 1e-12 against the oracle; R4/C4 contractions elementwise
@@ -581,6 +582,30 @@ def test_contract_misaligned_views(talsh_gpu, oracle, dk):
             v.destruct()
     for b in bases:
         b.destruct()
+
+
+@pytest.mark.parametrize("c", (24, 32, 7))
+def test_contract_skinny_r8(talsh_gpu, oracle, c):
+    """R8 contractions whose GEMM has at most 32 columns and more than 65,536 rows (the narrow-tile path, as
+    the reference list's N = 24 shapes take): D overwritten and then accumulated into, with D written in place
+    and through the plain layout, against the oracle (normwise relative error <= 1e-12)."""
+    ts = talsh_gpu
+    a, b, d = 256, 300, 40
+    cases = [("D(a,b,c)+=L(b,d,a)*R(d,c)", [a, b, c], [b, d, a], [d, c]),
+             ("D(a,b,c)+=L(a,b,d)*R(d,c)", [a, b, c], [a, b, d], [d, c]),
+             ("D(a,c,b)+=L(d,a,b)*R(c,d)", [a, c, b], [d, a, b], [c, d])]
+    for k, (pat, dd, ld, rd) in enumerate(cases):
+        L, R, D0 = tensor(R8, ld, 81 + k), tensor(R8, rd, 82 + k), tensor(R8, dd, 83 + k)
+        lt, rt, dt = on_gpu(ts, R8, ld, L), on_gpu(ts, R8, rd, R), on_gpu(ts, R8, dd, D0)
+        say("skinny", pat, dd, ld, rd)
+        ts.check(ts.contract(pat, dt, lt, rt, alpha=0.5, accumulative=ts.NOPE), "overwrite")
+        want = oracle.contract(R8, pat, dd, np.zeros_like(D0), ld, L, rd, R, (0.5, 0.0), 0)
+        assert relerr(snapshot(ts, dt), want) <= 1e-12, (pat, c)
+        ts.check(ts.contract(pat, dt, lt, rt, alpha=-1.5), "accumulate")
+        want = oracle.contract(R8, pat, dd, want, ld, L, rd, R, (-1.5, 0.0), 1)
+        assert relerr(snapshot(ts, dt), want) <= 1e-12, (pat, c)
+        for t in (lt, rt, dt):
+            t.destruct()
 
 
 @pytest.mark.parametrize("dk", (R8, C4))
