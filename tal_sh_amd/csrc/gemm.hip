@@ -1350,12 +1350,13 @@ bool small_real(int64_t M, int64_t N, int64_t batch) {
   return t128 < gemm_slots<S, 1, 128, 128, kWavesSingle, sizeof(S) == 4 ? 2 * BK : BK>();
 }
 
-// Real FP64 GEMMs with at most 32 columns and many 128-row tiles (the
-// N = 24 contractions of the reference's benchmark list) are latency-bound
+// Real GEMMs with at most 32 columns and many 128-row tiles (the N = 24
+// contractions of the reference's benchmark list) are latency-bound
 // on their A tile loads: 128 x 32 tiles of 4 waves (64 x 16 of C each) keep
 // more workgroups, and so more of A, in flight per CU than 128 x 128 tiles
 // of which three quarters of the columns are empty.  Those GEMMs ran 6-10 ->
-// 16-28 TFLOP/s (K = 48-312, M = 97,344-5,308,416).
+// 16-28 TFLOP/s in FP64 and 9-20 -> 20-49 in FP32 (K = 48-312, M = 97,344
+// to 5,308,416).
 bool skinny_real(int64_t M, int64_t N, int64_t batch) { return batch == 1 && N <= 32 && M >= 128 * 512; }
 
 size_t gemm_tail_workspace(int dk, int64_t M, int64_t N, int64_t K, int64_t batch) {
@@ -1367,6 +1368,8 @@ size_t gemm_tail_workspace(int dk, int64_t M, int64_t N, int64_t K, int64_t batc
                  ? tail_bytes<double, 1, 64, 64>(tail_split<double, 1, 64, 64, kWavesSingle, 2 * BK>(M, N, K, batch))
                  : tail_bytes<double, 1, 128, 128>(tail_split<double, 1, 128, 128, kWavesSingle, BK>(M, N, K, batch));
     case 4:
+      if (skinny_real(M, N, batch))
+        return tail_bytes<float, 1, 128, 32>(tail_split<float, 1, 128, 32, 4, 2 * BK>(M, N, K, batch));
       return small_real<float>(M, N, batch)
                  ? tail_bytes<float, 1, 64, 64>(tail_split<float, 1, 64, 64, 4, BK>(M, N, K, batch))
                  : tail_bytes<float, 1, 128, 128>(
@@ -1413,6 +1416,7 @@ int launch_gemm(int dk, int r4mode, char ta, char tb, int conja, int conjb, int6
       return small_real<double>(M, N, batch) ? TAL_GEMM_KIND(double, 1, 64, 64, kWavesSingle, 2 * BK)
                                              : TAL_GEMM_KIND(double, 1, 128, 128, kWavesSingle, BK);
     case 4:
+      if (skinny_real(M, N, batch)) return TAL_GEMM_KIND(float, 1, 128, 32, 4, 2 * BK);
       return small_real<float>(M, N, batch) ? TAL_GEMM_KIND(float, 1, 64, 64, 4, BK)
                                             : TAL_GEMM_KIND(float, 1, 128, 128, kWavesSingle, 2 * BK);
     case 18: return TAL_GEMM_KIND(double, 2, 64, 64, kWavesSingle, BK);

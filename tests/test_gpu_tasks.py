@@ -584,26 +584,39 @@ def test_contract_misaligned_views(talsh_gpu, oracle, dk):
         b.destruct()
 
 
-@pytest.mark.parametrize("c", (24, 32, 7))
-def test_contract_skinny_r8(talsh_gpu, oracle, c):
-    """R8 contractions whose GEMM has at most 32 columns and more than 65,536 rows (the narrow-tile path, as
+@pytest.mark.parametrize("dk,c", ((R8, 24), (R8, 32), (R8, 7), (R4, 24), (R4, 9)))
+def test_contract_skinny(talsh_gpu, oracle, dk, c):
+    """Real contractions whose GEMM has at most 32 columns and more than 65,536 rows (the narrow-tile path, as
     the reference list's N = 24 shapes take): D overwritten and then accumulated into, with D written in place
-    and through the plain layout, against the oracle (normwise relative error <= 1e-12)."""
+    and through the plain layout, against the oracle (R8: normwise relative error <= 1e-12; R4: elementwise
+    within 1e-6 * (|D| + |alpha| * sum |l| |r|))."""
     ts = talsh_gpu
     a, b, d = 256, 300, 40
     cases = [("D(a,b,c)+=L(b,d,a)*R(d,c)", [a, b, c], [b, d, a], [d, c]),
              ("D(a,b,c)+=L(a,b,d)*R(d,c)", [a, b, c], [a, b, d], [d, c]),
              ("D(a,c,b)+=L(d,a,b)*R(c,d)", [a, c, b], [d, a, b], [c, d])]
+
+    def check(got, want, prev, alpha, pat):
+        if dk == R8:
+            assert relerr(got, want) <= 1e-12, (pat, c)
+        else:
+            plain = pat.replace("+(", "(")
+            mag = oracle.contract(R8, plain, dd, np.zeros(int(np.prod(dd))), ld, np.abs(L).astype(np.float64), rd,
+                                  np.abs(R).astype(np.float64))
+            assert np.all(np.abs(got - want) <= 1e-6 * (abs(alpha) * mag + np.abs(prev))), (pat, c)
+
     for k, (pat, dd, ld, rd) in enumerate(cases):
-        L, R, D0 = tensor(R8, ld, 81 + k), tensor(R8, rd, 82 + k), tensor(R8, dd, 83 + k)
-        lt, rt, dt = on_gpu(ts, R8, ld, L), on_gpu(ts, R8, rd, R), on_gpu(ts, R8, dd, D0)
-        say("skinny", pat, dd, ld, rd)
+        L, R, D0 = tensor(dk, ld, 81 + k), tensor(dk, rd, 82 + k), tensor(dk, dd, 83 + k)
+        lt, rt, dt = on_gpu(ts, dk, ld, L), on_gpu(ts, dk, rd, R), on_gpu(ts, dk, dd, D0)
+        say("skinny", dk, pat, dd, ld, rd)
         ts.check(ts.contract(pat, dt, lt, rt, alpha=0.5, accumulative=ts.NOPE), "overwrite")
-        want = oracle.contract(R8, pat, dd, np.zeros_like(D0), ld, L, rd, R, (0.5, 0.0), 0)
-        assert relerr(snapshot(ts, dt), want) <= 1e-12, (pat, c)
+        zero = np.zeros_like(D0)
+        want = oracle.contract(dk, pat, dd, zero, ld, L, rd, R, (0.5, 0.0), 0)
+        check(snapshot(ts, dt), want, zero, 0.5, pat)
         ts.check(ts.contract(pat, dt, lt, rt, alpha=-1.5), "accumulate")
-        want = oracle.contract(R8, pat, dd, want, ld, L, rd, R, (-1.5, 0.0), 1)
-        assert relerr(snapshot(ts, dt), want) <= 1e-12, (pat, c)
+        prev = want
+        want = oracle.contract(dk, pat, dd, prev, ld, L, rd, R, (-1.5, 0.0), 1)
+        check(snapshot(ts, dt), want, prev, -1.5, pat)
         for t in (lt, rt, dt):
             t.destruct()
 
