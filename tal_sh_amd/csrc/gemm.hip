@@ -293,7 +293,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2)
   // consecutive rows (m, C's and D's contiguous axis) of 4 columns instead of
   // 4 rows, or 4 rows 4 apart, of 16 columns: runs of 128 bytes (FP64) rather
   // than 32.  Rank-6 contractions with K = 24 writing D in place ran 10-12 ->
-  // 14-18 TFLOP/s, a 5136^3 GEMM 47.3 -> 50.0; long-K GEMMs writing D in
+  // 14-18 TFLOP/s, and 18.5-23 with the offsets loaded ahead of the stores;
+  // a 5136^3 GEMM 47.3 -> 50.0; long-K GEMMs writing D in
   // place lost 0.6-1 % with it and keep the direct stores.  LDS accesses of
   // one wave complete in order, so the patch needs no barrier between its
   // writes and reads.
@@ -302,6 +303,21 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2)
     static_assert(sizeof(As) >= sizeof(S) * NW * NC * 16 * TP, "patches fit the A tiles' space");
     S *patch = &As[0][0][0][0] + wave * NC * 16 * TP;
     const int tl = lane & 15, tg = lane >> 4;
+    // the lane's row and column offsets into C, all loaded before the first
+    // store (one table latency instead of one per store)
+    int64_t roff[SM], coff[SN][4];
+#pragma unroll
+    for (int i = 0; i < SM; ++i) {
+      const int64_t m = m0 + wm * (BM / 2) + i * 16 + tl;
+      roff[i] = m < M ? (cmap ? cmap[m] : m) : -1;
+    }
+#pragma unroll
+    for (int j = 0; j < SN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t n = n0 + wn * WBN + j * 16 + 4 * r + tg;
+        coff[j][r] = n < N ? (cmap ? cmap[M + n] : n * ldc) : -1;
+      }
 #pragma unroll
     for (int i = 0; i < SM; ++i)
 #pragma unroll
@@ -318,10 +334,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2)
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int64_t m = m0 + wm * (BM / 2) + i * 16 + tl;
-          const int64_t n = n0 + wn * WBN + j * 16 + 4 * r + tg;
-          if (m < M && n < N) {
-            S *c = C + (cmap ? cmap[m] + cmap[M + n] : m + n * ldc) * NC;
+          if (roff[i] >= 0 && coff[j][r] >= 0) {
+            S *c = C + (roff[i] + coff[j][r]) * NC;
             if (NC == 1) {
               const S x = alpha_r * v[0][r];
               c[0] = beta_one ? c[0] + x : x;
