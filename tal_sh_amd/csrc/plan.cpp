@@ -316,6 +316,23 @@ int plan_uncached(const DigitalPattern &p, const int *ddims, const int *ldims, c
           c.K = 1;
           for (int s : K) c.K *= ext_of[s];
           c.batch = batch;
+          // the GEMM stages K-contiguous operand tiles into LDS without
+          // scattering: for long-K products an M-contiguous A ('N') or an
+          // N-contiguous B ('T') costs more than transposing that operand
+          // once the other GEMM dimension is large (5136^3 matrix product:
+          // 48.5 TFLOP/s on the operands as given, 58.5 with L transposed)
+          if (batch == 1 && c.K >= 1024) {
+            if (!c.a.transpose && c.a.op == 'N' && c.N >= 1024) {
+              c.a.transpose = true;
+              c.a.op = 'T';
+              c.a.o2n = o2n_of(sx, xt);
+            }
+            if (!c.b.transpose && c.b.op == 'T' && c.M >= 1024) {
+              c.b.transpose = true;
+              c.b.op = 'N';
+              c.b.o2n = o2n_of(sy, yn);
+            }
+          }
           c.a.ld = std::max<int64_t>(1, c.a.op == 'T' ? c.K : c.M);
           c.b.ld = std::max<int64_t>(1, c.b.op == 'N' ? c.K : c.N);
           if (c.d_transpose) {
