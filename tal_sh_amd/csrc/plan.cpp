@@ -320,14 +320,17 @@ int plan_uncached(const DigitalPattern &p, const int *ddims, const int *ldims, c
           // scattering: for long-K products an M-contiguous A ('N') or an
           // N-contiguous B ('T') costs more than transposing that operand
           // once the other GEMM dimension is large (5136^3 matrix product:
-          // 48.5 TFLOP/s on the operands as given, 58.5 with L transposed)
+          // 48.5 TFLOP/s on the operands as given, 58.5 with L transposed).
+          // Operands above 2^28 elements are left as given: their temporary
+          // would take a large share of the device pool.
+          constexpr double kMaxForcedCopy = 268435456.0;
           if (batch == 1 && c.K >= 1024) {
-            if (!c.a.transpose && c.a.op == 'N' && c.N >= 1024) {
+            if (!c.a.transpose && c.a.op == 'N' && c.N >= 1024 && vx <= kMaxForcedCopy) {
               c.a.transpose = true;
               c.a.op = 'T';
               c.a.o2n = o2n_of(sx, xt);
             }
-            if (!c.b.transpose && c.b.op == 'T' && c.M >= 1024) {
+            if (!c.b.transpose && c.b.op == 'T' && c.M >= 1024 && vy <= kMaxForcedCopy) {
               c.b.transpose = true;
               c.b.op = 'N';
               c.b.o2n = o2n_of(sy, yn);
