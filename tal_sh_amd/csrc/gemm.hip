@@ -1372,8 +1372,11 @@ bool small_real(int64_t M, int64_t N, int64_t batch) {
 // more workgroups, and so more of A, in flight per CU than 128 x 128 tiles
 // of which three quarters of the columns are empty.  Those GEMMs ran 6-10 ->
 // 16-28 TFLOP/s in FP64 and 9-20 -> 20-49 in FP32 (K = 48-312, M = 97,344
-// to 5,308,416).
-bool skinny_real(int64_t M, int64_t N, int64_t batch) { return batch == 1 && N <= 32 && M >= 128 * 512; }
+// to 5,308,416); up to 96 columns, three narrow tiles still beat one wide
+// one from 4,096 rows (5184 x 72 x 5184: 12.8 -> 19.2 TFLOP/s per step).
+bool skinny_real(int64_t M, int64_t N, int64_t batch) {
+  return batch == 1 && ((N <= 32 && M >= 128 * 512) || (N <= 96 && M >= 4096));
+}
 
 size_t gemm_tail_workspace(int dk, int64_t M, int64_t N, int64_t K, int64_t batch) {
   switch (dk) {

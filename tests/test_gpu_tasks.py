@@ -584,10 +584,10 @@ def test_contract_misaligned_views(talsh_gpu, oracle, dk):
         b.destruct()
 
 
-@pytest.mark.parametrize("dk,c", ((R8, 24), (R8, 32), (R8, 7), (R4, 24), (R4, 9)))
+@pytest.mark.parametrize("dk,c", ((R8, 24), (R8, 32), (R8, 7), (R8, 72), (R4, 24), (R4, 9), (R4, 96)))
 def test_contract_skinny(talsh_gpu, oracle, dk, c):
-    """Real contractions whose GEMM has at most 32 columns and more than 65,536 rows (the narrow-tile path, as
-    the reference list's N = 24 shapes take): D overwritten and then accumulated into, with D written in place
+    """Real contractions whose GEMM has at most 96 columns and more than 65,536 rows (the narrow-tile path, as
+    the reference list's N = 24 and N = 72 shapes take): D overwritten and then accumulated into, with D written in place
     and through the plain layout, against the oracle (R8: normwise relative error <= 1e-12; R4: elementwise
     within 1e-6 * (|D| + |alpha| * sum |l| |r|))."""
     ts = talsh_gpu
